@@ -1,0 +1,753 @@
+// C ABI of the MI355X explicit SMA step: handle management, colouring, the
+// per-step launch sequence (captured into hipGraphs) and the point-wise law.
+// Declarations and the reference procedures each entry point replaces are in
+// include/alya_sma.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "alya_sma.h"
+#include "sld_kernels.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(x)                                                                \
+  do {                                                                           \
+    hipError_t e_ = (x);                                                         \
+    if (e_ != hipSuccess)                                                        \
+      return fail(SMA_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_));  \
+  } while (0)
+
+template <typename T>
+int dalloc(T** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  HIPCHK(hipMalloc((void**)p, count * sizeof(T)));
+  return 0;
+}
+
+// Synchronous upload (set-up only): the host buffer may go away on return.
+template <typename T>
+int upload(T** p, const T* host, size_t count, hipStream_t) {
+  int rc = dalloc(p, count);
+  if (rc) return rc;
+  if (count) HIPCHK(hipMemcpy(*p, host, count * sizeof(T), hipMemcpyHostToDevice));
+  return 0;
+}
+
+unsigned nblocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+}  // namespace
+
+struct GraphKey {
+  double dt, phi;
+  int scheme, mode, store, epb, xcd, tn;
+  bool operator<(const GraphKey& o) const {
+    return std::tie(dt, phi, scheme, mode, store, epb, xcd, tn) <
+           std::tie(o.dt, o.phi, o.scheme, o.mode, o.store, o.epb, o.xcd, o.tn);
+  }
+};
+
+struct sld_dev {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int64_t npoin = 0, nelem = 0;
+  int32_t ncolor = 0;
+  std::vector<int32_t> color_ptr;   // [ncolor+1]
+  std::vector<int32_t> order;       // processed position -> caller element
+  // device arrays
+  int32_t* lnods = nullptr;
+  double* coord[3] = {};
+  double* u[3] = {};
+  double* v[3] = {};
+  double* a[3] = {};
+  double* fint64[3] = {};
+  float* fint32[3] = {};
+  double* mass = nullptr;
+  double* rmass = nullptr;
+  uint8_t* fixmask = nullptr;
+  int32_t* func_of_node = nullptr;
+  double* func_table = nullptr;
+  int32_t* func_npts = nullptr;
+  int32_t* func_smooth = nullptr;
+  int32_t nfunc = 0, func_maxpts = 0;
+  double* fext0[3] = {};
+  int32_t fext_func = -1;
+  double* props = nullptr;
+  int32_t nmat = 0;
+  double* rho = nullptr;
+  int32_t* mat_of_elem = nullptr;
+  double* E_of_elem = nullptr;
+  double* sdv[2][3] = {};
+  int cur = 1;                       // index of the TIME_N history slot
+  double* S_out = nullptr;
+  uint32_t* flags_out = nullptr;
+  unsigned long long* counters = nullptr;
+  double* clock = nullptr;
+  int mode = SLD_MODE_FP64, store = 0, epb = 32, xcd = 1, use_graph = 1;
+  double dtcrit = 0.0;
+  std::map<GraphKey, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
+  std::vector<void*> allocs;
+  void* sur = nullptr;               // attached surrogate (mode 2)
+};
+
+namespace {
+
+template <typename T>
+int track(sld_dev* d, T* p) {
+  d->allocs.push_back((void*)p);
+  return 0;
+}
+
+sld::NodalArgs nodal_args(sld_dev* d) {
+  sld::NodalArgs A{};
+  A.npoin = d->npoin;
+  const bool f32 = d->mode != SLD_MODE_FP64;
+  for (int k = 0; k < 3; ++k) {
+    A.u[k] = d->u[k];
+    A.v[k] = d->v[k];
+    A.a[k] = d->a[k];
+    A.fint[k] = f32 ? (void*)d->fint32[k] : (void*)d->fint64[k];
+    A.fext0[k] = d->fext0[k];
+  }
+  A.fint_fp32 = f32 ? 1 : 0;
+  A.rmass = d->rmass;
+  A.fixmask = d->fixmask;
+  A.func_of_node = d->func_of_node;
+  A.func_table = d->func_table;
+  A.func_npts = d->func_npts;
+  A.func_smooth = d->func_smooth;
+  A.func_maxpts = d->func_maxpts;
+  A.fext_func = d->fext_func;
+  A.clock = d->clock;
+  return A;
+}
+
+template <typename R, int EPB, bool STORE>
+void launch_element(const sld::ElemArgs& A, hipStream_t s) {
+  const int ne = A.e_end - A.e_begin;
+  if (ne <= 0) return;
+  hipLaunchKernelGGL((sld::sld_element<R, EPB, STORE>), dim3(nblocks(ne, EPB)), dim3(EPB * 8), 0,
+                     s, A);
+}
+
+template <typename R, bool STORE>
+void launch_element_epb(int epb, const sld::ElemArgs& A, hipStream_t s) {
+  if (epb == 8) launch_element<R, 8, STORE>(A, s);
+  else if (epb == 16) launch_element<R, 16, STORE>(A, s);
+  else launch_element<R, 32, STORE>(A, s);
+}
+
+bool surrogate_attached(const sld_dev* d) { return d->sur != nullptr; }
+int surrogate_enqueue_elements(sld_dev*, double, int) {
+  return fail(SMA_ERR_STATE, "surrogate mode is not available in this build");
+}
+void surrogate_detach(sld_dev* d) { d->sur = nullptr; }
+
+// One explicit step; tn = history slot holding TIME_N.
+int enqueue_step(sld_dev* d, double dt, int scheme, double phi, int tn) {
+  hipStream_t s = d->stream;
+  sld::NodalArgs N = nodal_args(d);
+  const unsigned nb = nblocks(d->npoin, 256);
+  if (scheme == 0) hipLaunchKernelGGL((sld::sld_predict<0>), dim3(nb), dim3(256), 0, s, N, dt, phi);
+  else hipLaunchKernelGGL((sld::sld_predict<1>), dim3(nb), dim3(256), 0, s, N, dt, phi);
+  hipLaunchKernelGGL(sld::sld_clock, dim3(1), dim3(64), 0, s, d->clock, dt);
+
+  if (d->mode == SLD_MODE_SURROGATE) {
+    int rc = surrogate_enqueue_elements(d, dt, tn);
+    if (rc) return rc;
+  } else {
+    sld::ElemArgs E{};
+    E.nelem = d->nelem;
+    E.lnods = d->lnods;
+    for (int k = 0; k < 3; ++k) {
+      E.coord[k] = d->coord[k];
+      E.u[k] = d->u[k];
+      E.sdv_n[k] = d->sdv[tn][k];
+      E.sdv_k[k] = d->sdv[1 - tn][k];
+      E.fint[k] = N.fint[k];
+    }
+    E.props = d->props;
+    E.mat_of_elem = d->mat_of_elem;
+    E.E_of_elem = d->E_of_elem;
+    E.S_out = d->S_out;
+    E.flags_out = d->flags_out;
+    E.counters = d->counters;
+    E.xcd_remap = d->xcd;
+    for (int c = 0; c < d->ncolor; ++c) {
+      E.e_begin = d->color_ptr[c];
+      E.e_end = d->color_ptr[c + 1];
+      const bool st = d->store != 0;
+      if (d->mode == SLD_MODE_FP64) {
+        if (st) launch_element_epb<double, true>(d->epb, E, s);
+        else launch_element_epb<double, false>(d->epb, E, s);
+      } else {
+        if (st) launch_element_epb<float, true>(d->epb, E, s);
+        else launch_element_epb<float, false>(d->epb, E, s);
+      }
+    }
+  }
+  if (scheme == 0) hipLaunchKernelGGL((sld::sld_correct<0>), dim3(nb), dim3(256), 0, s, N, dt);
+  else hipLaunchKernelGGL((sld::sld_correct<1>), dim3(nb), dim3(256), 0, s, N, dt);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+void free_graphs(sld_dev* d) {
+  for (auto& kv : d->graphs) {
+    hipGraphExecDestroy(kv.second.second);
+    hipGraphDestroy(kv.second.first);
+  }
+  d->graphs.clear();
+}
+
+double host_critical_dt(const double* coord, const int32_t* lnods, int64_t nelem,
+                        const double* props, const double* rho, const int32_t* mat,
+                        const double* Eel) {
+  // sld_updtss.f90:119-162 with elmlen.f90 (ndime==3, hnatu = 2 for HEX08)
+  static const int sg[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
+                               {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
+  double dtmin = 1e300;
+  for (int64_t e = 0; e < nelem; ++e) {
+    double xm[3][3] = {{0}};
+    for (int k = 0; k < 8; ++k) {
+      const double* x = coord + 3 * (int64_t)lnods[8 * e + k];
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) xm[a][b] += x[a] * 0.125 * sg[k][b];
+    }
+    double t1 = xm[1][1] * xm[2][2] - xm[2][1] * xm[1][2];
+    double t2 = -xm[1][0] * xm[2][2] + xm[2][0] * xm[1][2];
+    double t3 = xm[1][0] * xm[2][1] - xm[2][0] * xm[1][1];
+    double det = xm[0][0] * t1 + xm[0][1] * t2 + xm[0][2] * t3;
+    double dn = 1.0 / det, tr[3][3];
+    tr[0][0] = t1 * dn; tr[1][0] = t2 * dn; tr[2][0] = t3 * dn;
+    tr[1][1] = (xm[0][0] * xm[2][2] - xm[2][0] * xm[0][2]) * dn;
+    tr[2][1] = (-xm[0][0] * xm[2][1] + xm[0][1] * xm[2][0]) * dn;
+    tr[2][2] = (xm[0][0] * xm[1][1] - xm[1][0] * xm[0][1]) * dn;
+    tr[0][1] = (-xm[0][1] * xm[2][2] + xm[2][1] * xm[0][2]) * dn;
+    tr[0][2] = (xm[0][1] * xm[1][2] - xm[1][1] * xm[0][2]) * dn;
+    tr[1][2] = (-xm[0][0] * xm[1][2] + xm[1][0] * xm[0][2]) * dn;
+    double hmin = 1e300;
+    for (int i = 0; i < 3; ++i)
+      hmin = std::min(hmin, 2.0 / std::sqrt(tr[i][0] * tr[i][0] + tr[i][1] * tr[i][1] +
+                                            tr[i][2] * tr[i][2]));
+    const int m = mat ? mat[e] : 0;
+    const double E = Eel ? Eel[e] : props[18 * m];
+    const double nu = props[18 * m + 1];
+    const double lam = nu * E / ((1.0 + nu) * (1.0 - 2.0 * nu));
+    const double mu = E / (2.0 * (1.0 + nu));
+    const double c = std::sqrt((lam + 2.0 * mu) / rho[m]);
+    dtmin = std::min(dtmin, hmin / c);
+  }
+  return dtmin;
+}
+
+}  // namespace
+
+
+extern "C" {
+
+const char* sma_last_error(void) { return g_err.c_str(); }
+
+int32_t sld_color_elements(int64_t nelem, int64_t npoin, const int32_t* lnods, int32_t* color) {
+  if (nelem < 0 || npoin <= 0 || (!lnods && nelem) || (!color && nelem))
+    return fail(SMA_ERR_ARG, "sld_color_elements: bad arguments");
+  std::vector<uint64_t> used((size_t)npoin, 0);
+  int32_t ncolor = 0;
+  for (int64_t e = 0; e < nelem; ++e) {
+    uint64_t m = 0;
+    for (int a = 0; a < 8; ++a) {
+      const int32_t p = lnods[8 * e + a];
+      if (p < 0 || p >= npoin) return fail(SMA_ERR_ARG, "sld_color_elements: node index out of range");
+      m |= used[p];
+    }
+    if (m == ~0ull) return fail(SMA_ERR_ARG, "sld_color_elements: more than 64 colours needed");
+    const int c = __builtin_ctzll(~m);
+    color[e] = c;
+    ncolor = std::max(ncolor, c + 1);
+    for (int a = 0; a < 8; ++a) used[lnods[8 * e + a]] |= (1ull << c);
+  }
+  return ncolor;
+}
+
+int sld_mesh_box(int32_t nx, int32_t ny, int32_t nz, double lx, double ly, double lz,
+                 double* coord, int32_t* lnods) {
+  if (nx <= 0 || ny <= 0 || nz <= 0 || !coord || !lnods)
+    return fail(SMA_ERR_ARG, "sld_mesh_box: bad arguments");
+  const int64_t px = nx + 1, py = ny + 1, pz = nz + 1;
+  if (px * py * pz > INT32_MAX) return fail(SMA_ERR_ARG, "sld_mesh_box: too many nodes");
+  for (int64_t k = 0; k < pz; ++k)
+    for (int64_t j = 0; j < py; ++j)
+      for (int64_t i = 0; i < px; ++i) {
+        const int64_t p = i + px * (j + py * k);
+        coord[3 * p + 0] = lx * (double)i / nx;
+        coord[3 * p + 1] = ly * (double)j / ny;
+        coord[3 * p + 2] = lz * (double)k / nz;
+      }
+  for (int64_t k = 0; k < nz; ++k)
+    for (int64_t j = 0; j < ny; ++j)
+      for (int64_t i = 0; i < nx; ++i) {
+        const int64_t e = i + (int64_t)nx * (j + (int64_t)ny * k);
+        auto P = [&](int64_t a, int64_t b, int64_t c) { return (int32_t)(a + px * (b + py * c)); };
+        int32_t* L = lnods + 8 * e;
+        L[0] = P(i, j, k);         L[1] = P(i + 1, j, k);
+        L[2] = P(i + 1, j + 1, k); L[3] = P(i, j + 1, k);
+        L[4] = P(i, j, k + 1);     L[5] = P(i + 1, j, k + 1);
+        L[6] = P(i + 1, j + 1, k + 1); L[7] = P(i, j + 1, k + 1);
+      }
+  return SMA_OK;
+}
+
+int sld_dev_create(const sld_mesh_desc* mesh, const sld_material_desc* mat,
+                   const sld_bc_desc* bc, sld_dev** out) {
+  if (!mesh || !mat || !bc || !out) return fail(SMA_ERR_ARG, "sld_dev_create: null argument");
+  *out = nullptr;
+  const int64_t np = mesh->npoin, ne = mesh->nelem;
+  if (np <= 0 || ne <= 0 || !mesh->lnods || !mesh->coord)
+    return fail(SMA_ERR_ARG, "sld_dev_create: empty mesh");
+  if (ne > (int64_t)INT32_MAX / 8 || np > INT32_MAX)
+    return fail(SMA_ERR_ARG, "sld_dev_create: mesh too large for 32-bit indices");
+  if (mat->nmat <= 0 || !mat->props || !mat->rho)
+    return fail(SMA_ERR_ARG, "sld_dev_create: no material");
+  if (!bc->fixmask) return fail(SMA_ERR_ARG, "sld_dev_create: fixmask required");
+  if (mat->mat_of_elem)
+    for (int64_t e = 0; e < ne; ++e)
+      if (mat->mat_of_elem[e] < 0 || mat->mat_of_elem[e] >= mat->nmat)
+        return fail(SMA_ERR_ARG, "sld_dev_create: material index out of range");
+  if (bc->nfunc < 0 || (bc->nfunc > 0 && (!bc->func_table || !bc->func_npts || bc->func_maxpts < 1)))
+    return fail(SMA_ERR_ARG, "sld_dev_create: bad load-function tables");
+  for (int f = 0; f < bc->nfunc; ++f)
+    if (bc->func_npts[f] < 1 || bc->func_npts[f] > bc->func_maxpts)
+      return fail(SMA_ERR_ARG, "sld_dev_create: func_npts out of range");
+  if (bc->func_of_node)
+    for (int64_t p = 0; p < np; ++p)
+      if (bc->func_of_node[p] < -1 || bc->func_of_node[p] >= bc->nfunc)
+        return fail(SMA_ERR_ARG, "sld_dev_create: func_of_node out of range");
+  const int32_t fext_func = bc->fext0 ? bc->fext_func : -1;
+  if (fext_func < -1 || fext_func >= bc->nfunc)
+    return fail(SMA_ERR_ARG, "sld_dev_create: fext_func out of range");
+
+  // colouring and colour-sorted order
+  std::vector<int32_t> color((size_t)ne);
+  const int32_t ncolor = sld_color_elements(ne, np, mesh->lnods, color.data());
+  if (ncolor < 0) return ncolor;
+  std::vector<int64_t> referenced((size_t)np, 0);
+  for (int64_t i = 0; i < 8 * ne; ++i) referenced[mesh->lnods[i]] = 1;
+  for (int64_t p = 0; p < np; ++p)
+    if (!referenced[p]) return fail(SMA_ERR_ARG, "sld_dev_create: node not used by any element (zero mass)");
+
+  sld_dev* d = new sld_dev();
+  d->npoin = np;
+  d->nelem = ne;
+  d->ncolor = ncolor;
+  d->color_ptr.assign(ncolor + 1, 0);
+  for (int64_t e = 0; e < ne; ++e) d->color_ptr[color[e] + 1]++;
+  for (int c = 0; c < ncolor; ++c) d->color_ptr[c + 1] += d->color_ptr[c];
+  d->order.resize(ne);
+  {
+    std::vector<int32_t> fill(d->color_ptr.begin(), d->color_ptr.end() - 1);
+    for (int64_t e = 0; e < ne; ++e) d->order[fill[color[e]]++] = (int32_t)e;
+  }
+  auto bail = [&](int rc) {
+    sld_dev_destroy(d);
+    return rc;
+  };
+  if (hipGetDevice(&d->device) != hipSuccess) return bail(fail(SMA_ERR_HIP, "no HIP device"));
+  if (hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking) != hipSuccess)
+    return bail(fail(SMA_ERR_HIP, "hipStreamCreate failed"));
+  hipStream_t s = d->stream;
+
+  // host staging in processed (colour) order and SoA layouts
+  std::vector<int32_t> ln((size_t)ne * 8);
+  for (int64_t i = 0; i < ne; ++i)
+    std::memcpy(&ln[8 * i], mesh->lnods + 8 * (int64_t)d->order[i], 8 * sizeof(int32_t));
+  std::vector<double> soa((size_t)np);
+  int rc = 0;
+#define TRY(x) do { rc = (x); if (rc) return bail(rc); } while (0)
+  TRY(upload(&d->lnods, ln.data(), ln.size(), s)); track(d, d->lnods);
+  std::vector<std::vector<double>> stage;
+  for (int k = 0; k < 3; ++k) {
+    stage.emplace_back((size_t)np);
+    for (int64_t p = 0; p < np; ++p) stage.back()[p] = mesh->coord[3 * p + k];
+    TRY(upload(&d->coord[k], stage.back().data(), (size_t)np, s)); track(d, d->coord[k]);
+    TRY(dalloc(&d->u[k], np)); track(d, d->u[k]);
+    TRY(dalloc(&d->v[k], np)); track(d, d->v[k]);
+    TRY(dalloc(&d->a[k], np)); track(d, d->a[k]);
+    TRY(dalloc(&d->fint64[k], np)); track(d, d->fint64[k]);
+    TRY(dalloc(&d->fint32[k], np)); track(d, d->fint32[k]);
+    HIPCHK(hipMemsetAsync(d->u[k], 0, np * sizeof(double), s));
+    HIPCHK(hipMemsetAsync(d->v[k], 0, np * sizeof(double), s));
+    HIPCHK(hipMemsetAsync(d->a[k], 0, np * sizeof(double), s));
+    HIPCHK(hipMemsetAsync(d->fint64[k], 0, np * sizeof(double), s));
+    HIPCHK(hipMemsetAsync(d->fint32[k], 0, np * sizeof(float), s));
+    if (bc->fext0) {
+      stage.emplace_back((size_t)np);
+      for (int64_t p = 0; p < np; ++p) stage.back()[p] = bc->fext0[3 * p + k];
+      TRY(upload(&d->fext0[k], stage.back().data(), (size_t)np, s)); track(d, d->fext0[k]);
+    }
+  }
+  d->fext_func = fext_func;
+  TRY(upload(&d->fixmask, bc->fixmask, (size_t)np, s)); track(d, d->fixmask);
+  if (bc->func_of_node) { TRY(upload(&d->func_of_node, bc->func_of_node, (size_t)np, s)); track(d, d->func_of_node); }
+  d->nfunc = bc->nfunc;
+  d->func_maxpts = bc->func_maxpts;
+  if (bc->nfunc > 0) {
+    TRY(upload(&d->func_table, bc->func_table, (size_t)bc->nfunc * bc->func_maxpts * 4, s)); track(d, d->func_table);
+    TRY(upload(&d->func_npts, bc->func_npts, (size_t)bc->nfunc, s)); track(d, d->func_npts);
+    std::vector<int32_t> sm((size_t)bc->nfunc, 0);
+    if (bc->func_smooth) std::memcpy(sm.data(), bc->func_smooth, sm.size() * sizeof(int32_t));
+    stage.emplace_back(); // keep staging alive until the stream syncs
+    TRY(upload(&d->func_smooth, sm.data(), sm.size(), s)); track(d, d->func_smooth);
+    HIPCHK(hipStreamSynchronize(s));
+  }
+  d->nmat = mat->nmat;
+  TRY(upload(&d->props, mat->props, (size_t)mat->nmat * 18, s)); track(d, d->props);
+  TRY(upload(&d->rho, mat->rho, (size_t)mat->nmat, s)); track(d, d->rho);
+  std::vector<int32_t> me;
+  std::vector<double> Ee;
+  if (mat->mat_of_elem) {
+    me.resize(ne);
+    for (int64_t i = 0; i < ne; ++i) me[i] = mat->mat_of_elem[d->order[i]];
+    TRY(upload(&d->mat_of_elem, me.data(), me.size(), s)); track(d, d->mat_of_elem);
+  }
+  if (mat->E_of_elem) {
+    Ee.resize(ne);
+    for (int64_t i = 0; i < ne; ++i) Ee[i] = mat->E_of_elem[d->order[i]];
+    TRY(upload(&d->E_of_elem, Ee.data(), Ee.size(), s)); track(d, d->E_of_elem);
+  }
+  const int64_t ngp = ne * 8;
+  for (int sl = 0; sl < 2; ++sl)
+    for (int k = 0; k < 3; ++k) {
+      TRY(dalloc(&d->sdv[sl][k], ngp)); track(d, d->sdv[sl][k]);
+      HIPCHK(hipMemsetAsync(d->sdv[sl][k], 0, ngp * sizeof(double), s));
+    }
+  TRY(dalloc(&d->counters, 8)); track(d, d->counters);
+  HIPCHK(hipMemsetAsync(d->counters, 0, 8 * sizeof(unsigned long long), s));
+  TRY(dalloc(&d->clock, 1)); track(d, d->clock);
+  HIPCHK(hipMemsetAsync(d->clock, 0, sizeof(double), s));
+  TRY(dalloc(&d->mass, np)); track(d, d->mass);
+  TRY(dalloc(&d->rmass, np)); track(d, d->rmass);
+  HIPCHK(hipMemsetAsync(d->mass, 0, np * sizeof(double), s));
+
+  // lumped mass, coloured so the assembly is deterministic
+  for (int c = 0; c < ncolor; ++c) {
+    sld::MassArgs M{};
+    M.e_begin = d->color_ptr[c];
+    M.e_end = d->color_ptr[c + 1];
+    M.lnods = d->lnods;
+    for (int k = 0; k < 3; ++k) M.coord[k] = d->coord[k];
+    M.rho = d->rho;
+    M.mat_of_elem = d->mat_of_elem;
+    M.mass = d->mass;
+    const int n = M.e_end - M.e_begin;
+    if (n > 0) hipLaunchKernelGGL(sld::sld_mass, dim3(nblocks(n, 64)), dim3(64), 0, s, M);
+  }
+  hipLaunchKernelGGL(sld::sld_reciprocal, dim3(nblocks(np, 256)), dim3(256), 0, s, d->mass,
+                     d->rmass, np);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s));
+  d->dtcrit = host_critical_dt(mesh->coord, mesh->lnods, ne, mat->props, mat->rho,
+                               mat->mat_of_elem, mat->E_of_elem);
+#undef TRY
+  *out = d;
+  return SMA_OK;
+}
+
+int sld_dev_set_mode(sld_dev* d, int32_t mode) {
+  if (!d) return fail(SMA_ERR_ARG, "null handle");
+  if (mode < SLD_MODE_FP64 || mode > SLD_MODE_SURROGATE) return fail(SMA_ERR_ARG, "bad mode");
+  if (mode == SLD_MODE_SURROGATE && !surrogate_attached(d))
+    return fail(SMA_ERR_STATE, "surrogate mode needs sld_dev_set_surrogate first");
+  if (mode != d->mode) {
+    // carry the assembled force over to the precision the new mode reads
+    HIPCHK(hipStreamSynchronize(d->stream));
+    d->mode = mode;
+  }
+  return SMA_OK;
+}
+
+int sld_dev_set_store(sld_dev* d, int32_t store) {
+  if (!d) return fail(SMA_ERR_ARG, "null handle");
+  if (store && !d->S_out) {
+    const int64_t ngp = d->nelem * 8;
+    if (dalloc(&d->S_out, 9 * ngp)) return SMA_ERR_HIP;
+    track(d, d->S_out);
+    if (dalloc(&d->flags_out, ngp)) return SMA_ERR_HIP;
+    track(d, d->flags_out);
+    HIPCHK(hipMemsetAsync(d->S_out, 0, 9 * ngp * sizeof(double), d->stream));
+    HIPCHK(hipMemsetAsync(d->flags_out, 0, ngp * sizeof(uint32_t), d->stream));
+  }
+  d->store = store ? 1 : 0;
+  return SMA_OK;
+}
+
+int sld_dev_set_tile(sld_dev* d, int32_t epb, int32_t xcd) {
+  if (!d) return fail(SMA_ERR_ARG, "null handle");
+  if (epb != 8 && epb != 16 && epb != 32) return fail(SMA_ERR_ARG, "elems_per_block must be 8, 16 or 32");
+  d->epb = epb;
+  d->xcd = xcd ? 1 : 0;
+  return SMA_OK;
+}
+
+int sld_dev_step(sld_dev* d, double dt, int32_t scheme, double phi, int64_t nsteps) {
+  if (!d) return fail(SMA_ERR_ARG, "null handle");
+  if (scheme != 0 && scheme != 1) return fail(SMA_ERR_ARG, "scheme must be 0 (CD) or 1 (TW)");
+  if (!(dt > 0.0)) return fail(SMA_ERR_ARG, "dt must be positive");
+  if (nsteps < 0) return fail(SMA_ERR_ARG, "nsteps must be >= 0");
+  if (d->mode == SLD_MODE_SURROGATE && !surrogate_attached(d))
+    return fail(SMA_ERR_STATE, "surrogate mode without a surrogate");
+  int64_t left = nsteps;
+  if (d->use_graph && left >= 2) {
+    GraphKey key{dt, phi, scheme, d->mode, d->store, d->epb, d->xcd, d->cur};
+    auto it = d->graphs.find(key);
+    if (it == d->graphs.end()) {
+      hipGraph_t g = nullptr;
+      hipGraphExec_t ge = nullptr;
+      HIPCHK(hipStreamBeginCapture(d->stream, hipStreamCaptureModeThreadLocal));
+      int rc = enqueue_step(d, dt, scheme, phi, d->cur);
+      if (!rc) rc = enqueue_step(d, dt, scheme, phi, 1 - d->cur);
+      hipError_t ec = hipStreamEndCapture(d->stream, &g);
+      if (rc) return rc;
+      if (ec != hipSuccess) return fail(SMA_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+      HIPCHK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+      it = d->graphs.emplace(key, std::make_pair(g, ge)).first;
+    }
+    for (; left >= 2; left -= 2) HIPCHK(hipGraphLaunch(it->second.second, d->stream));
+  }
+  for (; left > 0; --left) {
+    int rc = enqueue_step(d, dt, scheme, phi, d->cur);
+    if (rc) return rc;
+    d->cur = 1 - d->cur;
+  }
+  return SMA_OK;
+}
+
+double sld_dev_critical_dt(const sld_dev* d) { return d ? d->dtcrit : -1.0; }
+
+void* sld_dev_stream(sld_dev* d) { return d ? (void*)d->stream : nullptr; }
+
+int sld_dev_sync(sld_dev* d) {
+  if (!d) return fail(SMA_ERR_ARG, "null handle");
+  HIPCHK(hipStreamSynchronize(d->stream));
+  return SMA_OK;
+}
+
+int sld_dev_coloring(const sld_dev* d, int32_t* ncolor, int32_t* color_ptr, int32_t max_colors,
+                     int32_t* elem_order) {
+  if (!d || !ncolor) return fail(SMA_ERR_ARG, "null argument");
+  *ncolor = d->ncolor;
+  if (color_ptr) {
+    if (max_colors < d->ncolor) return fail(SMA_ERR_ARG, "color_ptr too small");
+    std::memcpy(color_ptr, d->color_ptr.data(), (d->ncolor + 1) * sizeof(int32_t));
+  }
+  if (elem_order) std::memcpy(elem_order, d->order.data(), d->nelem * sizeof(int32_t));
+  return SMA_OK;
+}
+
+int sld_dev_get(sld_dev* d, int32_t field, void* host, int64_t count) {
+  if (!d || !host) return fail(SMA_ERR_ARG, "null argument");
+  HIPCHK(hipStreamSynchronize(d->stream));
+  const int64_t np = d->npoin, ne = d->nelem, ngp = ne * 8;
+  auto need = [&](int64_t n) { return count == n ? 0 : fail(SMA_ERR_ARG, "count does not match field size"); };
+  std::vector<double> tmp;
+  switch (field) {
+    case SLD_FIELD_U: case SLD_FIELD_V: case SLD_FIELD_A: case SLD_FIELD_FINT: {
+      if (need(3 * np)) return SMA_ERR_ARG;
+      double* out = (double*)host;
+      tmp.resize(np);
+      for (int k = 0; k < 3; ++k) {
+        if (field == SLD_FIELD_FINT && d->mode != SLD_MODE_FP64) {
+          std::vector<float> t32(np);
+          HIPCHK(hipMemcpy(t32.data(), d->fint32[k], np * sizeof(float), hipMemcpyDeviceToHost));
+          for (int64_t p = 0; p < np; ++p) tmp[p] = t32[p];
+        } else {
+          double* src = field == SLD_FIELD_U ? d->u[k] : field == SLD_FIELD_V ? d->v[k]
+                      : field == SLD_FIELD_A ? d->a[k] : d->fint64[k];
+          HIPCHK(hipMemcpy(tmp.data(), src, np * sizeof(double), hipMemcpyDeviceToHost));
+        }
+        for (int64_t p = 0; p < np; ++p) out[3 * p + k] = tmp[p];
+      }
+      return SMA_OK;
+    }
+    case SLD_FIELD_MASS:
+      if (need(np)) return SMA_ERR_ARG;
+      HIPCHK(hipMemcpy(host, d->mass, np * sizeof(double), hipMemcpyDeviceToHost));
+      return SMA_OK;
+    case SLD_FIELD_SDV: {
+      if (need(3 * ngp)) return SMA_ERR_ARG;
+      double* out = (double*)host;
+      const int k_slot = 1 - d->cur;  // ITER_K of the last step
+      tmp.resize(ngp);
+      for (int k = 0; k < 3; ++k) {
+        HIPCHK(hipMemcpy(tmp.data(), d->sdv[k_slot][k], ngp * sizeof(double), hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < ne; ++i)
+          for (int g = 0; g < 8; ++g) out[(8 * (int64_t)d->order[i] + g) * 3 + k] = tmp[8 * i + g];
+      }
+      return SMA_OK;
+    }
+    case SLD_FIELD_STRESS: {
+      if (!d->S_out) return fail(SMA_ERR_STATE, "stress not stored (sld_dev_set_store)");
+      if (need(9 * ngp)) return SMA_ERR_ARG;
+      double* out = (double*)host;
+      tmp.resize(9 * ngp);
+      HIPCHK(hipMemcpy(tmp.data(), d->S_out, 9 * ngp * sizeof(double), hipMemcpyDeviceToHost));
+      for (int k = 0; k < 9; ++k)
+        for (int64_t i = 0; i < ne; ++i)
+          for (int g = 0; g < 8; ++g)
+            out[(8 * (int64_t)d->order[i] + g) * 9 + k] = tmp[k * ngp + 8 * i + g];
+      return SMA_OK;
+    }
+    case SLD_FIELD_FLAGS: {
+      if (!d->flags_out) return fail(SMA_ERR_STATE, "flags not stored (sld_dev_set_store)");
+      if (need(ngp)) return SMA_ERR_ARG;
+      std::vector<uint32_t> t(ngp);
+      HIPCHK(hipMemcpy(t.data(), d->flags_out, ngp * sizeof(uint32_t), hipMemcpyDeviceToHost));
+      uint32_t* out = (uint32_t*)host;
+      for (int64_t i = 0; i < ne; ++i)
+        for (int g = 0; g < 8; ++g) out[8 * (int64_t)d->order[i] + g] = t[8 * i + g];
+      return SMA_OK;
+    }
+    case SLD_FIELD_TIME:
+      if (need(1)) return SMA_ERR_ARG;
+      HIPCHK(hipMemcpy(host, d->clock, sizeof(double), hipMemcpyDeviceToHost));
+      return SMA_OK;
+    case SLD_FIELD_COUNTERS:
+      if (need(8)) return SMA_ERR_ARG;
+      HIPCHK(hipMemcpy(host, d->counters, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+      return SMA_OK;
+    default:
+      return fail(SMA_ERR_ARG, "unknown field");
+  }
+}
+
+int sld_dev_set(sld_dev* d, int32_t field, const void* host, int64_t count) {
+  if (!d || !host) return fail(SMA_ERR_ARG, "null argument");
+  HIPCHK(hipStreamSynchronize(d->stream));
+  const int64_t np = d->npoin, ne = d->nelem, ngp = ne * 8;
+  std::vector<double> tmp;
+  switch (field) {
+    case SLD_FIELD_U: case SLD_FIELD_V: case SLD_FIELD_A: {
+      if (count != 3 * np) return fail(SMA_ERR_ARG, "count does not match field size");
+      const double* in = (const double*)host;
+      tmp.resize(np);
+      for (int k = 0; k < 3; ++k) {
+        for (int64_t p = 0; p < np; ++p) tmp[p] = in[3 * p + k];
+        double* dst = field == SLD_FIELD_U ? d->u[k] : field == SLD_FIELD_V ? d->v[k] : d->a[k];
+        HIPCHK(hipMemcpy(dst, tmp.data(), np * sizeof(double), hipMemcpyHostToDevice));
+      }
+      return SMA_OK;
+    }
+    case SLD_FIELD_SDV: {
+      if (count != 3 * ngp) return fail(SMA_ERR_ARG, "count does not match field size");
+      const double* in = (const double*)host;
+      tmp.resize(ngp);
+      for (int k = 0; k < 3; ++k) {
+        for (int64_t i = 0; i < ne; ++i)
+          for (int g = 0; g < 8; ++g) tmp[8 * i + g] = in[(8 * (int64_t)d->order[i] + g) * 3 + k];
+        for (int sl = 0; sl < 2; ++sl)
+          HIPCHK(hipMemcpy(d->sdv[sl][k], tmp.data(), ngp * sizeof(double), hipMemcpyHostToDevice));
+      }
+      return SMA_OK;
+    }
+    case SLD_FIELD_TIME:
+      if (count != 1) return fail(SMA_ERR_ARG, "count must be 1");
+      HIPCHK(hipMemcpy(d->clock, host, sizeof(double), hipMemcpyHostToDevice));
+      return SMA_OK;
+    default:
+      return fail(SMA_ERR_ARG, "field cannot be set");
+  }
+}
+
+int sld_dev_destroy(sld_dev* d) {
+  if (!d) return SMA_OK;
+  if (d->stream) hipStreamSynchronize(d->stream);
+  free_graphs(d);
+  surrogate_detach(d);
+  for (void* p : d->allocs) hipFree(p);
+  if (d->stream) hipStreamDestroy(d->stream);
+  delete d;
+  return SMA_OK;
+}
+
+static int law_launch(int64_t n, int32_t nmat, const double* props, const int32_t* mat, const double* E,
+                      const double* F, const double* sdv_old, double* sdv_new, double* S,
+                      uint32_t* flags, int32_t precision, hipStream_t s) {
+  sld::LawArgs A{n, nmat, props, mat, E, F, sdv_old, sdv_new, S, flags};
+  if (precision == 0)
+    hipLaunchKernelGGL((sld::sma160_points<false>), dim3(nblocks(n, 256)), dim3(256), 0, s, A);
+  else
+    hipLaunchKernelGGL((sld::sma160_points<true>), dim3(nblocks(n, 256)), dim3(256), 0, s, A);
+  HIPCHK(hipGetLastError());
+  return SMA_OK;
+}
+
+int sma160_eval_device(int64_t n, int32_t n_mat, const double* props, const int32_t* mat_of_gp,
+                       const double* E_of_gp, const double* F, const double* T,
+                       const double* sdv_old, double* sdv_new, double* S, uint32_t* flags,
+                       int32_t precision, void* stream) {
+  (void)T;
+  if (n < 0 || n_mat <= 0 || (n > 0 && (!props || !F || !sdv_old || !sdv_new || !S)))
+    return fail(SMA_ERR_ARG, "sma160_eval_device: bad arguments");
+  if (precision != 0 && precision != 1) return fail(SMA_ERR_ARG, "precision must be 0 or 1");
+  if (n == 0) return SMA_OK;
+  return law_launch(n, n_mat, props, mat_of_gp, E_of_gp, F, sdv_old, sdv_new, S, flags, precision,
+                    (hipStream_t)stream);
+}
+
+int sma160_eval(int64_t n, int32_t n_mat, const double* props, const int32_t* mat_of_gp,
+                const double* E_of_gp, const double* F, const double* T, const double* sdv_old,
+                double* sdv_new, double* S, uint32_t* flags, int32_t precision) {
+  (void)T;
+  if (n < 0 || n_mat <= 0 || (n > 0 && (!props || !F || !sdv_old || !sdv_new || !S)))
+    return fail(SMA_ERR_ARG, "sma160_eval: bad arguments");
+  if (precision != 0 && precision != 1) return fail(SMA_ERR_ARG, "precision must be 0 or 1");
+  if (n == 0) return SMA_OK;
+  if (mat_of_gp)
+    for (int64_t i = 0; i < n; ++i)
+      if (mat_of_gp[i] < 0 || mat_of_gp[i] >= n_mat)
+        return fail(SMA_ERR_ARG, "sma160_eval: material index out of range");
+  double *dp = nullptr, *dE = nullptr, *dF = nullptr, *dso = nullptr, *dsn = nullptr, *dS = nullptr;
+  int32_t* dm = nullptr;
+  uint32_t* dfl = nullptr;
+  int rc = 0;
+  auto cleanup = [&]() {
+    hipFree(dp); hipFree(dE); hipFree(dF); hipFree(dso); hipFree(dsn); hipFree(dS);
+    hipFree(dm); hipFree(dfl);
+  };
+  do {
+    if ((rc = upload(&dp, props, (size_t)n_mat * 18, nullptr))) break;
+    if (mat_of_gp && (rc = upload(&dm, mat_of_gp, (size_t)n, nullptr))) break;
+    if (E_of_gp && (rc = upload(&dE, E_of_gp, (size_t)n, nullptr))) break;
+    if ((rc = upload(&dF, F, (size_t)9 * n, nullptr))) break;
+    if ((rc = upload(&dso, sdv_old, (size_t)3 * n, nullptr))) break;
+    if ((rc = dalloc(&dsn, (size_t)3 * n))) break;
+    if ((rc = dalloc(&dS, (size_t)9 * n))) break;
+    if (flags && (rc = dalloc(&dfl, (size_t)n))) break;
+    if ((rc = law_launch(n, n_mat, dp, dm, dE, dF, dso, dsn, dS, dfl, precision, nullptr))) break;
+    hipError_t e;
+    if ((e = hipMemcpy(sdv_new, dsn, 3 * n * sizeof(double), hipMemcpyDeviceToHost)) != hipSuccess ||
+        (e = hipMemcpy(S, dS, 9 * n * sizeof(double), hipMemcpyDeviceToHost)) != hipSuccess ||
+        (flags && (e = hipMemcpy(flags, dfl, n * sizeof(uint32_t), hipMemcpyDeviceToHost)) != hipSuccess)) {
+      rc = fail(SMA_ERR_HIP, std::string("sma160_eval copy-back: ") + hipGetErrorString(e));
+      break;
+    }
+  } while (0);
+  cleanup();
+  return rc;
+}
+
+}  // extern "C"

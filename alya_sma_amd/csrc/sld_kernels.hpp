@@ -1,0 +1,489 @@
+// HIP kernels of the explicit solid-mechanics step (gfx950, wave64).
+//
+// Per step, on one stream (captured into a hipGraph by the host):
+//   sld_predict   nodal predictor, Dirichlet overwrite, zeroes f_int
+//                 (mod_sld_solution_methods.f90:122-145 CD, :406-429 TW;
+//                 sld_updbcs.f90:79-114 + sld_funbou.f90:95-136)
+//   sld_clock     t^{n+1} = t^n + dt (setgts.f90:98)
+//   sld_element   one launch per colour: SoA gather into LDS element tiles,
+//                 reference Jacobian + deformation gradient at the 8 Gauss
+//                 points, SMA return map, Kirchhoff stress, internal force,
+//                 deterministic coloured scatter into f_int
+//                 (sld_element_operations.f90:119-256, elmca2.f90:105-111,
+//                 sld_elmpre.f90:149-162, sld_builtin_materials.f90:316-371,
+//                 sld_elmmat.f90:137-150, mod_matrix.f90:2327-2431)
+//   sld_correct   a = (f_ext - f_int) / m, velocity corrector / TW Dirichlet
+//                 zeroing (solric.f90:294-330; mod_sld_solution_methods.f90:
+//                 181-210 CD, :494-527 TW)
+// The history (xS, F_AS, F_SA) lives in two SoA buffers that swap roles
+// every step (sld_updunk.f90:149-186, svegm TIME_N <- ITER_K).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "sma_law.hpp"
+
+namespace sld {
+
+// HEX08 natural derivatives at the 2x2x2 Gauss points: dN_a/dxi_k at GP g is
+// 0.5 * s_ak * f(a,g,m1) * f(a,g,m2) where f = 0.5*(1 +- 0.577350269189626)
+// depending on whether node a and GP g lie on the same side in direction m
+// (mod_lagrange.f90:935-1000, mod_quadrature_gauss.f90:2320-2321,2372-2385).
+__device__ __forceinline__ int node_sign(int a, int k) {
+  // node order (-,-,-) (+,-,-) (+,+,-) (-,+,-) (-,-,+) (+,-,+) (+,+,+) (-,+,+)
+  const int sx = 0x66, sy = 0xCC, sz = 0xF0;  // bit a set = +1
+  int m = (k == 0) ? sx : (k == 1) ? sy : sz;
+  return ((m >> a) & 1) ? 1 : -1;
+}
+__device__ __forceinline__ int gp_sign(int g, int k) {
+  // GP order: xi outer, eta, zeta inner; bit = 1 means +0.5773...
+  int bit = (k == 0) ? (g >> 2) & 1 : (k == 1) ? (g >> 1) & 1 : g & 1;
+  return bit ? 1 : -1;
+}
+template <typename R>
+__device__ __forceinline__ R hex_deriv(int a, int g, int k) {
+  const double gpos = 0.577350269189626;
+  const R cp = (R)(0.5 * (1.0 + gpos)), cm = (R)(0.5 * (1.0 - gpos));
+  R f[3];
+#pragma unroll
+  for (int m = 0; m < 3; ++m) f[m] = (node_sign(a, m) == gp_sign(g, m)) ? cp : cm;
+  R s = (R)(0.5 * node_sign(a, k));
+  if (k == 0) return s * f[1] * f[2];
+  if (k == 1) return s * f[0] * f[2];
+  return s * f[0] * f[1];
+}
+template <typename R>
+__device__ __forceinline__ R hex_shape(int a, int g) {
+  const double gpos = 0.577350269189626;
+  const R cp = (R)(0.5 * (1.0 + gpos)), cm = (R)(0.5 * (1.0 - gpos));
+  R f0 = (node_sign(a, 0) == gp_sign(g, 0)) ? cp : cm;
+  R f1 = (node_sign(a, 1) == gp_sign(g, 1)) ? cp : cm;
+  R f2 = (node_sign(a, 2) == gp_sign(g, 2)) ? cp : cm;
+  return f0 * f1 * f2;
+}
+
+// Load tables: DISCRETE[,SMOOTH] (sld_funbou.f90 itask 2), rows (vx,vy,vz,t).
+__device__ __forceinline__ double funbou(const double* __restrict__ tab, int npts,
+                                         int smooth, int c, double t) {
+#pragma clang fp contract(off)
+  double x = 0.0;
+  for (int i = 0; i < npts - 1; ++i) {
+    double t1 = tab[4 * i + 3], t2 = tab[4 * (i + 1) + 3];
+    if (t >= t1 && t <= t2) {
+      double p1 = tab[4 * i + c], p2 = tab[4 * (i + 1) + c], m;
+      if (smooth == 1) {
+        m = (t - t1) / (t2 - t1);
+        x = p1 + (p2 - p1) * (m * m * m) * (10.0 - 15.0 * m + 6.0 * (m * m));
+      } else {
+        m = (p2 - p1) / (t2 - t1);
+        x = p1 + m * (t - t1);
+      }
+      break;
+    }
+  }
+  double tlast = tab[4 * (npts - 1) + 3];
+  if (t > tlast) x = tab[4 * (npts - 1) + c] * t / tlast;
+  return x;
+}
+
+struct NodalArgs {
+  int64_t npoin;
+  double* u[3];
+  double* v[3];
+  double* a[3];
+  void* fint[3];               // double or float per precision mode
+  int fint_fp32;
+  const double* rmass;         // 1/m per node
+  const uint8_t* fixmask;
+  const int32_t* func_of_node; // may be null
+  const double* func_table;    // [nfunc][maxpts][4]
+  const int32_t* func_npts;
+  const int32_t* func_smooth;
+  int32_t func_maxpts;
+  const double* fext0[3];      // may be null
+  int32_t fext_func;           // -1 = none
+  double* clock;               // device time t^n
+};
+
+template <int SCHEME>  // 0 = central difference, 1 = Tchamwa-Wielgosz
+__global__ void __launch_bounds__(256) sld_predict(NodalArgs A, double dt, double phi) {
+#pragma clang fp contract(off)
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.npoin) return;
+  const double cutim = A.clock[0] + dt;
+  const double t_end = cutim + dt, t_mid = 0.5 * (cutim + t_end);
+  const int fix = A.fixmask[i];
+  const int fn = A.func_of_node ? A.func_of_node[i] : -1;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    double u = A.u[d][i], v = A.v[d][i], a = A.a[d][i];
+    double un, vn;
+    if (SCHEME == 0) {
+      vn = v + (t_mid - cutim) * a;
+      un = u + dt * vn;
+    } else {
+      vn = v + dt * a;
+      un = u + dt * v + phi * (dt * dt) * a;
+    }
+    if ((fix >> d) & 1) {
+      un = 0.0;
+      if (fn >= 0)
+        un = funbou(A.func_table + (size_t)fn * A.func_maxpts * 4, A.func_npts[fn],
+                    A.func_smooth[fn], d, cutim);
+    }
+    A.u[d][i] = un;
+    A.v[d][i] = vn;
+    if (A.fint_fp32) ((float*)A.fint[d])[i] = 0.f;
+    else ((double*)A.fint[d])[i] = 0.0;
+  }
+}
+
+__global__ void sld_clock(double* clock, double dt) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) clock[0] = clock[0] + dt;
+}
+
+template <int SCHEME>
+__global__ void __launch_bounds__(256) sld_correct(NodalArgs A, double dt) {
+#pragma clang fp contract(off)
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.npoin) return;
+  const double cutim = A.clock[0];  // already advanced by sld_clock
+  const double t_end = cutim + dt, t_mid = 0.5 * (cutim + t_end);
+  double fx = 0.0;
+  if (A.fext_func >= 0)
+    fx = funbou(A.func_table + (size_t)A.fext_func * A.func_maxpts * 4,
+                A.func_npts[A.fext_func], A.func_smooth[A.fext_func], 0, cutim);
+  const double rm = A.rmass[i];
+  const int fix = A.fixmask[i];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    double fi = A.fint_fp32 ? (double)((const float*)A.fint[d])[i] : ((const double*)A.fint[d])[i];
+    double r = 0.0 - fi;
+    if (A.fext_func >= 0) r = r + fx * A.fext0[d][i];
+    double acc = 0.0 + r * rm;
+    if (SCHEME == 0) {
+      A.a[d][i] = acc;
+      A.v[d][i] = A.v[d][i] + (t_end - t_mid) * acc;
+    } else {
+      if ((fix >> d) & 1) { A.v[d][i] = 0.0; acc = 0.0; }
+      A.a[d][i] = acc;
+    }
+  }
+}
+
+// ----------------------------------------------------------------------------
+// Element kernel
+// ----------------------------------------------------------------------------
+struct ElemArgs {
+  int32_t e_begin, e_end;       // colour range in the colour-sorted element order
+  int64_t nelem;
+  const int32_t* lnods;         // [nelem][8]
+  const double* coord[3];       // SoA reference coordinates
+  const double* u[3];           // SoA displacement (ITER_K)
+  const double* props;          // [nmat][18]
+  const int32_t* mat_of_elem;   // [nelem] or null
+  const double* E_of_elem;      // [nelem] or null
+  const double* sdv_n[3];       // TIME_N history, SoA over GP = 8*e + g
+  double* sdv_k[3];             // ITER_K history
+  void* fint[3];                // nodal internal force (double or float)
+  double* S_out;                // optional PK2 [9][ngp]
+  uint32_t* flags_out;          // optional [ngp]
+  unsigned long long* counters; // [8] per-flag Gauss-point counts
+  int32_t xcd_remap;            // 1 = spread consecutive tiles per XCD
+};
+
+template <typename R>
+struct ElemTraits;
+template <>
+struct ElemTraits<double> { using Acc = double; };
+template <>
+struct ElemTraits<float> { using Acc = float; };
+
+// Wave-level flag counters: one 64-bit atomic per wave and per set bit type.
+__device__ __forceinline__ void count_flags(unsigned long long* ctr, uint32_t fl, bool valid) {
+  const uint32_t bits[7] = {sma::FLAG_KF_AS, sma::FLAG_KF_SA, sma::FLAG_NEWTON_FAIL,
+                            sma::FLAG_EIGEN_FAIL, sma::FLAG_DETF_NONPOS,
+                            sma::FLAG_SURR_USED, sma::FLAG_SURR_FALLBACK};
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int b = 0; b < 7; ++b) {
+    unsigned long long m = __ballot(valid && (fl & bits[b]));
+    if (m && lane == 0) atomicAdd(ctr + b, (unsigned long long)__popcll(m));
+  }
+}
+
+// EPB elements per block, 8 threads (one per Gauss point) per element.
+// R = kinematics/stress precision; the return map is always FP64.
+template <typename R, int EPB, bool STORE>
+__global__ void __launch_bounds__(EPB * 8) sld_element(ElemArgs A) {
+  __shared__ double s_abs[EPB][8][6];     // gathered absolute x and u
+  __shared__ R s_x[EPB][8][3];            // x_a - x_0
+  __shared__ R s_u[EPB][8][3];            // u_a - u_0
+  __shared__ R s_q[EPB][8][9];            // Q = detJ tau j^-T per Gauss point
+  __shared__ int s_node[EPB][8];
+
+  const int tid = threadIdx.x;
+  const int le = tid >> 3, g = tid & 7;
+  int blk = blockIdx.x;
+  if (A.xcd_remap) {
+    // workgroups are dealt round-robin to the 8 XCDs: give each XCD a
+    // contiguous run of tiles so neighbouring elements share its L2
+    const int nb = gridDim.x, per = (nb + 7) / 8;
+    const int xcd = blk & 7, k = blk >> 3;
+    const int cand = xcd * per + k;
+    // only remap when the grid divides evenly (keeps the map a bijection)
+    if (nb % 8 == 0) blk = cand;
+  }
+  const int e = A.e_begin + blk * EPB + le;
+  const bool valid = e < A.e_end;
+
+  // -- phase 1: gather the element's 8 nodes into LDS (thread g loads node g)
+  int node = 0;
+  if (valid) {
+    node = A.lnods[(int64_t)e * 8 + g];
+    s_abs[le][g][0] = A.coord[0][node];
+    s_abs[le][g][1] = A.coord[1][node];
+    s_abs[le][g][2] = A.coord[2][node];
+    s_abs[le][g][3] = A.u[0][node];
+    s_abs[le][g][4] = A.u[1][node];
+    s_abs[le][g][5] = A.u[2][node];
+  }
+  s_node[le][g] = node;
+  __syncthreads();
+  if (valid) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      s_x[le][g][d] = (R)(s_abs[le][g][d] - s_abs[le][0][d]);
+      s_u[le][g][d] = (R)(s_abs[le][g][3 + d] - s_abs[le][0][3 + d]);
+    }
+  }
+  __syncthreads();
+
+  // -- phase 2: Gauss point g of element e
+  uint32_t fl = 0;
+  if (valid) {
+    R J[9] = {0}, Gu[9] = {0};  // col-major: J(i,k) = J[i + 3k]
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      R D0 = hex_deriv<R>(a, g, 0), D1 = hex_deriv<R>(a, g, 1), D2 = hex_deriv<R>(a, g, 2);
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        R x = s_x[le][a][i], uu = s_u[le][a][i];
+        J[i] += x * D0; J[i + 3] += x * D1; J[i + 6] += x * D2;
+        Gu[i] += uu * D0; Gu[i + 3] += uu * D1; Gu[i + 6] += uu * D2;
+      }
+    }
+    R Ji[9];
+    const R detJ = sma::inv3<R>(J, Ji);
+    // H = F - I = Gu J^-1
+    R H[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        H[i + 3 * j] = Gu[i] * Ji[3 * j] + Gu[i + 3] * Ji[1 + 3 * j] + Gu[i + 6] * Ji[2 + 3 * j];
+
+    const int mat = A.mat_of_elem ? A.mat_of_elem[e] : 0;
+    const double* pr = A.props + 18 * mat;
+    const double E = A.E_of_elem ? A.E_of_elem[e] : pr[0];
+    const sma::Mat m = sma::make_mat(pr, E);
+    const int64_t gp = (int64_t)e * 8 + g;
+    const double xS0 = A.sdv_n[0][gp], FAS0 = A.sdv_n[1][gp], FSA0 = A.sdv_n[2][gp];
+
+    R tau[6];
+    R Fm[9];
+    sma::ReturnMap rm;
+    if constexpr (sizeof(R) == 8) {
+      double F[9];
+#pragma unroll
+      for (int q = 0; q < 9; ++q) F[q] = H[q] + ((q % 4 == 0) ? 1.0 : 0.0);
+      sma::Spectral<double> sp = sma::spectral_fp64(F);
+      rm = sma::return_map(m, sp.theta, sp.e_n, xS0, FAS0, FSA0);
+      sma::kirchhoff<double>(m, sp, rm.xS, tau);
+      if (!sp.eig_ok) rm.flags |= sma::FLAG_EIGEN_FAIL;
+#pragma unroll
+      for (int q = 0; q < 9; ++q) Fm[q] = F[q];
+    } else {
+      float Hf[9];
+#pragma unroll
+      for (int q = 0; q < 9; ++q) Hf[q] = H[q];
+      sma::Spectral<float> sp = sma::spectral_fp32(Hf);
+      rm = sma::return_map(m, sp.theta, sp.e_n, xS0, FAS0, FSA0);
+      sma::kirchhoff<float>(m, sp, rm.xS, tau);
+      if (!sp.eig_ok) rm.flags |= sma::FLAG_EIGEN_FAIL;
+#pragma unroll
+      for (int q = 0; q < 9; ++q) Fm[q] = H[q] + ((q % 4 == 0) ? R(1) : R(0));
+    }
+    A.sdv_k[0][gp] = rm.xS;
+    A.sdv_k[1][gp] = rm.F_AS;
+    A.sdv_k[2][gp] = rm.F_SA;
+    fl = rm.flags;
+
+    // current Jacobian j = J + Gu = F J; Q = detJ * tau * j^-T
+    R jc[9], jci[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) jc[q] = J[q] + Gu[q];
+    const R detj = sma::inv3<R>(jc, jci);
+    if (!(detj / detJ > R(0))) fl |= sma::FLAG_DETF_NONPOS;
+    const R T[9] = {tau[0], tau[1], tau[2], tau[1], tau[3], tau[4], tau[2], tau[4], tau[5]};
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+        s_q[le][g][3 * i + k] =
+            detJ * (T[i] * jci[k] + T[i + 3] * jci[k + 3] + T[i + 6] * jci[k + 6]);
+
+    if constexpr (STORE) {
+      R Fi[9], S[9];
+      sma::inv3<R>(Fm, Fi);
+      sma::pk2<R>(Fi, tau, S);
+      const int64_t ngp = A.nelem * 8;
+#pragma unroll
+      for (int q = 0; q < 9; ++q) A.S_out[q * ngp + gp] = (double)S[q];
+      A.flags_out[gp] = fl;
+    }
+  }
+  count_flags(A.counters, fl, valid);
+  __syncthreads();
+
+  // -- phase 3: thread g now owns node a = g: f_ai = sum_g' sum_k Q_ik(g') D_ak(g')
+  if (valid) {
+    const int a = g;
+    using Acc = typename ElemTraits<R>::Acc;
+    Acc f0 = 0, f1 = 0, f2 = 0;
+#pragma unroll
+    for (int gg = 0; gg < 8; ++gg) {
+      R D0 = hex_deriv<R>(a, gg, 0), D1 = hex_deriv<R>(a, gg, 1), D2 = hex_deriv<R>(a, gg, 2);
+      const R* q = s_q[le][gg];
+      f0 += q[0] * D0 + q[1] * D1 + q[2] * D2;
+      f1 += q[3] * D0 + q[4] * D1 + q[5] * D2;
+      f2 += q[6] * D0 + q[7] * D1 + q[8] * D2;
+    }
+    // coloured scatter: no other element of this colour touches this node
+    R* fx = (R*)A.fint[0];
+    R* fy = (R*)A.fint[1];
+    R* fz = (R*)A.fint[2];
+    fx[node] += f0;
+    fy[node] += f1;
+    fz[node] += f2;
+  }
+}
+
+// ----------------------------------------------------------------------------
+// Setup: lumped mass (sld_elmmat.f90:199-209), one thread per element, per colour.
+// ----------------------------------------------------------------------------
+struct MassArgs {
+  int32_t e_begin, e_end;
+  const int32_t* lnods;
+  const double* coord[3];
+  const double* rho;
+  const int32_t* mat_of_elem;
+  double* mass;
+};
+
+__global__ void sld_mass(MassArgs A) {
+#pragma clang fp contract(off)
+  const int e = A.e_begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= A.e_end) return;
+  double x[8][3];
+  int nodes[8];
+  for (int a = 0; a < 8; ++a) {
+    nodes[a] = A.lnods[(int64_t)e * 8 + a];
+    for (int d = 0; d < 3; ++d) x[a][d] = A.coord[d][nodes[a]];
+  }
+  const double rho = A.rho[A.mat_of_elem ? A.mat_of_elem[e] : 0];
+  double elmuu[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int g = 0; g < 8; ++g) {
+    double J[9] = {0};
+    for (int a = 0; a < 8; ++a)
+      for (int k = 0; k < 3; ++k) {
+        double D = hex_deriv<double>(a, g, k);
+        for (int i = 0; i < 3; ++i) J[i + 3 * k] = J[i + 3 * k] + x[a][i] * D;
+      }
+    // jacobi.f90 ndime==3 determinant by first-row cofactors
+    double t1 = J[4] * J[8] - J[5] * J[7];
+    double t2 = -J[1] * J[8] + J[2] * J[7];
+    double t3 = J[1] * J[5] - J[2] * J[4];
+    double vol = 1.0 * (J[0] * t1 + J[3] * t2 + J[6] * t3);
+    for (int a = 0; a < 8; ++a)
+      for (int b = 0; b < 8; ++b)
+        elmuu[a] = elmuu[a] + rho * vol * hex_shape<double>(a, g) * hex_shape<double>(b, g);
+  }
+  for (int a = 0; a < 8; ++a) A.mass[nodes[a]] += elmuu[a];
+}
+
+__global__ void sld_reciprocal(const double* __restrict__ m, double* __restrict__ rm, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) rm[i] = 1.0 / m[i];
+}
+
+// ----------------------------------------------------------------------------
+// Point-wise law (drop-in for eval_sld_stress_model_160 with flagt = 0)
+// ----------------------------------------------------------------------------
+struct LawArgs {
+  int64_t n;
+  int32_t nmat;
+  const double* props;
+  const int32_t* mat_of_gp;
+  const double* E_of_gp;
+  const double* F;        // [9][n]
+  const double* sdv_old;  // [3][n]
+  double* sdv_new;        // [3][n]
+  double* S;              // [9][n]
+  uint32_t* flags;        // [n] or null
+};
+
+template <bool FP32_KIN>
+__global__ void __launch_bounds__(256) sma160_points(LawArgs A) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.n) return;
+  int mat = A.mat_of_gp ? A.mat_of_gp[i] : 0;
+  uint32_t bad = 0;
+  if (mat < 0 || mat >= A.nmat) {  // device arrays cannot be checked on the host
+    bad = sma::FLAG_BAD_INPUT;
+    mat = 0;
+  }
+  const double* pr = A.props + 18 * mat;
+  const double E = A.E_of_gp ? A.E_of_gp[i] : pr[0];
+  const sma::Mat m = sma::make_mat(pr, E);
+  double F[9];
+#pragma unroll
+  for (int q = 0; q < 9; ++q) F[q] = A.F[q * A.n + i];
+  const double xS0 = A.sdv_old[i], FAS0 = A.sdv_old[A.n + i], FSA0 = A.sdv_old[2 * A.n + i];
+  sma::ReturnMap rm;
+  double S[9];
+  if constexpr (!FP32_KIN) {
+    sma::Spectral<double> sp = sma::spectral_fp64(F);
+    rm = sma::return_map(m, sp.theta, sp.e_n, xS0, FAS0, FSA0);
+    if (!sp.eig_ok) rm.flags |= sma::FLAG_EIGEN_FAIL;
+    double tau[6], Fi[9];
+    sma::kirchhoff<double>(m, sp, rm.xS, tau);
+    double det = sma::inv3<double>(F, Fi);
+    if (!(det > 0.0)) rm.flags |= sma::FLAG_DETF_NONPOS;
+    sma::pk2<double>(Fi, tau, S);
+  } else {
+    float H[9], Ff[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+      H[q] = (float)(F[q] - ((q % 4 == 0) ? 1.0 : 0.0));
+      Ff[q] = (float)F[q];
+    }
+    sma::Spectral<float> sp = sma::spectral_fp32(H);
+    rm = sma::return_map(m, sp.theta, sp.e_n, xS0, FAS0, FSA0);
+    if (!sp.eig_ok) rm.flags |= sma::FLAG_EIGEN_FAIL;
+    float tau[6], Fi[9], Sf[9];
+    sma::kirchhoff<float>(m, sp, rm.xS, tau);
+    float det = sma::inv3<float>(Ff, Fi);
+    if (!(det > 0.f)) rm.flags |= sma::FLAG_DETF_NONPOS;
+    sma::pk2<float>(Fi, tau, Sf);
+#pragma unroll
+    for (int q = 0; q < 9; ++q) S[q] = Sf[q];
+  }
+  A.sdv_new[i] = rm.xS;
+  A.sdv_new[A.n + i] = rm.F_AS;
+  A.sdv_new[2 * A.n + i] = rm.F_SA;
+#pragma unroll
+  for (int q = 0; q < 9; ++q) A.S[q * A.n + i] = S[q];
+  if (A.flags) A.flags[i] = rm.flags | bad;
+}
+
+}  // namespace sld

@@ -1,0 +1,409 @@
+// Device-side superelastic SMA law (Alya solidz stress model 160) for gfx950.
+//
+// Reference: sld_stress_model_160.f90 (compute_kirchoff_stress :212-500,
+// eval_loading_function :513-537, eval_beta :539-556 with beta forced to 1,
+// INV2x2 :558-573), dsyevq3.f90:22-137 + dsytrd3.f90:22-109 (Kopp's
+// Householder + QL), eigsrt.f90:19-48 (descending selection sort).
+//
+// Two spectral front ends feed one FP64 return map:
+//   * spectral_fp64: b = F F^T, Kopp QL on b, sqrt/pow/log exactly as the
+//     reference sequences them (plain FP64 mode, parity <= 1e-10);
+//   * spectral_fp32: the same QL on b - I = H + H^T + H H^T (H = F - I) in
+//     FP32, with e_i = log1p(mu_i)/2 - theta/3.  Working on b - I keeps the
+//     small strains at full FP32 relative precision (FP32 kinematics mode).
+// The loading functions, branch flags and the 2x2 Newton on (dh_AS, dh_SA)
+// always run in FP64, with FMA contraction off so that every branch decision
+// sees the same rounding as the reference.
+//
+// All loops over the 3 eigenpairs are written with compile-time indices so
+// that the state stays in VGPRs (no dynamically indexed private arrays).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sma {
+
+constexpr uint32_t FLAG_KF_AS = 0x01u;
+constexpr uint32_t FLAG_KF_SA = 0x02u;
+constexpr uint32_t FLAG_NEWTON_FAIL = 0x04u;
+constexpr uint32_t FLAG_EIGEN_FAIL = 0x08u;
+constexpr uint32_t FLAG_DETF_NONPOS = 0x10u;
+constexpr uint32_t FLAG_SURR_USED = 0x20u;
+constexpr uint32_t FLAG_SURR_FALLBACK = 0x40u;
+constexpr uint32_t FLAG_BAD_INPUT = 0x80u;
+constexpr int FLAG_ITER_SHIFT = 8;
+
+// Material constants derived once per point from props(18) and E
+// (compute_kirchoff_stress :263-289).
+struct Mat {
+  double K, G, eL, alfa, Rf_AS, Rs_AS, Rf_SA, Rs_SA, G_1, C_AS, C_SA;
+};
+
+__host__ __device__ __forceinline__ Mat make_mat(const double* __restrict__ p, double E) {
+#pragma clang fp contract(off)
+  Mat m;
+  double v = p[1];
+  m.eL = p[2];
+  m.alfa = p[14];
+  m.C_AS = p[5];
+  m.C_SA = p[11];
+  m.K = E / (3.0 - 6.0 * v);
+  m.G = E / (2.0 + 2.0 * v);
+  double aux = sqrt(2.0 / 3.0) + m.alfa;
+  m.Rf_AS = p[4] * aux + p[5] * p[6];
+  m.Rs_AS = p[3] * aux + p[5] * p[7];
+  m.Rf_SA = p[10] * aux + p[11] * p[12];
+  m.Rs_SA = p[9] * aux + p[11] * p[13];
+  m.G_1 = (2.0 * m.G + 9.0 * (m.alfa * m.alfa) * m.K) * m.eL;
+  return m;
+}
+
+// One Givens step of the implicit-shift QL sweep (dsyevq3.f90:99-129) on
+// scalar state: off-diagonals ej, ej1, diagonals wj, wj1, eigenvector columns
+// (a0,a1,a2) = column j and (b0,b1,b2) = column j+1.  Scalars, not arrays, so
+// that every value stays in a VGPR.
+template <typename R>
+__host__ __device__ __forceinline__ void ql_rot(R& s, R& c, R& p, R& g, R ej, R& ej1, R wj, R& wj1,
+                                       R& a0, R& a1, R& a2, R& b0, R& b1, R& b2) {
+#pragma clang fp contract(off)
+  R f = s * ej;
+  R b = c * ej;
+  // Both branches of :102-114 as one select (identical arithmetic per branch):
+  // |f| > |g|: c = g/f, r = sqrt(1+c^2), e = f r, s = 1/r, c = c s
+  // else     : s = f/g, r = sqrt(1+s^2), e = g r, c = 1/r, s = s c
+  const bool big = fabs(f) > fabs(g);
+  const R num = big ? g : f, den = big ? f : g;
+  const R t0 = num / den;
+  R r = sqrt(R(1) + t0 * t0);
+  ej1 = den * r;
+  const R inv = R(1) / r;
+  const R tr = t0 * inv;
+  c = big ? tr : inv;
+  s = big ? inv : tr;
+  g = wj1 - p;
+  r = (wj - g) * s + R(2) * c * b;
+  p = s * r;
+  wj1 = g + p;
+  g = c * r - b;
+  R t;
+  t = b0; b0 = s * a0 + c * t; a0 = c * a0 - s * t;
+  t = b1; b1 = s * a1 + c * t; a1 = c * a1 - s * t;
+  t = b2; b2 = s * a2 + c * t; a2 = c * a2 - s * t;
+}
+
+// Shift of dsyevq3.f90:87-94 for (L, M): wl, wl1 = W(L), W(L+1); wm = W(M).
+template <typename R>
+__host__ __device__ __forceinline__ R ql_shift(R wl, R wl1, R wm, R el) {
+#pragma clang fp contract(off)
+  R g = (wl1 - wl) / (R(2) * el);
+  R r = sqrt(R(1) + g * g);
+  if (g >= R(0)) return wm - wl + el / (g + r);
+  return wm - wl + el / (g - r);
+}
+
+// Symmetric 3x3 eigen-decomposition: dsytrd3 + dsyevq3 + eigsrt (largest
+// eigenvalue first).  Eigenvectors in q[9] column-major.  Returns false on
+// the 30-iteration bail-out (dsyevq3.f90:82-84).
+template <typename R>
+__host__ __device__ __forceinline__ bool eig3(R a00, R a01, R a02, R a11, R a12, R a22,
+                                     R& w0, R& w1, R& w2, R (&q)[9]) {
+#pragma clang fp contract(off)
+  // dsytrd3.f90:22-109 (Q starts as the identity)
+  R q00 = 1, q01 = 0, q02 = 0, q10 = 0, q11 = 1, q12 = 0, q20 = 0, q21 = 0, q22 = 1;
+  R e0, e1, e2 = 0;
+  R h = a01 * a01 + a02 * a02;
+  R g = (a01 > R(0)) ? -sqrt(h) : sqrt(h);
+  e0 = g;
+  R f = g * a01;
+  R u1 = a01 - g, u2 = a02;
+  R omega = h - f;
+  if (omega > R(0)) {
+    omega = R(1) / omega;
+    R f1 = a11 * u1 + a12 * u2;          // I = 2: A(2,2)U(2) + A(2,3)U(3)
+    R p1 = omega * f1;
+    R K = R(0) + u1 * f1;
+    R f2 = a12 * u1 + a22 * u2;          // I = 3: A(2,3)U(2) + A(3,3)U(3)
+    R p2 = omega * f2;
+    K = K + u2 * f2;
+    K = R(0.5) * K * (omega * omega);
+    p1 = p1 - K * u1;
+    p2 = p2 - K * u2;
+    w0 = a00;
+    w1 = a11 - R(2) * p1 * u1;
+    w2 = a22 - R(2) * p2 * u2;
+    R fa = omega * u1;
+    q11 = q11 - fa * u1;
+    q21 = q21 - fa * u2;
+    R fb = omega * u2;
+    q12 = q12 - fb * u1;
+    q22 = q22 - fb * u2;
+    e1 = a12 - p1 * u2 - u1 * p2;
+  } else {
+    w0 = a00; w1 = a11; w2 = a22;
+    e1 = a12;
+  }
+  // dsyevq3.f90:64-134: L = 1 (0-based 0), then L = 2
+  bool ok = true;
+  {
+    int niter = 0;
+    for (int it = 0; it < 50; ++it) {
+      R g0 = fabs(w0) + fabs(w1);
+      if (fabs(e0) + g0 == g0) break;                 // M = L
+      R g1 = fabs(w1) + fabs(w2);
+      const bool m1 = (fabs(e1) + g1 == g1);          // M = 2, else M = 3
+      if (++niter >= 30) { ok = false; break; }
+      R s = 1, c = 1, p = 0;
+      if (m1) {
+        R gg = ql_shift<R>(w0, w1, w1, e0);
+        ql_rot<R>(s, c, p, gg, e0, e1, w0, w1, q00, q10, q20, q01, q11, q21);
+        w0 = w0 - p; e0 = gg; e1 = 0;
+      } else {
+        R gg = ql_shift<R>(w0, w1, w2, e0);
+        ql_rot<R>(s, c, p, gg, e1, e2, w1, w2, q01, q11, q21, q02, q12, q22);
+        ql_rot<R>(s, c, p, gg, e0, e1, w0, w1, q00, q10, q20, q01, q11, q21);
+        w0 = w0 - p; e0 = gg; e2 = 0;
+      }
+    }
+  }
+  if (ok) {
+    int niter = 0;
+    for (int it = 0; it < 50; ++it) {
+      R g1 = fabs(w1) + fabs(w2);
+      if (fabs(e1) + g1 == g1) break;
+      if (++niter >= 30) { ok = false; break; }
+      R s = 1, c = 1, p = 0;
+      R gg = ql_shift<R>(w1, w2, w2, e1);
+      ql_rot<R>(s, c, p, gg, e1, e2, w1, w2, q01, q11, q21, q02, q12, q22);
+      w1 = w1 - p; e1 = gg; e2 = 0;
+    }
+  }
+  // eigsrt.f90:19-48 as selects.  Pass 1 moves the last maximum ('>=') to
+  // slot 1; pass 2 swaps slots 2 and 3 whenever d(3) >= d(2).
+  {
+    const bool k1 = (w1 >= w0) && !(w2 >= w1);
+    const bool k2 = (w2 >= ((w1 >= w0) ? w1 : w0));
+    const R pmax = k2 ? w2 : (k1 ? w1 : w0);
+    const R nw1 = k1 ? w0 : w1, nw2 = k2 ? w0 : w2;
+    R c00 = k1 ? q01 : (k2 ? q02 : q00), c10 = k1 ? q11 : (k2 ? q12 : q10),
+      c20 = k1 ? q21 : (k2 ? q22 : q20);
+    R c01 = k1 ? q00 : q01, c11 = k1 ? q10 : q11, c21 = k1 ? q20 : q21;
+    R c02 = k2 ? q00 : q02, c12 = k2 ? q10 : q12, c22 = k2 ? q20 : q22;
+    w0 = pmax;
+    const bool sw = nw2 >= nw1;
+    w1 = sw ? nw2 : nw1;
+    w2 = sw ? nw1 : nw2;
+    q00 = c00; q10 = c10; q20 = c20;
+    q01 = sw ? c02 : c01; q11 = sw ? c12 : c11; q21 = sw ? c22 : c21;
+    q02 = sw ? c01 : c02; q12 = sw ? c11 : c12; q22 = sw ? c21 : c22;
+  }
+  q[0] = q00; q[1] = q10; q[2] = q20;
+  q[3] = q01; q[4] = q11; q[5] = q21;
+  q[6] = q02; q[7] = q12; q[8] = q22;
+  return ok;
+}
+
+// Log-strain invariants that the return map consumes, plus the principal
+// directions and deviatoric strains the stress needs.
+template <typename R>
+struct Spectral {
+  double theta, e_n;            // FP64 for the return map
+  R e[3], d[3];                 // principal log strains and e_i / e_n
+  R n[9];                       // principal directions (columns)
+  bool eig_ok;
+};
+
+// Plain FP64: compute_kirchoff_stress :293-332 in reference order.
+__host__ __device__ __forceinline__ Spectral<double> spectral_fp64(const double (&F)[9]) {
+#pragma clang fp contract(off)
+  Spectral<double> s;
+  // b = matmul(F, transpose(F)) (col-major F: F(i,k) = F[i + 3k])
+  double b00 = F[0] * F[0] + F[3] * F[3] + F[6] * F[6];
+  double b01 = F[0] * F[1] + F[3] * F[4] + F[6] * F[7];
+  double b02 = F[0] * F[2] + F[3] * F[5] + F[6] * F[8];
+  double b11 = F[1] * F[1] + F[4] * F[4] + F[7] * F[7];
+  double b12 = F[1] * F[2] + F[4] * F[5] + F[7] * F[8];
+  double b22 = F[2] * F[2] + F[5] * F[5] + F[8] * F[8];
+  double w0, w1, w2;
+  s.eig_ok = eig3<double>(b00, b01, b02, b11, b12, b22, w0, w1, w2, s.n);
+  double h1 = sqrt(w0), h2 = sqrt(w1), h3 = sqrt(w2);
+  double J = h1 * h2 * h3;
+  s.theta = log(J);
+  double Jd = pow(J, -1.0 / 3.0);
+  s.e[0] = log(Jd * h1);
+  s.e[1] = log(Jd * h2);
+  s.e[2] = log(Jd * h3);
+  s.e_n = sqrt(s.e[0] * s.e[0] + s.e[1] * s.e[1] + s.e[2] * s.e[2]);
+  const double eps = 2.220446049250313080847e-16;  // epsilon(0.0_rp)
+  if (s.e_n > eps) {
+    s.d[0] = s.e[0] / s.e_n; s.d[1] = s.e[1] / s.e_n; s.d[2] = s.e[2] / s.e_n;
+  } else {
+    s.d[0] = s.d[1] = s.d[2] = 0.0;
+  }
+  return s;
+}
+
+// FP32 kinematics: spectral decomposition of b - I from H = F - I.
+__host__ __device__ __forceinline__ Spectral<float> spectral_fp32(const float (&H)[9]) {
+  Spectral<float> s;
+  // (b - I)_ij = H_ij + H_ji + sum_k H_ik H_jk
+  float c00 = 2.f * H[0] + (H[0] * H[0] + H[3] * H[3] + H[6] * H[6]);
+  float c11 = 2.f * H[4] + (H[1] * H[1] + H[4] * H[4] + H[7] * H[7]);
+  float c22 = 2.f * H[8] + (H[2] * H[2] + H[5] * H[5] + H[8] * H[8]);
+  float c01 = H[3] + H[1] + (H[0] * H[1] + H[3] * H[4] + H[6] * H[7]);
+  float c02 = H[6] + H[2] + (H[0] * H[2] + H[3] * H[5] + H[6] * H[8]);
+  float c12 = H[7] + H[5] + (H[1] * H[2] + H[4] * H[5] + H[7] * H[8]);
+  float mu0, mu1, mu2;
+  s.eig_ok = eig3<float>(c00, c01, c02, c11, c12, c22, mu0, mu1, mu2, s.n);
+  float l0 = 0.5f * log1pf(mu0), l1 = 0.5f * log1pf(mu1), l2 = 0.5f * log1pf(mu2);
+  float th = l0 + l1 + l2;
+  float t3 = th * (1.0f / 3.0f);
+  s.e[0] = l0 - t3; s.e[1] = l1 - t3; s.e[2] = l2 - t3;
+  float en = sqrtf(s.e[0] * s.e[0] + s.e[1] * s.e[1] + s.e[2] * s.e[2]);
+  s.theta = (double)th;
+  s.e_n = (double)en;
+  if (en > 1.1920929e-07f) {
+    float r = 1.0f / en;
+    s.d[0] = s.e[0] * r; s.d[1] = s.e[1] * r; s.d[2] = s.e[2] * r;
+  } else {
+    s.d[0] = s.d[1] = s.d[2] = 0.f;
+  }
+  return s;
+}
+
+// eval_loading_function :513-537 (beta = 1 per :554, temperature inert).
+__host__ __device__ __forceinline__ double loading(const Mat& m, double C, double theta,
+                                          double enor, double xS) {
+#pragma clang fp contract(off)
+  double pres = m.K * (theta - 3.0 * m.alfa * m.eL * xS);
+  double trac = 2.0 * m.G * (enor - m.eL * xS);
+  return trac * 1.0 + 3.0 * m.alfa * pres - C * 0.0;
+}
+
+// Branch flags and return map (:334-484).  sdv in = TIME_N (xS_0, F_AS_0,
+// F_SA_0); sdv out = ITER_K.  Returns status bits; xS is the clamped value.
+struct ReturnMap {
+  double xS, F_AS, F_SA;
+  uint32_t flags;
+};
+
+__host__ __device__ __forceinline__ bool flags_only(const Mat& m, double theta, double e_n,
+                                           double xS_0, double F_AS_0, double F_SA_0,
+                                           bool& kf_AS, bool& kf_SA,
+                                           double& F_AS, double& F_SA) {
+#pragma clang fp contract(off)
+  F_AS = loading(m, m.C_AS, theta, e_n, xS_0);
+  kf_AS = (F_AS > F_AS_0) && (F_AS - m.Rs_AS > 0.0) && (xS_0 < 1.0);
+  F_SA = loading(m, m.C_SA, theta, e_n, xS_0);
+  kf_SA = (F_SA < F_SA_0) && (F_SA - m.Rs_SA < 0.0) && (xS_0 > 0.0);
+  return kf_AS || kf_SA;
+}
+
+__host__ __device__ __forceinline__ ReturnMap return_map(const Mat& m, double theta, double e_n,
+                                                double xS_0, double F_AS_0, double F_SA_0) {
+#pragma clang fp contract(off)
+  ReturnMap r;
+  bool kf_AS, kf_SA;
+  double F_AS, F_SA;
+  flags_only(m, theta, e_n, xS_0, F_AS_0, F_SA_0, kf_AS, kf_SA, F_AS, F_SA);
+  r.flags = (kf_AS ? FLAG_KF_AS : 0u) | (kf_SA ? FLAG_KF_SA : 0u);
+  double xS = xS_0;
+  if (kf_AS || kf_SA) {
+    const double H_AS = kf_AS ? 1.0 : 0.0, H_SA = kf_SA ? 1.0 : 0.0;
+    double F_AS_dot = F_AS - F_AS_0, F_SA_dot = F_SA - F_SA_0;
+    double dh_AS = 0.0, dh_SA = 0.0;
+    double RHS_AS = (F_AS - m.Rf_AS) * dh_AS + H_AS * F_AS_dot * (1.0 - xS);
+    double RHS_SA = (F_SA - m.Rf_SA) * dh_SA - H_SA * F_SA_dot * xS;
+    double resi = RHS_AS * RHS_AS + RHS_SA * RHS_SA;   // squared first (:403)
+    int iter = 0;
+    while (iter < 65 && resi > 1.0e-8) {
+      ++iter;
+      double m11 = -m.G_1 * dh_AS - (H_AS * F_AS_dot + H_AS * (1.0 - xS) * m.G_1) + (F_AS - m.Rf_AS);
+      double m12 = -m.G_1 * dh_AS - (H_AS * F_AS_dot + H_AS * (1.0 - xS) * m.G_1);
+      double m21 = -m.G_1 * dh_SA - (H_SA * F_SA_dot - H_SA * xS * m.G_1);
+      double m22 = -m.G_1 * dh_SA - (H_SA * F_SA_dot - H_SA * xS * m.G_1) + (F_SA - m.Rf_SA);
+      double det = m11 * m22 - m12 * m21;
+      double r11 = m22 / det, r12 = -m12 / det, r21 = -m21 / det, r22 = m11 / det;
+      double sol1 = r11 * RHS_AS + r12 * RHS_SA;
+      double sol2 = r21 * RHS_AS + r22 * RHS_SA;
+      dh_AS = dh_AS - sol1;
+      dh_SA = dh_SA - sol2;
+      xS = xS + dh_SA + dh_AS;                         // running totals (:429-432)
+      F_AS = loading(m, m.C_AS, theta, e_n, xS);
+      F_AS_dot = F_AS - F_AS_0;
+      F_SA = loading(m, m.C_SA, theta, e_n, xS);
+      F_SA_dot = F_SA - F_SA_0;
+      RHS_AS = (F_AS - m.Rf_AS) * dh_AS + H_AS * F_AS_dot * (1.0 - xS);
+      RHS_SA = (F_SA - m.Rf_SA) * dh_SA - H_SA * F_SA_dot * xS;
+      resi = sqrt(RHS_AS * RHS_AS + RHS_SA * RHS_SA);
+    }
+    if (iter >= 65) {                                  // revert (:459-465)
+      r.flags |= FLAG_NEWTON_FAIL;
+      xS = xS_0;
+      F_AS = F_AS_0;
+      F_SA = F_SA_0;
+    }
+    r.flags |= (uint32_t)(iter & 0xff) << FLAG_ITER_SHIFT;
+  }
+  r.F_AS = F_AS;
+  r.F_SA = F_SA;
+  r.xS = fmax(fmin(1.0, xS), 0.0);                      // clamp after storing F's
+  return r;
+}
+
+// Kirchhoff stress (:487-491) as a symmetric tensor t00 t01 t02 t11 t12 t22.
+template <typename R>
+__host__ __device__ __forceinline__ void kirchhoff(const Mat& m, const Spectral<R>& s, double xS,
+                                          R (&t)[6]) {
+  R cvol = (R)(m.K * (s.theta - 3.0 * m.alfa * m.eL * xS));
+  R exS = (R)(m.eL * xS);
+  R twoG = (R)(2.0 * m.G);
+  R c0 = twoG * (s.e[0] - exS * s.d[0]);
+  R c1 = twoG * (s.e[1] - exS * s.d[1]);
+  R c2 = twoG * (s.e[2] - exS * s.d[2]);
+  const R* n = s.n;
+  t[0] = cvol + c0 * n[0] * n[0] + c1 * n[3] * n[3] + c2 * n[6] * n[6];
+  t[3] = cvol + c0 * n[1] * n[1] + c1 * n[4] * n[4] + c2 * n[7] * n[7];
+  t[5] = cvol + c0 * n[2] * n[2] + c1 * n[5] * n[5] + c2 * n[8] * n[8];
+  t[1] = c0 * n[0] * n[1] + c1 * n[3] * n[4] + c2 * n[6] * n[7];
+  t[2] = c0 * n[0] * n[2] + c1 * n[3] * n[5] + c2 * n[6] * n[8];
+  t[4] = c0 * n[1] * n[2] + c1 * n[4] * n[5] + c2 * n[7] * n[8];
+}
+
+// Column-major 3x3 inverse by cofactors (one reciprocal); returns det.
+template <typename R>
+__host__ __device__ __forceinline__ R inv3(const R (&M)[9], R (&Mi)[9]) {
+  R c00 = M[4] * M[8] - M[5] * M[7];
+  R c01 = M[2] * M[7] - M[1] * M[8];
+  R c02 = M[1] * M[5] - M[2] * M[4];
+  R det = M[0] * c00 + M[3] * c01 + M[6] * c02;
+  R r = R(1) / det;
+  Mi[0] = c00 * r;
+  Mi[1] = c01 * r;
+  Mi[2] = c02 * r;
+  Mi[3] = (M[5] * M[6] - M[3] * M[8]) * r;
+  Mi[4] = (M[0] * M[8] - M[2] * M[6]) * r;
+  Mi[5] = (M[2] * M[3] - M[0] * M[5]) * r;
+  Mi[6] = (M[3] * M[7] - M[4] * M[6]) * r;
+  Mi[7] = (M[1] * M[6] - M[0] * M[7]) * r;
+  Mi[8] = (M[0] * M[4] - M[1] * M[3]) * r;
+  return det;
+}
+
+// S = F^-1 tau F^-T (col-major result) from Finv and symmetric tau.
+template <typename R>
+__host__ __device__ __forceinline__ void pk2(const R (&Fi)[9], const R (&t)[6], R (&S)[9]) {
+  R T[9] = {t[0], t[1], t[2], t[1], t[3], t[4], t[2], t[4], t[5]};
+  R A[9];  // A = Finv * tau
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      A[i + 3 * j] = Fi[i] * T[3 * j] + Fi[i + 3] * T[1 + 3 * j] + Fi[i + 6] * T[2 + 3 * j];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      S[i + 3 * j] = A[i] * Fi[j] + A[i + 3] * Fi[j + 3] + A[i + 6] * Fi[j + 6];
+}
+
+}  // namespace sma

@@ -1,0 +1,138 @@
+"""Seeded synthetic problems of the explicit SMA step.
+
+Each builder returns a dict of plain numpy arrays -- the same arrays the
+device library (``sld_dev_create``) and the CPU oracle take -- plus run
+parameters.  Units: mm, MPa, tonne/mm^3, s.
+
+Material parameters follow the reference's parco_sld(1:18) layout
+(sld_reaphy.f90:477-490; meaning per sld_stress_model_160.f90:263-279):
+[E, nu, eL, sigma_s_AS, sigma_f_AS, C_AS, Tf_AS, Ts_AS, sigma_n_AS (unused),
+ sigma_s_SA, sigma_f_SA, C_SA, Tf_SA, Ts_SA, alpha, beta_AS, beta_SA, unused].
+"""
+import numpy as np
+
+from . import _lib
+
+# E = 60 GPa, nu = 0.33, eL = 0.05, AS start/finish 400/500 MPa,
+# SA start/finish 250/150 MPa; temperature slopes zero (T is inert).
+NITINOL = np.array([60000.0, 0.33, 0.05, 400.0, 500.0, 0, 0, 0, 0,
+                    250.0, 150.0, 0, 0, 0, 0, 1.0, 1.0, 0], dtype=np.float64)
+RHO_NITINOL = 6.45e-9   # 6450 kg/m^3 in tonne/mm^3
+
+
+def box_mesh(nx, ny, nz, lx, ly, lz):
+    """Structured HEX08 box through the library's host helper."""
+    npoin = (nx + 1) * (ny + 1) * (nz + 1)
+    coord = np.zeros((npoin, 3))
+    lnods = np.zeros((nx * ny * nz, 8), dtype=np.int32)
+    L = _lib.lib()
+    _lib.check(L.sld_mesh_box(nx, ny, nz, lx, ly, lz, coord.ctypes.data, lnods.ctypes.data),
+               "sld_mesh_box")
+    return coord, lnods
+
+
+def box_mesh_np(nx, ny, nz, lx, ly, lz):
+    """Same numbering as sld_mesh_box, in numpy (no library needed)."""
+    i, j, k = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), np.arange(nz + 1), indexing="ij")
+    px, py = nx + 1, ny + 1
+    idx = lambda a, b, c: a + px * (b + py * c)
+    coord = np.zeros(((nx + 1) * (ny + 1) * (nz + 1), 3))
+    p = idx(i, j, k).ravel()
+    coord[p, 0] = lx * i.ravel() / nx
+    coord[p, 1] = ly * j.ravel() / ny
+    coord[p, 2] = lz * k.ravel() / nz
+    ei, ej, ek = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    e = (ei + nx * (ej + ny * ek)).ravel()
+    ei, ej, ek = ei.ravel(), ej.ravel(), ek.ravel()
+    lnods = np.zeros((nx * ny * nz, 8), dtype=np.int32)
+    lnods[e] = np.stack([idx(ei, ej, ek), idx(ei + 1, ej, ek), idx(ei + 1, ej + 1, ek),
+                         idx(ei, ej + 1, ek), idx(ei, ej, ek + 1), idx(ei + 1, ej, ek + 1),
+                         idx(ei + 1, ej + 1, ek + 1), idx(ei, ej + 1, ek + 1)], axis=1)
+    return coord, lnods
+
+
+def smooth_table(t_end, values):
+    """DISCRETE,SMOOTH load table: equally spaced (t, v) knots over [0, t_end].
+    values: [npts][3] per-component amplitudes.  Rows (v_x, v_y, v_z, t)."""
+    values = np.asarray(values, dtype=np.float64)
+    n = values.shape[0]
+    tab = np.zeros((n, 4))
+    tab[:, :3] = values
+    tab[:, 3] = np.linspace(0.0, t_end, n)
+    return tab
+
+
+def bar_problem(nx=10, ny=2, nz=2, lx=10.0, ly=2.0, lz=2.0, strain=0.06,
+                nsteps=2000, safety=0.5, props=NITINOL, rho=RHO_NITINOL, use_lib=False):
+    """Uniaxial SMA bar (displacement-controlled ramp to `strain`, then unload).
+
+    BCs as tests/uniaxial_test/sm154_d1C.sld.dat:80-107: u_x = 0 on x = 0,
+    symmetry u_y = 0 on y = 0 and u_z = 0 on z = 0, prescribed u_x on x = lx
+    through a DISCRETE,SMOOTH function 0 -> strain*lx -> 0 over nsteps steps.
+    """
+    coord, lnods = (box_mesh if use_lib else box_mesh_np)(nx, ny, nz, lx, ly, lz)
+    npoin = coord.shape[0]
+    fix = np.zeros(npoin, dtype=np.uint8)
+    x, y, z = coord[:, 0], coord[:, 1], coord[:, 2]
+    tol = 1e-9 * max(lx, ly, lz)
+    fix[np.abs(x) < tol] |= 1
+    fix[np.abs(y) < tol] |= 2
+    fix[np.abs(z) < tol] |= 4
+    tip = np.abs(x - lx) < tol
+    fix[tip] |= 1
+    func_of_node = np.full(npoin, -1, dtype=np.int32)
+    func_of_node[tip] = 0
+    prob = dict(coord=coord, lnods=lnods, props=np.asarray(props, np.float64).reshape(-1, 18),
+                rho=np.array([rho], np.float64), fixmask=fix, func_of_node=func_of_node)
+    # dt from the critical step of the mesh (uniform box: h = min edge)
+    E, nu = prob["props"][0, 0], prob["props"][0, 1]
+    lam = nu * E / ((1 + nu) * (1 - 2 * nu))
+    mu = E / (2 * (1 + nu))
+    c = np.sqrt((lam + 2 * mu) / rho)
+    h = min(lx / nx, ly / ny, lz / nz)
+    dt = safety * h / c
+    t_end = nsteps * dt
+    tab = smooth_table(t_end, [[0, 0, 0], [strain * lx, 0, 0], [0, 0, 0]])
+    prob.update(func_table=tab[None], func_npts=np.array([3], np.int32),
+                func_smooth=np.array([1], np.int32), fext0=None, fext_func=-1)
+    prob["run"] = dict(dt=dt, nsteps=nsteps, safety=safety, scheme=0, phi=1.033)
+    prob["name"] = f"bar {nx}x{ny}x{nz} hex8 over {lx}x{ly}x{lz} mm"
+    return prob
+
+
+def cantilever_problem(nx=400, ny=20, nz=20, lx=50.0, ly=2.5, lz=2.5, traction=500.0,
+                       nsteps=5000, safety=0.5, seed=1, props=NITINOL, rho=RHO_NITINOL,
+                       E_noise=0.02, use_lib=False):
+    """Cantilever clamped at x = 0 with a tip traction in y on x = lx that is
+    ramped up and released (DISCRETE,SMOOTH 0 -> 1 -> 0 over nsteps).  The
+    per-element Young's modulus carries seeded uniform noise of +-E_noise."""
+    coord, lnods = (box_mesh if use_lib else box_mesh_np)(nx, ny, nz, lx, ly, lz)
+    npoin, nelem = coord.shape[0], lnods.shape[0]
+    tol = 1e-9 * lx
+    fix = np.zeros(npoin, dtype=np.uint8)
+    fix[np.abs(coord[:, 0]) < tol] = 7
+    func_of_node = np.full(npoin, -1, dtype=np.int32)
+    # tip traction -> nodal forces by tributary area on the x = lx face
+    tip = np.abs(coord[:, 0] - lx) < tol
+    hy, hz = ly / ny, lz / nz
+    wy = np.where((np.abs(coord[:, 1]) < tol) | (np.abs(coord[:, 1] - ly) < tol), 0.5, 1.0)
+    wz = np.where((np.abs(coord[:, 2]) < tol) | (np.abs(coord[:, 2] - lz) < tol), 0.5, 1.0)
+    fext0 = np.zeros((npoin, 3))
+    fext0[tip, 1] = traction * hy * hz * (wy * wz)[tip]
+    rng = np.random.default_rng(seed)
+    P = np.asarray(props, np.float64).reshape(-1, 18)
+    E_of_elem = P[0, 0] * (1.0 + E_noise * rng.uniform(-1.0, 1.0, nelem))
+    prob = dict(coord=coord, lnods=lnods, props=P, rho=np.array([rho], np.float64),
+                fixmask=fix, func_of_node=func_of_node, E_of_elem=E_of_elem, fext0=fext0)
+    E, nu = E_of_elem.max(), P[0, 1]
+    lam = nu * E / ((1 + nu) * (1 - 2 * nu))
+    mu = E / (2 * (1 + nu))
+    c = np.sqrt((lam + 2 * mu) / rho)
+    dt = safety * min(lx / nx, hy, hz) / c
+    t_end = nsteps * dt
+    tab = smooth_table(t_end, [[0, 0, 0], [1, 0, 0], [0, 0, 0]])
+    prob.update(func_table=tab[None], func_npts=np.array([3], np.int32),
+                func_smooth=np.array([1], np.int32), fext_func=0)
+    prob["run"] = dict(dt=dt, nsteps=nsteps, safety=safety, scheme=0, phi=1.033)
+    prob["name"] = f"cantilever {nx}x{ny}x{nz} hex8 over {lx}x{ly}x{lz} mm"
+    return prob

@@ -1,0 +1,184 @@
+/*
+ * alya_sma.h -- C ABI of the MI355X explicit solid-mechanics step with the
+ * superelastic SMA law (Alya solidz, stress model 160 "NITINOL").
+ *
+ * The reference exposes Fortran module procedures only; every entry point
+ * below names the reference procedure(s) it replaces.  Conventions:
+ *   - plain pointers and sizes, no C++ or torch types;
+ *   - int return status: 0 = ok, < 0 = error (text from sma_last_error());
+ *   - 3x3 tensors are column-major per point (k = row + 3*col), as Fortran
+ *     stores gpgdi(3,3,pgaus) and gpstr(3,3,pgaus);
+ *   - point arrays are SoA: component k of point i at [k*n + i];
+ *   - history per Gauss point is (xS, F_AS, F_SA), the reference's
+ *     svegm_sld(ielem)%a(1:3, igaus, :).
+ */
+#ifndef ALYA_SMA_H
+#define ALYA_SMA_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Per-Gauss-point status bits. */
+#define SMA_FLAG_KF_AS          0x01u  /* austenite->martensite branch (sld_stress_model_160.f90:351) */
+#define SMA_FLAG_KF_SA          0x02u  /* martensite->austenite branch (:368) */
+#define SMA_FLAG_NEWTON_FAIL    0x04u  /* 65-iteration cap, history reverted (:459-465) */
+#define SMA_FLAG_EIGEN_FAIL     0x08u  /* DSYEVQ3 30-iteration bail-out (dsyevq3.f90:82-84) */
+#define SMA_FLAG_DETF_NONPOS    0x10u  /* det F <= 0 (sld_elmpre.f90:229-240) */
+#define SMA_FLAG_SURR_USED      0x20u  /* surrogate answered this point */
+#define SMA_FLAG_SURR_FALLBACK  0x40u  /* surrogate check failed, exact law used */
+#define SMA_FLAG_BAD_INPUT      0x80u  /* material index out of range (material 0 used) */
+#define SMA_FLAG_ITER_SHIFT     8      /* Newton iterations in bits 8..15 */
+
+/* Status codes. */
+#define SMA_OK          0
+#define SMA_ERR_ARG    -1
+#define SMA_ERR_HIP    -2
+#define SMA_ERR_STATE  -3
+
+/* Text of the last error on the calling thread ("" if none). */
+const char *sma_last_error(void);
+
+/* ------------------------------------------------------------------------
+ * Point-wise constitutive law.
+ * Replaces eval_sld_stress_model_160(pgaus, props, gpgdi, gptmp, gpsdv,
+ *   gpstr, gpdds, flagt=0)  (sld_stress_model_160.f90:36-48, called from
+ *   sld_builtin_materials.f90:327-335).
+ * props [n_mat][18]: the reference's parco_sld(1:18, pmate) layout.
+ * mat_of_gp [n_gp] may be NULL (material 0); E_of_gp may be NULL (else it
+ *   overrides props[0] per point).  T is accepted and ignored, as the
+ *   reference hard-codes gp_tmp = 0 (:105).
+ * F [9][n_gp] in; sdv_old [3][n_gp] = TIME_N in; sdv_new [3][n_gp] = ITER_K
+ *   out; S [9][n_gp] second Piola-Kirchhoff out; flags [n_gp] out (may be NULL).
+ * precision: 0 = FP64, 1 = FP32 spectral/stress with FP64 return map.
+ * Host pointers; synchronous.
+ * ------------------------------------------------------------------------ */
+int sma160_eval(int64_t n_gp, int32_t n_mat, const double *props,
+                const int32_t *mat_of_gp, const double *E_of_gp,
+                const double *F, const double *T, const double *sdv_old,
+                double *sdv_new, double *S, uint32_t *flags, int32_t precision);
+
+/* Same as sma160_eval with every array (props included) a device pointer,
+ * enqueued on `stream` (a hipStream_t, NULL = default stream). */
+int sma160_eval_device(int64_t n_gp, int32_t n_mat, const double *props,
+                       const int32_t *mat_of_gp, const double *E_of_gp,
+                       const double *F, const double *T, const double *sdv_old,
+                       double *sdv_new, double *S, uint32_t *flags,
+                       int32_t precision, void *stream);
+
+/* ------------------------------------------------------------------------
+ * Explicit step on the device.
+ * Replaces the per-step call chain sld_explicit / sld_explicit_tw
+ *   (mod_sld_solution_methods.f90:101-223, :385-574) -> sld_matrix
+ *   (sld_matrix.f90:29-327) -> sld_elmope (sld_elmope.f90:53-99) ->
+ *   sld_element_operations (sld_element_operations.f90:24-407) ->
+ *   solver_solve/solric local mass (solric.f90:294-330), plus
+ *   sld_updbcs(ITASK_BEGSTE) (sld_updbcs.f90:79-114) and
+ *   sld_updunk(ITASK_ENDSTE) (sld_updunk.f90:149-186).
+ * ------------------------------------------------------------------------ */
+typedef struct sld_dev sld_dev;
+
+typedef struct {
+  int64_t npoin, nelem;
+  const int32_t *lnods;        /* [nelem][8] 0-based, HEX08 node order of mod_lagrange.f90:946-953 */
+  const double *coord;         /* [npoin][3] reference coordinates (coord(ndime,npoin)) */
+} sld_mesh_desc;
+
+typedef struct {
+  int32_t nmat;
+  const double *props;         /* [nmat][18] parco_sld(1:18, imate) */
+  const double *rho;           /* [nmat] densi_sld(1, imate) */
+  const int32_t *mat_of_elem;  /* [nelem] lmate_sld, or NULL */
+  const double *E_of_elem;     /* [nelem] per-element Young's modulus, or NULL */
+} sld_material_desc;
+
+typedef struct {
+  const uint8_t *fixmask;      /* [npoin] bit d = kfl_fixno_sld(d+1, ipoin) == 1 */
+  const int32_t *func_of_node; /* [npoin] load function index or -1 (kfl_funno_sld) */
+  int32_t nfunc, func_maxpts;
+  const int32_t *func_npts;    /* [nfunc] mtloa_sld */
+  const int32_t *func_smooth;  /* [nfunc] 1 = DISCRETE,SMOOTH (kfl_funty_sld(2,:)) */
+  const double *func_table;    /* [nfunc][func_maxpts][4] rows (v_x, v_y, v_z, t) = tload_sld */
+  const double *fext0;         /* [npoin][3] nodal force pattern, or NULL */
+  int32_t fext_func;           /* function scaling fext0 (component x), -1 = none */
+} sld_bc_desc;
+
+/* Builds the device state: validates indices, colours the elements
+ * (greedy, deterministic), uploads SoA arrays, assembles the lumped mass.
+ * Allocates on the current HIP device. */
+int sld_dev_create(const sld_mesh_desc *mesh, const sld_material_desc *mat,
+                   const sld_bc_desc *bc, sld_dev **out);
+
+/* Precision mode: 0 = FP64 (plain), 1 = FP32 kinematics/assembly with FP64
+ * return map and FP64 u/v/a accumulators, 2 = mode 1 + MLP surrogate
+ * (requires sld_dev_set_surrogate). */
+#define SLD_MODE_FP64       0
+#define SLD_MODE_FP32       1
+#define SLD_MODE_SURROGATE  2
+int sld_dev_set_mode(sld_dev *dev, int32_t mode);
+
+/* Store PK2 stress and status flags per Gauss point every step (for
+ * checking); off by default. */
+int sld_dev_set_store(sld_dev *dev, int32_t store);
+
+/* Advance nsteps explicit steps of size dt.  scheme 0 = central difference
+ * (sld_explicit), 1 = Tchamwa-Wielgosz with parameter phi (sld_explicit_tw).
+ * Asynchronous on the handle's stream; steps are replayed from a hipGraph. */
+int sld_dev_step(sld_dev *dev, double dt, int32_t scheme, double phi, int64_t nsteps);
+
+/* Critical time step min_e h_min(e)/c(e) (sld_updtss.f90:119-162). */
+double sld_dev_critical_dt(const sld_dev *dev);
+
+/* Field copies (synchronise the stream).  Element fields come back in the
+ * caller's element order. */
+#define SLD_FIELD_U        0   /* [npoin][3] displacement */
+#define SLD_FIELD_V        1   /* [npoin][3] velocity */
+#define SLD_FIELD_A        2   /* [npoin][3] acceleration */
+#define SLD_FIELD_FINT     3   /* [npoin][3] assembled internal force */
+#define SLD_FIELD_MASS     4   /* [npoin] lumped mass */
+#define SLD_FIELD_SDV      5   /* [nelem][8][3] history ITER_K */
+#define SLD_FIELD_STRESS   6   /* [nelem][8][9] PK2 (needs sld_dev_set_store) */
+#define SLD_FIELD_FLAGS    7   /* [nelem][8] uint32 (needs sld_dev_set_store) */
+#define SLD_FIELD_TIME     8   /* [1] current time */
+#define SLD_FIELD_COUNTERS 9   /* [8] uint64 Gauss-point counts per flag bit 0..6 */
+int sld_dev_get(sld_dev *dev, int32_t field, void *host, int64_t count);
+
+/* Overwrite state (restart, sld_restar.f90:63-65): fields U, V, A, SDV
+ * (both history slots) and TIME. */
+int sld_dev_set(sld_dev *dev, int32_t field, const void *host, int64_t count);
+
+/* Colouring used by the handle: colour_ptr [ncolor+1] and elem_order
+ * [nelem] (caller element index of the i-th processed element). */
+int sld_dev_coloring(const sld_dev *dev, int32_t *ncolor, int32_t *color_ptr,
+                     int32_t max_colors, int32_t *elem_order);
+
+/* Elements per block for the element kernel (8, 16 or 32; autotuned). */
+int sld_dev_set_tile(sld_dev *dev, int32_t elems_per_block, int32_t xcd_remap);
+
+/* Stream the handle enqueues on (a hipStream_t). */
+void *sld_dev_stream(sld_dev *dev);
+
+/* Wait for all work of the handle. */
+int sld_dev_sync(sld_dev *dev);
+
+int sld_dev_destroy(sld_dev *dev);
+
+/* ------------------------------------------------------------------------
+ * Host helpers (no GPU needed).
+ * ------------------------------------------------------------------------ */
+/* Structured box of nx*ny*nz HEX08 over [0,lx]x[0,ly]x[0,lz]; node (i,j,k)
+ * has index i + (nx+1)*(j + (ny+1)*k); element (i,j,k) index
+ * i + nx*(j + ny*k).  coord [(nx+1)(ny+1)(nz+1)][3], lnods [nx*ny*nz][8]. */
+int sld_mesh_box(int32_t nx, int32_t ny, int32_t nz, double lx, double ly, double lz,
+                 double *coord, int32_t *lnods);
+
+/* Greedy element colouring (no two elements of one colour share a node),
+ * elements visited in index order.  color [nelem] out; returns ncolor or < 0. */
+int32_t sld_color_elements(int64_t nelem, int64_t npoin, const int32_t *lnods,
+                           int32_t *color);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ALYA_SMA_H */

@@ -1,0 +1,125 @@
+/*
+ * TEST INFRASTRUCTURE ONLY -- the CPU oracle.
+ *
+ * A plain-C FP64 restatement of the reference's explicit solid-mechanics
+ * step with the superelastic SMA law (Alya solidz, stress model 160).  It is
+ * the checker for the HIP path and the CPU baseline timed beside it; only
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.
+ * Nothing in the product library links or calls it.
+ *
+ * Functions and what pins each (see tests/test_oracle_*.py):
+ *   orc_sm160_eval        sld_stress_model_160.f90:36-195 + :212-614,
+ *                         dsyevq3.f90, dsytrd3.f90, eigsrt.f90.
+ *                         Pinned by golden vectors written by the reference's
+ *                         own compiled law (tests/golden/), by the Hencky
+ *                         closed form below threshold and at saturation,
+ *                         frame indifference and the Newton-fail revert.
+ *   orc_hex8_tables       mod_lagrange.f90:935-1000, mod_quadrature_gauss.f90:
+ *                         2320-2321,2372-2385.  Pinned by golden vectors from
+ *                         the reference's shape3/gauss_qua and partition of unity.
+ *   orc_hex8_kinematics   jacobi.f90 (ndime==3), elmca2.f90:105-111,
+ *                         sld_elmpre.f90:149-162,219-226, invmtx.f90.
+ *                         Pinned by golden vectors from the reference's jacobi/
+ *                         invmtx and by exact affine fields.
+ *   orc_sld_*             element RHS, lumped mass, integrators, load tables:
+ *                         sld_elmmat.f90:115-209, sld_builtin_materials.f90:
+ *                         355-371, mod_sld_solution_methods.f90:101-223,
+ *                         385-574, solric.f90:294-330, sld_funbou.f90:95-136.
+ *                         Pinned by patch tests (rigid motion, constant
+ *                         stress), mass = rho*V, virtual work, the closed-form
+ *                         harmonic oscillator and the smooth-step formula.
+ */
+#ifndef ALYA_SMA_ORACLE_H
+#define ALYA_SMA_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Per-point status bits; identical values to include/alya_sma.h. */
+#define ORC_FLAG_KF_AS        0x01u
+#define ORC_FLAG_KF_SA        0x02u
+#define ORC_FLAG_NEWTON_FAIL  0x04u
+#define ORC_FLAG_EIGEN_FAIL   0x08u
+#define ORC_FLAG_DETF_NONPOS  0x10u
+#define ORC_FLAG_ITER_SHIFT   8        /* Newton iterations in bits 8..15 */
+
+/* Point-wise SMA law on SoA arrays (component k of point i at [k*n + i]).
+ * props: [n_mat][18]; mat_of_gp may be NULL (material 0 everywhere).
+ * E_of_gp may be NULL; when given it overrides props[0] per point.
+ * F and S are 3x3 column-major per point (k = row + 3*col).
+ * sdv_old/sdv_new: (xS, F_AS, F_SA).  Returns 0. */
+int orc_sm160_eval(int64_t n, const double *props, const int32_t *mat_of_gp,
+                   const double *E_of_gp, const double *F, const double *T,
+                   const double *sdv_old, double *sdv_new, double *S,
+                   uint32_t *flags);
+
+/* Single point, AoS, also returns the Kirchhoff stress tau (col-major). */
+uint32_t orc_sm160_point(const double props[18], const double F[9],
+                         const double sdv_old[3], double sdv_new[3],
+                         double S[9], double tau[9]);
+
+/* Kopp's QL eigen-solver restated (A read-only, col-major); returns 0 on
+ * convergence, 1 on the 30-iteration bail-out.  eigsrt descending. */
+int orc_dsyevq3(const double *A, double *Q, double w[3]);
+void orc_eigsrt(double d[3], double *V);
+
+/* HEX08 tables at the 2x2x2 Gauss points: posgp[8][3], weigp[8],
+ * shapf[8 gp][8 node], deriv[8 gp][8 node][3]. */
+void orc_hex8_tables(double *posgp, double *weigp, double *shapf, double *deriv);
+
+/* Per-element kinematics at the 8 GPs.  elcod/eldis: [8 node][3].
+ * Outputs gpcar[8 gp][8 node][3], gpvol[8], F[8 gp][9 col-major], detF[8]. */
+void orc_hex8_kinematics(const double *elcod, const double *eldis,
+                         double *gpcar, double *gpvol, double *F, double *detF);
+
+/* ------------------------------------------------------------------ */
+/* Whole explicit step                                                  */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    int64_t npoin, nelem;
+    const int32_t *lnods;      /* [nelem][8], 0-based, reference node order */
+    const double  *coord;      /* [npoin][3] reference coordinates          */
+    int32_t nmat;
+    const double  *props;      /* [nmat][18] SMA parameters                 */
+    const double  *rho;        /* [nmat] density                            */
+    const int32_t *mat_of_elem;/* [nelem] or NULL                           */
+    const double  *E_of_elem;  /* [nelem] Young's modulus override or NULL  */
+    const uint8_t *fixmask;    /* [npoin] bit d set = dof d prescribed       */
+    const int32_t *func_of_node; /* [npoin] Dirichlet function id or -1     */
+    int32_t nfunc;
+    const int32_t *func_npts;  /* [nfunc]                                   */
+    const int32_t *func_smooth;/* [nfunc] 1 = DISCRETE,SMOOTH, 0 = linear   */
+    const double  *func_table; /* [nfunc][maxpts][4] (v_x, v_y, v_z, t)      */
+    int32_t func_maxpts;
+    const double  *fext0;      /* [npoin][3] nodal force pattern or NULL     */
+    int32_t fext_func;         /* load function scaling fext0 (component x)  */
+    const int32_t *elem_order; /* [nelem] processing order or NULL           */
+    const int32_t *color_ptr;  /* [ncolor+1] ranges into elem_order or NULL  */
+    int32_t ncolor;
+} orc_model;
+
+typedef struct {
+    double *u, *v, *a;         /* [npoin][3] (ITER_K after a step)           */
+    double *fint;              /* [npoin][3] assembled internal force        */
+    double *rhs;               /* [npoin][3] assembled RHS = fext - fint     */
+    double *mass;              /* [npoin] lumped mass                        */
+    double *sdv;               /* [2][nelem][8][3]: [0]=ITER_K, [1]=TIME_N   */
+    double *S;                 /* [nelem][8][9] PK2 of the last step or NULL */
+    uint32_t *flags;           /* [nelem][8] or NULL                         */
+    double t;                  /* current time                               */
+    int64_t nsteps;
+} orc_state;
+
+int orc_sld_mass(const orc_model *m, orc_state *s);
+double orc_sld_critical_dt(const orc_model *m);
+/* One explicit step. scheme 0 = central difference, 1 = Tchamwa-Wielgosz. */
+int orc_sld_step(const orc_model *m, orc_state *s, double dt, int scheme, double phi);
+/* Evaluate load function f at time t (sld_funbou.f90 itask 2, component c). */
+double orc_funbou(const orc_model *m, int32_t f, int c, double t);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,274 @@
+"""TEST INFRASTRUCTURE ONLY -- ctypes bindings of the C oracle
+(oracle/liboracle_sma.so, built by ``make -C oracle``) and a runner for the
+reference harnesses under oracle/_ref (built by ``make -C oracle ref`` in a
+container that holds the reference sources).
+
+Every function here is a checker.  Nothing in ``alya_sma_amd`` imports it.
+"""
+import ctypes as C
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle_sma.so")
+REF_DIR = os.path.join(HERE, "_ref")
+
+FLAG_KF_AS = 0x01
+FLAG_KF_SA = 0x02
+FLAG_NEWTON_FAIL = 0x04
+FLAG_EIGEN_FAIL = 0x08
+FLAG_DETF_NONPOS = 0x10
+ITER_SHIFT = 8
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = C.CDLL(LIB_PATH)
+        d, i64, p = C.c_double, C.c_int64, C.c_void_p
+        _lib.orc_sm160_eval.argtypes = [i64, p, p, p, p, p, p, p, p, p]
+        _lib.orc_sm160_point.argtypes = [p, p, p, p, p, p]
+        _lib.orc_sm160_point.restype = C.c_uint32
+        _lib.orc_dsyevq3.argtypes = [p, p, p]
+        _lib.orc_eigsrt.argtypes = [p, p]
+        _lib.orc_hex8_tables.argtypes = [p, p, p, p]
+        _lib.orc_hex8_kinematics.argtypes = [p, p, p, p, p, p]
+        _lib.orc_sld_mass.argtypes = [p, p]
+        _lib.orc_sld_critical_dt.argtypes = [p]
+        _lib.orc_sld_critical_dt.restype = d
+        _lib.orc_sld_step.argtypes = [p, p, d, C.c_int, d]
+        _lib.orc_funbou.argtypes = [p, C.c_int32, C.c_int, d]
+        _lib.orc_funbou.restype = d
+    return _lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def sm160_eval(props, F, sdv_old, mat_of_gp=None, E_of_gp=None):
+    """SoA law evaluation.  props [nmat,18]; F [9,n] col-major per point;
+    sdv_old [3,n].  Returns (sdv_new [3,n], S [9,n], flags [n])."""
+    props = np.ascontiguousarray(props, dtype=np.float64)
+    F = np.ascontiguousarray(F, dtype=np.float64)
+    sdv_old = np.ascontiguousarray(sdv_old, dtype=np.float64)
+    n = F.shape[1]
+    sdv_new = np.zeros((3, n))
+    S = np.zeros((9, n))
+    flags = np.zeros(n, dtype=np.uint32)
+    mat = None if mat_of_gp is None else np.ascontiguousarray(mat_of_gp, dtype=np.int32)
+    E = None if E_of_gp is None else np.ascontiguousarray(E_of_gp, dtype=np.float64)
+    lib().orc_sm160_eval(n, _ptr(props), _ptr(mat), _ptr(E), _ptr(F), None,
+                         _ptr(sdv_old), _ptr(sdv_new), _ptr(S), _ptr(flags))
+    return sdv_new, S, flags
+
+
+def sm160_point(props, F, sdv_old):
+    """One point, F as a 3x3 array.  Returns (sdv_new, S 3x3, tau 3x3, flags)."""
+    props = np.ascontiguousarray(props, dtype=np.float64)
+    Fc = np.ascontiguousarray(np.asarray(F, dtype=np.float64).reshape(3, 3).T.ravel())
+    so = np.ascontiguousarray(sdv_old, dtype=np.float64)
+    sn = np.zeros(3)
+    S = np.zeros(9)
+    tau = np.zeros(9)
+    fl = lib().orc_sm160_point(_ptr(props), _ptr(Fc), _ptr(so), _ptr(sn), _ptr(S), _ptr(tau))
+    return sn, S.reshape(3, 3).T.copy(), tau.reshape(3, 3).T.copy(), int(fl)
+
+
+def dsyevq3(A):
+    Ac = np.ascontiguousarray(np.asarray(A, dtype=np.float64).T.ravel())
+    Q = np.zeros(9)
+    w = np.zeros(3)
+    rc = lib().orc_dsyevq3(_ptr(Ac), _ptr(Q), _ptr(w))
+    lib().orc_eigsrt(_ptr(w), _ptr(Q))
+    return rc, w, Q.reshape(3, 3).T.copy()
+
+
+def hex8_tables():
+    posgp = np.zeros((8, 3)); weigp = np.zeros(8)
+    shapf = np.zeros((8, 8)); deriv = np.zeros((8, 8, 3))
+    lib().orc_hex8_tables(_ptr(posgp), _ptr(weigp), _ptr(shapf), _ptr(deriv))
+    return posgp, weigp, shapf, deriv
+
+
+def hex8_kinematics(elcod, eldis):
+    """elcod, eldis [8,3].  Returns gpcar [8gp,8node,3], gpvol [8],
+    F [8,3,3] (row, col), detF [8]."""
+    elcod = np.ascontiguousarray(elcod, dtype=np.float64)
+    eldis = np.ascontiguousarray(eldis, dtype=np.float64)
+    gpcar = np.zeros((8, 8, 3)); gpvol = np.zeros(8); F = np.zeros((8, 9)); detF = np.zeros(8)
+    lib().orc_hex8_kinematics(_ptr(elcod), _ptr(eldis), _ptr(gpcar), _ptr(gpvol), _ptr(F), _ptr(detF))
+    return gpcar, gpvol, F.reshape(8, 3, 3).transpose(0, 2, 1).copy(), detF
+
+
+# ---------------------------------------------------------------- step oracle
+class _Model(C.Structure):
+    _fields_ = [("npoin", C.c_int64), ("nelem", C.c_int64),
+                ("lnods", C.c_void_p), ("coord", C.c_void_p),
+                ("nmat", C.c_int32), ("props", C.c_void_p), ("rho", C.c_void_p),
+                ("mat_of_elem", C.c_void_p), ("E_of_elem", C.c_void_p),
+                ("fixmask", C.c_void_p), ("func_of_node", C.c_void_p),
+                ("nfunc", C.c_int32), ("func_npts", C.c_void_p), ("func_smooth", C.c_void_p),
+                ("func_table", C.c_void_p), ("func_maxpts", C.c_int32),
+                ("fext0", C.c_void_p), ("fext_func", C.c_int32),
+                ("elem_order", C.c_void_p), ("color_ptr", C.c_void_p), ("ncolor", C.c_int32)]
+
+
+class _State(C.Structure):
+    _fields_ = [("u", C.c_void_p), ("v", C.c_void_p), ("a", C.c_void_p),
+                ("fint", C.c_void_p), ("rhs", C.c_void_p), ("mass", C.c_void_p),
+                ("sdv", C.c_void_p), ("S", C.c_void_p), ("flags", C.c_void_p),
+                ("t", C.c_double), ("nsteps", C.c_int64)]
+
+
+class StepOracle:
+    """FP64 CPU oracle of the explicit step on a problem described by the
+    same plain arrays the device library takes (see alya_sma_amd.problem)."""
+
+    def __init__(self, prob, elem_order=None, color_ptr=None, keep_stress=True):
+        self._keep = []
+        k = self._keep.append
+
+        def arr(x, dt):
+            if x is None:
+                return None
+            a = np.ascontiguousarray(x, dtype=dt)
+            k(a)
+            return a
+
+        m = _Model()
+        self.npoin = int(prob["coord"].shape[0])
+        self.nelem = int(prob["lnods"].shape[0])
+        m.npoin, m.nelem = self.npoin, self.nelem
+        m.lnods = _ptr(arr(prob["lnods"], np.int32))
+        m.coord = _ptr(arr(prob["coord"], np.float64))
+        props = arr(prob["props"], np.float64).reshape(-1, 18)
+        m.nmat = props.shape[0]
+        m.props = _ptr(props)
+        m.rho = _ptr(arr(prob["rho"], np.float64))
+        m.mat_of_elem = _ptr(arr(prob.get("mat_of_elem"), np.int32))
+        m.E_of_elem = _ptr(arr(prob.get("E_of_elem"), np.float64))
+        m.fixmask = _ptr(arr(prob["fixmask"], np.uint8))
+        m.func_of_node = _ptr(arr(prob["func_of_node"], np.int32))
+        ftab = arr(prob["func_table"], np.float64)
+        m.nfunc = ftab.shape[0]
+        m.func_maxpts = ftab.shape[1]
+        m.func_table = _ptr(ftab)
+        m.func_npts = _ptr(arr(prob["func_npts"], np.int32))
+        m.func_smooth = _ptr(arr(prob["func_smooth"], np.int32))
+        m.fext0 = _ptr(arr(prob.get("fext0"), np.float64))
+        m.fext_func = int(prob.get("fext_func", 0))
+        m.elem_order = _ptr(arr(elem_order, np.int32))
+        m.color_ptr = _ptr(arr(color_ptr, np.int32))
+        m.ncolor = 0 if color_ptr is None else len(color_ptr) - 1
+        self.m = m
+        n, e = self.npoin, self.nelem
+        self.u = np.zeros((n, 3)); self.v = np.zeros((n, 3)); self.a = np.zeros((n, 3))
+        self.fint = np.zeros((n, 3)); self.rhs = np.zeros((n, 3)); self.mass = np.zeros(n)
+        self.sdv = np.zeros((2, e, 8, 3))
+        self.S = np.zeros((e, 8, 9)) if keep_stress else None
+        self.flags = np.zeros((e, 8), dtype=np.uint32)
+        s = _State()
+        s.u, s.v, s.a = _ptr(self.u), _ptr(self.v), _ptr(self.a)
+        s.fint, s.rhs, s.mass = _ptr(self.fint), _ptr(self.rhs), _ptr(self.mass)
+        s.sdv, s.S, s.flags = _ptr(self.sdv), _ptr(self.S), _ptr(self.flags)
+        s.t, s.nsteps = 0.0, 0
+        self.s = s
+        lib().orc_sld_mass(C.byref(self.m), C.byref(self.s))
+
+    def critical_dt(self):
+        return lib().orc_sld_critical_dt(C.byref(self.m))
+
+    def step(self, dt, scheme=0, phi=1.033, n=1):
+        for _ in range(n):
+            lib().orc_sld_step(C.byref(self.m), C.byref(self.s), dt, scheme, phi)
+
+    @property
+    def t(self):
+        return self.s.t
+
+    def funbou(self, f, c, t):
+        return lib().orc_funbou(C.byref(self.m), f, c, t)
+
+
+# ------------------------------------------------------- reference harnesses
+def ref_available():
+    return os.path.exists(os.path.join(REF_DIR, "ref_sm160"))
+
+
+def ref_sm160(props_per_point, F, sdv_old):
+    """Run the reference's compiled law.  props_per_point [n,18], F [9,n]
+    col-major, sdv_old [3,n].  Returns (sdv_new, S, warn) where warn holds
+    per point bit 0 = 'WARNING 1' (both flags), bit 1 = 'WARNING 2' (Newton
+    iteration cap), bit 2 = 'DSYEVQ3: No convergence'."""
+    n = F.shape[1]
+    with tempfile.TemporaryDirectory() as td:
+        fin = os.path.join(td, "in.bin")
+        fout = os.path.join(td, "out.bin")
+        with open(fin, "wb") as f:
+            np.int64(n).tofile(f)
+            np.ascontiguousarray(props_per_point, dtype=np.float64).tofile(f)
+            np.ascontiguousarray(F.T, dtype=np.float64).tofile(f)
+            np.ascontiguousarray(sdv_old.T, dtype=np.float64).tofile(f)
+        out = subprocess.run([os.path.join(REF_DIR, "ref_sm160"), fin, fout],
+                             check=True, capture_output=True, text=True).stdout
+        raw = np.fromfile(fout, dtype=np.float64)
+    sdv_new = raw[:3 * n].reshape(n, 3).T.copy()
+    S = raw[3 * n:].reshape(n, 9).T.copy()
+    warn = np.zeros(n, dtype=np.uint32)
+    cur = -1
+    for line in out.splitlines():
+        if line.startswith("#PT "):
+            cur = int(line[4:])
+        elif "WARNING 1" in line:
+            warn[cur] |= 1
+        elif "WARNING 2" in line:
+            warn[cur] |= 2
+        elif "No convergence" in line:
+            warn[cur] |= 4
+    return sdv_new, S, warn
+
+
+def ref_hex8(elcod, eldis):
+    """elcod, eldis [n,8,3].  Returns dict of the reference tables and
+    per-element gpcar [n,8gp,8node,3], gpvol [n,8], F [n,8,3,3], detF [n,8]."""
+    n = elcod.shape[0]
+    with tempfile.TemporaryDirectory() as td:
+        fin = os.path.join(td, "in.bin")
+        fout = os.path.join(td, "out.bin")
+        with open(fin, "wb") as f:
+            np.int64(n).tofile(f)
+            np.ascontiguousarray(elcod, dtype=np.float64).tofile(f)
+            np.ascontiguousarray(eldis, dtype=np.float64).tofile(f)
+        subprocess.run([os.path.join(REF_DIR, "ref_hex8"), fin, fout], check=True,
+                       capture_output=True)
+        raw = np.fromfile(fout, dtype=np.float64)
+    o = 0
+
+    def take(cnt, shape):
+        nonlocal o
+        a = raw[o:o + cnt].reshape(shape)
+        o += cnt
+        return a
+
+    posgp = take(24, (8, 3))
+    weigp = take(8, (8,))
+    shapf = take(64, (8, 8))                         # [gp][node]
+    deriv = take(192, (8, 8, 3))                     # [gp][node][dim]
+    gpcar = take(192 * n, (n, 8, 8, 3))
+    gpvol = take(8 * n, (n, 8))
+    F = take(72 * n, (n, 8, 3, 3)).transpose(0, 1, 3, 2)   # col-major -> [row][col]
+    detF = take(8 * n, (n, 8))
+    return dict(posgp=posgp, weigp=weigp, shapf=shapf, deriv=deriv,
+                gpcar=gpcar, gpvol=gpvol, F=F.copy(), detF=detF)

@@ -1,0 +1,245 @@
+/*
+ * TEST INFRASTRUCTURE ONLY -- CPU oracle of one explicit solidz step.
+ *
+ * Restates, FP64, in reference operation order:
+ *   sld_updtss.f90:119-162 + elmlen.f90 (ndime==3) + sld_velsnd.f90:117-124
+ *       critical time step  dt_cri = min_e hleng_min(e) / sqrt((lam+2mu)/rho)
+ *   sld_funbou.f90:95-136 (itask 2)   DISCRETE[,SMOOTH] load tables
+ *   sld_updbcs.f90:79-114             bvess <- f(t^{n+1}) on fixed dofs
+ *   mod_sld_solution_methods.f90:101-223 (CD) and :385-574 (TW, prescribed
+ *       displacement branch)          predictor / corrector
+ *   sld_element_operations.f90:119-256 element loop: gather, elmca2,
+ *       sld_elmpre, sld_elmcla -> sld_builtin_materials.f90:316-371 (sm160,
+ *       P = F S), sld_elmmat.f90:115-209 (f_int, lumped mass)
+ *   mod_matrix.f90:2327-2431          element -> nodal assembly
+ *   solric.f90:294-330                a = r * (1/m)
+ *   sld_updunk.f90:149-186            TIME_N <- ITER_K (history ping-pong)
+ * The element loop runs in the order given by elem_order (colour-grouped when
+ * color_ptr is set, with OpenMP across the elements of one colour, so each
+ * node's contributions are summed in the same order as the GPU path).
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include "oracle.h"
+
+static double elem_E(const orc_model *m, int64_t e, const double **props_out)
+{
+    int32_t mat = m->mat_of_elem ? m->mat_of_elem[e] : 0;
+    *props_out = m->props + 18 * mat;
+    return m->E_of_elem ? m->E_of_elem[e] : (*props_out)[0];
+}
+
+static void gather(const orc_model *m, const double *x, int64_t e, double *el)
+{
+    for (int a = 0; a < 8; a++) {
+        int64_t p = m->lnods[8 * e + a];
+        el[3 * a + 0] = x[3 * p + 0];
+        el[3 * a + 1] = x[3 * p + 1];
+        el[3 * a + 2] = x[3 * p + 2];
+    }
+}
+
+int orc_sld_mass(const orc_model *m, orc_state *s)
+{
+    double posgp[24], weigp[8], shapf[64], deriv[192];
+    orc_hex8_tables(posgp, weigp, shapf, deriv);
+    memset(s->mass, 0, sizeof(double) * m->npoin);
+    for (int64_t ee = 0; ee < m->nelem; ee++) {
+        int64_t e = m->elem_order ? m->elem_order[ee] : ee;
+        double elcod[24], zero[24] = {0}, gpcar[192], gpvol[8], F[72], detF[8], elmuu[8] = {0};
+        const double *pr;
+        int32_t mat = m->mat_of_elem ? m->mat_of_elem[e] : 0;
+        (void)elem_E(m, e, &pr);
+        gather(m, m->coord, e, elcod);
+        orc_hex8_kinematics(elcod, zero, gpcar, gpvol, F, detF);
+        /* sld_elmmat.f90:199-209 */
+        for (int g = 0; g < 8; g++)
+            for (int a = 0; a < 8; a++)
+                for (int b = 0; b < 8; b++)
+                    elmuu[a] = elmuu[a] + m->rho[mat] * gpvol[g] * shapf[8 * g + a] * shapf[8 * g + b];
+        for (int a = 0; a < 8; a++) s->mass[m->lnods[8 * e + a]] += elmuu[a];
+    }
+    return 0;
+}
+
+double orc_sld_critical_dt(const orc_model *m)
+{
+    /* dercg: HEX08 derivatives at the centre (s=t=z=0) */
+    double D[24];
+    const double sgn[8][3] = {{-1,-1,-1},{1,-1,-1},{1,1,-1},{-1,1,-1},
+                              {-1,-1,1},{1,-1,1},{1,1,1},{-1,1,1}};
+    for (int a = 0; a < 8; a++)
+        for (int d = 0; d < 3; d++) D[3 * a + d] = 0.125 * sgn[a][d];
+    double dtmin = 1e300;
+    for (int64_t e = 0; e < m->nelem; e++) {
+        double elcod[24], xm[3][3] = {{0}}, tr[3][3], h[3];
+        const double *pr;
+        int32_t mat = m->mat_of_elem ? m->mat_of_elem[e] : 0;
+        double E = elem_E(m, e, &pr), nu = pr[1];
+        gather(m, m->coord, e, elcod);
+        for (int k = 0; k < 8; k++)
+            for (int a = 0; a < 3; a++)
+                for (int b = 0; b < 3; b++) xm[a][b] += elcod[3 * k + a] * D[3 * k + b];
+        double t1 =  xm[1][1] * xm[2][2] - xm[2][1] * xm[1][2];
+        double t2 = -xm[1][0] * xm[2][2] + xm[2][0] * xm[1][2];
+        double t3 =  xm[1][0] * xm[2][1] - xm[2][0] * xm[1][1];
+        double det = xm[0][0] * t1 + xm[0][1] * t2 + xm[0][2] * t3;
+        double dn = 1.0 / det;
+        tr[0][0] = t1 * dn; tr[1][0] = t2 * dn; tr[2][0] = t3 * dn;
+        tr[1][1] = ( xm[0][0] * xm[2][2] - xm[2][0] * xm[0][2]) * dn;
+        tr[2][1] = (-xm[0][0] * xm[2][1] + xm[0][1] * xm[2][0]) * dn;
+        tr[2][2] = ( xm[0][0] * xm[1][1] - xm[1][0] * xm[0][1]) * dn;
+        tr[0][1] = (-xm[0][1] * xm[2][2] + xm[2][1] * xm[0][2]) * dn;
+        tr[0][2] = ( xm[0][1] * xm[1][2] - xm[1][1] * xm[0][2]) * dn;
+        tr[1][2] = (-xm[0][0] * xm[1][2] + xm[1][0] * xm[0][2]) * dn;
+        for (int i = 0; i < 3; i++)
+            h[i] = 2.0 / sqrt(tr[i][0] * tr[i][0] + tr[i][1] * tr[i][1] + tr[i][2] * tr[i][2]);
+        double hmin = fmin(h[0], fmin(h[1], h[2]));
+        double lam = nu * E / ((1.0 + nu) * (1.0 - 2.0 * nu));
+        double mu = E / (2.0 * (1.0 + nu));
+        double c = sqrt((lam + 2.0 * mu) / m->rho[mat]);
+        double dt = hmin / c;
+        if (dt < dtmin) dtmin = dt;
+    }
+    return dtmin;
+}
+
+double orc_funbou(const orc_model *m, int32_t f, int c, double t)
+{
+    const double *tab = m->func_table + (size_t)f * m->func_maxpts * 4;
+    int np = m->func_npts[f];
+    double x = 0.0;
+    for (int i = 0; i < np - 1; i++) {
+        double t1 = tab[4 * i + 3], t2 = tab[4 * (i + 1) + 3];
+        if (t >= t1 && t <= t2) {
+            double p1 = tab[4 * i + c], p2 = tab[4 * (i + 1) + c], mm;
+            if (m->func_smooth[f] == 1) {
+                mm = (t - t1) / (t2 - t1);
+                x = p1 + (p2 - p1) * (mm * mm * mm) * (10.0 - 15.0 * mm + 6.0 * (mm * mm));
+            } else {
+                mm = (p2 - p1) / (t2 - t1);
+                x = p1 + mm * (t - t1);
+            }
+            break;
+        }
+    }
+    double tlast = tab[4 * (np - 1) + 3];
+    if (t > tlast) x = tab[4 * (np - 1) + c] * t / tlast;
+    return x;
+}
+
+/* Element loop: fint, rhs and the new history/stress for every element. */
+static void element_loop(const orc_model *m, orc_state *s)
+{
+    double *sdv_k = s->sdv, *sdv_n = s->sdv + 24 * m->nelem;
+    int ncol = m->color_ptr ? m->ncolor : 1;
+    memset(s->fint, 0, sizeof(double) * 3 * m->npoin);
+    memset(s->rhs, 0, sizeof(double) * 3 * m->npoin);
+    for (int c = 0; c < ncol; c++) {
+        int64_t lo = m->color_ptr ? m->color_ptr[c] : 0;
+        int64_t hi = m->color_ptr ? m->color_ptr[c + 1] : m->nelem;
+        int64_t ee;
+        #pragma omp parallel for schedule(dynamic, 64) if (m->color_ptr != NULL)
+        for (ee = lo; ee < hi; ee++) {
+            int64_t e = m->elem_order ? m->elem_order[ee] : ee;
+            double elcod[24], eldis[24], gpcar[192], gpvol[8], F[72], detF[8];
+            double elfint[24] = {0};
+            double props[18];
+            const double *pr;
+            double E = elem_E(m, e, &pr);
+            memcpy(props, pr, sizeof(props));
+            props[0] = E;
+            gather(m, m->coord, e, elcod);
+            gather(m, s->u, e, eldis);
+            orc_hex8_kinematics(elcod, eldis, gpcar, gpvol, F, detF);
+            for (int g = 0; g < 8; g++) {
+                double S[9], P[9];
+                const double *Fg = F + 9 * g;
+                double *sk = sdv_k + 24 * e + 3 * g, *sn = sdv_n + 24 * e + 3 * g;
+                uint32_t fl = orc_sm160_point(props, Fg, sn, sk, S, 0);
+                if (!(detF[g] > 0.0)) fl |= ORC_FLAG_DETF_NONPOS;
+                if (s->flags) s->flags[8 * e + g] = fl;
+                if (s->S) memcpy(s->S + 72 * e + 9 * g, S, sizeof(S));
+                /* P_jk = sum_i F_ji S_ik  (sld_builtin_materials.f90:359-369) */
+                for (int q = 0; q < 9; q++) P[q] = 0.0;
+                for (int k = 0; k < 3; k++)
+                    for (int j = 0; j < 3; j++)
+                        for (int i = 0; i < 3; i++)
+                            P[j + 3 * k] = P[j + 3 * k] + Fg[j + 3 * i] * S[i + 3 * k];
+                /* sld_elmmat.f90:137-150 */
+                double volux = gpvol[g];
+                for (int a = 0; a < 8; a++)
+                    for (int i = 0; i < 3; i++)
+                        for (int j = 0; j < 3; j++)
+                            elfint[3 * a + i] = elfint[3 * a + i]
+                                + (P[i + 3 * j] + 0.0) * gpcar[24 * g + 3 * a + j] * volux;
+            }
+            for (int a = 0; a < 8; a++) {
+                int64_t p = m->lnods[8 * e + a];
+                for (int i = 0; i < 3; i++) {
+                    double r = 0.0 - elfint[3 * a + i] - 0.0;
+                    s->rhs[3 * p + i] += r;
+                    s->fint[3 * p + i] += elfint[3 * a + i];
+                }
+            }
+        }
+    }
+}
+
+int orc_sld_step(const orc_model *m, orc_state *s, double dt, int scheme, double phi)
+{
+    int64_t np = m->npoin;
+    double cutim = s->t + dt;             /* setgts.f90:98 */
+    double t_end = cutim + dt, t_mid = 0.5 * (cutim + t_end);
+    double fx = 0.0;
+    if (m->fext0) fx = orc_funbou(m, m->fext_func, 0, cutim);
+
+    /* predictor */
+    for (int64_t p = 0; p < np; p++) {
+        for (int d = 0; d < 3; d++) {
+            int fixed = (m->fixmask[p] >> d) & 1;
+            double bv = 0.0;
+            if (fixed && m->func_of_node && m->func_of_node[p] >= 0)
+                bv = orc_funbou(m, m->func_of_node[p], d, cutim);
+            double *u = &s->u[3 * p + d], *v = &s->v[3 * p + d], *a = &s->a[3 * p + d];
+            if (scheme == 0) {
+                double vh = *v + (t_mid - cutim) * *a;
+                double un = *u + dt * vh;
+                if (fixed) un = bv;
+                *v = vh;
+                *u = un;
+            } else {
+                double vn = *v + dt * *a;
+                double un = *u + dt * *v + phi * (dt * dt) * *a;
+                if (fixed) un = bv;
+                *v = vn;
+                *u = un;
+            }
+        }
+    }
+    element_loop(m, s);
+    if (m->fext0)
+        for (int64_t p = 0; p < np; p++)
+            for (int d = 0; d < 3; d++) s->rhs[3 * p + d] += fx * m->fext0[3 * p + d];
+    /* solve + corrector */
+    for (int64_t p = 0; p < np; p++) {
+        double fact1 = 1.0 / s->mass[p];
+        for (int d = 0; d < 3; d++) {
+            double acc = 0.0 + s->rhs[3 * p + d] * fact1;
+            int fixed = (m->fixmask[p] >> d) & 1;
+            if (scheme == 0) {
+                s->a[3 * p + d] = acc;
+                s->v[3 * p + d] = s->v[3 * p + d] + (t_end - t_mid) * acc;
+            } else {
+                s->a[3 * p + d] = acc;
+                if (fixed) { s->v[3 * p + d] = 0.0; s->a[3 * p + d] = 0.0; }
+            }
+        }
+    }
+    /* sld_updunk ENDSTE: TIME_N <- ITER_K */
+    memcpy(s->sdv + 24 * m->nelem, s->sdv, sizeof(double) * 24 * m->nelem);
+    s->t = cutim;
+    s->nsteps++;
+    return 0;
+}

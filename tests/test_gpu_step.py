@@ -1,0 +1,148 @@
+"""Whole explicit step on the GPU (sld_dev_*) against the CPU oracle run in
+the same colour order: nodal fields, forces, stresses <= 1e-10 relative in
+FP64, history and branch flags exact; determinism; restart; FP32 error."""
+import numpy as np
+import pytest
+
+from alya_sma_amd import problems
+from alya_sma_amd.solid import Solidz
+from oracle import oracle_py as O
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-10
+
+
+def _rel(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+def _pair(prob, store=True):
+    dev = Solidz(prob)
+    if store:
+        dev.set_store(True)
+    ptr, order = dev.coloring()
+    orc = O.StepOracle(prob, elem_order=order, color_ptr=ptr)
+    return dev, orc
+
+
+def _compare(dev, orc, label):
+    u, v, a, f = dev.get("u"), dev.get("v"), dev.get("a"), dev.get("fint")
+    sdv = dev.get("sdv")
+    S = dev.get("stress")
+    fl = dev.get("flags")
+    errs = dict(u=_rel(u, orc.u), v=_rel(v, orc.v), a=_rel(a, orc.a), fint=_rel(f, orc.fint),
+                S=_rel(S, orc.S))
+    print(label, {k: f"{x:.1e}" for k, x in errs.items()}, flush=True)
+    for k, x in errs.items():
+        assert x < TOL, (label, k, x)
+    np.testing.assert_array_equal(fl & 3, orc.flags & 3)
+    assert _rel(sdv[..., 0], orc.sdv[0, ..., 0]) < 1e-9
+    assert _rel(sdv[..., 1:], orc.sdv[0, ..., 1:]) < TOL
+
+
+@pytest.mark.parametrize("scheme", [0, 1])
+def test_bar_config0_full_cycle(scheme):
+    """Config 0: 10x2x2 bar, ramp to 6% strain and back, 2000 steps."""
+    prob = problems.bar_problem()
+    r = prob["run"]
+    print(f"bar config0 scheme {scheme}: 2000 steps", flush=True)
+    dev, orc = _pair(prob)
+    with dev:
+        np.testing.assert_allclose(dev.get("mass"), orc.mass, rtol=1e-13)
+        for chunk in range(4):
+            dev.step(r["dt"], nsteps=500, scheme=scheme, phi=1.033)
+            orc.step(r["dt"], scheme=scheme, phi=1.033, n=500)
+            _compare(dev, orc, f"bar scheme {scheme} step {500 * (chunk + 1)}")
+        xs = orc.sdv[0, ..., 0]
+        assert xs.max() == 1.0 or scheme == 1            # the cycle reached saturation (CD)
+        c = dev.counters()
+        assert c["kf_AS"] > 0 and c["kf_SA"] > 0
+        t = dev.get("time")[0]
+        assert abs(t - orc.t) <= 1e-15 * orc.t
+
+
+def test_small_cantilever_with_noise_and_tip_load():
+    prob = problems.cantilever_problem(nx=24, ny=4, nz=4, lx=6.0, ly=1.0, lz=1.0, nsteps=600,
+                                       traction=700.0)
+    r = prob["run"]
+    print("cantilever 24x4x4: 600 steps", flush=True)
+    dev, orc = _pair(prob)
+    with dev:
+        dev.step(r["dt"], nsteps=300, scheme=0)
+        orc.step(r["dt"], scheme=0, n=300)
+        _compare(dev, orc, "cantilever 300")
+        dev.step(r["dt"], nsteps=300, scheme=0)
+        orc.step(r["dt"], scheme=0, n=300)
+        _compare(dev, orc, "cantilever 600")
+        assert np.any(orc.flags & 1)
+
+
+def test_deterministic_bitwise():
+    prob = problems.cantilever_problem(nx=24, ny=4, nz=4, lx=6.0, ly=1.0, lz=1.0, nsteps=200,
+                                       traction=700.0)
+    r = prob["run"]
+    outs = []
+    for _ in range(2):
+        with Solidz(prob) as dev:
+            dev.step(r["dt"], nsteps=200)
+            outs.append((dev.get("u"), dev.get("fint"), dev.get("sdv")))
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
+
+
+def test_restart_bitwise():
+    prob = problems.bar_problem(nsteps=400)
+    dt = prob["run"]["dt"]
+    with Solidz(prob) as a:
+        a.step(dt, nsteps=400)
+        ref = (a.get("u"), a.get("v"), a.get("sdv"))
+    with Solidz(prob) as b:
+        b.step(dt, nsteps=200)
+        state = {k: b.get(k) for k in ("u", "v", "a", "sdv", "time")}
+    with Solidz(prob) as c:
+        for k, x in state.items():
+            c.set(k, x)
+        c.step(dt, nsteps=200)
+        got = (c.get("u"), c.get("v"), c.get("sdv"))
+    for x, y in zip(ref, got):
+        assert np.array_equal(x, y)
+
+
+def test_odd_step_counts_and_eager_path_agree():
+    prob = problems.bar_problem(nsteps=200)
+    dt = prob["run"]["dt"]
+    with Solidz(prob) as a, Solidz(prob) as b:
+        a.step(dt, nsteps=37)
+        for _ in range(37):
+            b.step(dt, nsteps=1)
+        assert np.array_equal(a.get("u"), b.get("u"))
+        assert np.array_equal(a.get("sdv"), b.get("sdv"))
+
+
+@pytest.mark.parametrize("epb", [8, 16, 32])
+def test_tiles_agree(epb):
+    prob = problems.cantilever_problem(nx=16, ny=4, nz=4, lx=4.0, ly=1.0, lz=1.0, nsteps=100,
+                                       traction=700.0)
+    dt = prob["run"]["dt"]
+    with Solidz(prob) as a, Solidz(prob) as b:
+        b.set_tile(epb, xcd_remap=(epb != 16))
+        a.step(dt, nsteps=100)
+        b.step(dt, nsteps=100)
+        assert np.array_equal(a.get("u"), b.get("u"))
+
+
+def test_fp32_mode_error_vs_fp64_oracle():
+    prob = problems.bar_problem()
+    r = prob["run"]
+    dev, orc = _pair(prob)
+    with dev:
+        dev.set_mode(1)
+        dev.step(r["dt"], nsteps=1000, scheme=0)
+        orc.step(r["dt"], scheme=0, n=1000)
+        S = dev.get("stress")
+        eS = np.linalg.norm(S - orc.S) / np.linalg.norm(orc.S)
+        eu = np.linalg.norm(dev.get("u") - orc.u) / np.linalg.norm(orc.u)
+        exs = np.max(np.abs(dev.get("sdv")[..., 0] - orc.sdv[0, ..., 0]))
+        print(f"fp32 mode, bar 1000 steps: stress rel L2 {eS:.2e}, u rel L2 {eu:.2e}, "
+              f"xS max abs {exs:.2e}", flush=True)
+        assert eS < 1e-3 and eu < 1e-3
