@@ -56,11 +56,12 @@ def build_fortran():
     flib = os.path.join(fdir, "libalya_sma_f.so")
     _run([FLANG, "-O2", "-fPIC", "-shared", "-J" + bdir, "-o", flib] + srcs +
          ["-L" + HERE, "-lalya_sma_amd", "-Wl,-rpath,$ORIGIN/.."])
-    drv = os.path.join(fdir, "sld_bar_driver.f90")
-    if os.path.exists(drv):
-        _run([FLANG, "-O2", "-J" + bdir, "-o", os.path.join(fdir, "sld_bar_driver"), drv,
-              "-L" + fdir, "-lalya_sma_f", "-L" + HERE, "-lalya_sma_amd",
-              "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,$ORIGIN/.."])
+    for name in ("sld_bar_driver", "sld_law_driver"):
+        drv = os.path.join(fdir, name + ".f90")
+        if os.path.exists(drv):
+            _run([FLANG, "-O2", "-J" + bdir, "-o", os.path.join(fdir, name), drv,
+                  "-L" + fdir, "-lalya_sma_f", "-L" + HERE, "-lalya_sma_amd",
+                  "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,$ORIGIN/.."])
     shutil.rmtree(bdir, ignore_errors=True)
     return flib
 

@@ -158,14 +158,49 @@ int surrogate_enqueue_elements(sld_dev*, double, int) {
 }
 void surrogate_detach(sld_dev* d) { d->sur = nullptr; }
 
+// Optional per-launch HIP-event timing (eager launches only, never captured).
+struct KernelTimer {
+  std::vector<hipEvent_t> ev;
+  std::vector<int> kind;        // 0 predict, 1 element, 2 correct, 3 other
+  size_t used = 0;
+  hipStream_t s = nullptr;
+  void mark(int k) {
+    if (used + 2 > ev.size()) {
+      hipEvent_t a, b;
+      hipEventCreate(&a);
+      hipEventCreate(&b);
+      ev.push_back(a);
+      ev.push_back(b);
+      kind.push_back(k);
+    } else {
+      kind[used / 2] = k;
+    }
+    hipEventRecord(ev[used], s);
+  }
+  void done() {
+    hipEventRecord(ev[used + 1], s);
+    used += 2;
+  }
+  ~KernelTimer() {
+    for (auto e : ev) hipEventDestroy(e);
+  }
+};
+
+#define TBEGIN(k) do { if (T) T->mark(k); } while (0)
+#define TEND() do { if (T) T->done(); } while (0)
+
 // One explicit step; tn = history slot holding TIME_N.
-int enqueue_step(sld_dev* d, double dt, int scheme, double phi, int tn) {
+int enqueue_step(sld_dev* d, double dt, int scheme, double phi, int tn, KernelTimer* T = nullptr) {
   hipStream_t s = d->stream;
   sld::NodalArgs N = nodal_args(d);
   const unsigned nb = nblocks(d->npoin, 256);
+  TBEGIN(0);
   if (scheme == 0) hipLaunchKernelGGL((sld::sld_predict<0>), dim3(nb), dim3(256), 0, s, N, dt, phi);
   else hipLaunchKernelGGL((sld::sld_predict<1>), dim3(nb), dim3(256), 0, s, N, dt, phi);
+  TEND();
+  TBEGIN(3);
   hipLaunchKernelGGL(sld::sld_clock, dim3(1), dim3(64), 0, s, d->clock, dt);
+  TEND();
 
   if (d->mode == SLD_MODE_SURROGATE) {
     int rc = surrogate_enqueue_elements(d, dt, tn);
@@ -192,6 +227,8 @@ int enqueue_step(sld_dev* d, double dt, int scheme, double phi, int tn) {
       E.e_begin = d->color_ptr[c];
       E.e_end = d->color_ptr[c + 1];
       const bool st = d->store != 0;
+      if (E.e_end <= E.e_begin) continue;
+      TBEGIN(1);
       if (d->mode == SLD_MODE_FP64) {
         if (st) launch_element_epb<double, true>(d->epb, E, s);
         else launch_element_epb<double, false>(d->epb, E, s);
@@ -199,10 +236,13 @@ int enqueue_step(sld_dev* d, double dt, int scheme, double phi, int tn) {
         if (st) launch_element_epb<float, true>(d->epb, E, s);
         else launch_element_epb<float, false>(d->epb, E, s);
       }
+      TEND();
     }
   }
+  TBEGIN(2);
   if (scheme == 0) hipLaunchKernelGGL((sld::sld_correct<0>), dim3(nb), dim3(256), 0, s, N, dt);
   else hipLaunchKernelGGL((sld::sld_correct<1>), dim3(nb), dim3(256), 0, s, N, dt);
+  TEND();
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -533,6 +573,49 @@ int sld_dev_step(sld_dev* d, double dt, int32_t scheme, double phi, int64_t nste
     if (rc) return rc;
     d->cur = 1 - d->cur;
   }
+  return SMA_OK;
+}
+
+int sld_dev_time_kernels(sld_dev* d, double dt, int32_t scheme, double phi, int32_t nsteps,
+                         double* out) {
+  if (!d || !out) return fail(SMA_ERR_ARG, "null argument");
+  if (scheme != 0 && scheme != 1) return fail(SMA_ERR_ARG, "scheme must be 0 (CD) or 1 (TW)");
+  if (!(dt > 0.0) || nsteps <= 0) return fail(SMA_ERR_ARG, "dt > 0 and nsteps > 0 required");
+  if (d->mode == SLD_MODE_SURROGATE && !surrogate_attached(d))
+    return fail(SMA_ERR_STATE, "surrogate mode without a surrogate");
+  KernelTimer T;
+  T.s = d->stream;
+  double sum[4] = {0, 0, 0, 0};
+  int64_t cnt[4] = {0, 0, 0, 0};
+  hipEvent_t t0, t1;
+  HIPCHK(hipEventCreate(&t0));
+  HIPCHK(hipEventCreate(&t1));
+  HIPCHK(hipEventRecord(t0, d->stream));
+  for (int k = 0; k < nsteps; ++k) {
+    T.used = 0;
+    int rc = enqueue_step(d, dt, scheme, phi, d->cur, &T);
+    if (rc) return rc;
+    d->cur = 1 - d->cur;
+    HIPCHK(hipStreamSynchronize(d->stream));
+    for (size_t i = 0; i < T.used / 2; ++i) {
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, T.ev[2 * i], T.ev[2 * i + 1]));
+      sum[T.kind[i]] += ms;
+      cnt[T.kind[i]] += 1;
+    }
+  }
+  HIPCHK(hipEventRecord(t1, d->stream));
+  HIPCHK(hipEventSynchronize(t1));
+  float tot = 0.f;
+  HIPCHK(hipEventElapsedTime(&tot, t0, t1));
+  hipEventDestroy(t0);
+  hipEventDestroy(t1);
+  out[0] = cnt[0] ? sum[0] / cnt[0] : 0.0;    // predictor, ms per launch
+  out[1] = cnt[1] ? sum[1] / cnt[1] : 0.0;    // element kernel, ms per launch
+  out[2] = cnt[2] ? sum[2] / cnt[2] : 0.0;    // corrector, ms per launch
+  out[3] = (double)cnt[1] / nsteps;          // element launches per step
+  out[4] = (sum[0] + sum[1] + sum[2] + sum[3]) / nsteps;  // kernel ms per step
+  out[5] = (double)tot / nsteps;             // wall ms per step on the stream (eager)
   return SMA_OK;
 }
 

@@ -134,6 +134,15 @@ class Solidz:
     def stream(self):
         return lib().sld_dev_stream(self.h)
 
+    def time_kernels(self, dt, nsteps=4, scheme=0, phi=1.033):
+        """HIP-event timing of every launch over nsteps eager steps."""
+        out = np.zeros(6)
+        check(lib().sld_dev_time_kernels(self.h, float(dt), int(scheme), float(phi), int(nsteps),
+                                         _p(out)), "sld_dev_time_kernels")
+        keys = ["predict_ms", "element_ms", "correct_ms", "element_launches_per_step",
+                "kernel_ms_per_step", "stream_ms_per_step"]
+        return dict(zip(keys, out.tolist()))
+
     def critical_dt(self):
         return lib().sld_dev_critical_dt(self.h)
 

@@ -1,0 +1,171 @@
+"""Benchmark: Gauss-point updates per second of the explicit SMA step.
+
+Workload (N = 1): the SMA cantilever, hex8 400x20x20 = 160 000 elements
+(1.28 M Gauss points) over 50 x 2.5 x 2.5 mm, clamped at x = 0, tip traction
+ramped up and released over 5000 central-difference steps, per-element
+Young's modulus with seeded +-2% noise.  Inputs are resident in HBM before
+the timed region; one step = predictor, element kernel per colour, corrector.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+
+Prints one JSON line.  `value` is the throughput of the fastest precision
+mode whose stress error against the FP64 path stays within 1e-3 (relative
+L2 at the last step); every mode's rate and error is listed under "modes".
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Gauss-pt updates/s per explicit step; surrogate vs plain at ≤1e-3 stress err"
+PEAK_HBM_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--modes", default="fp64,fp32")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--epb", type=int, default=0, help="elements per block (0 = default)")
+    return ap.parse_args()
+
+
+def nodes_per_colour(lnods, order, ptr):
+    out = []
+    for c in range(len(ptr) - 1):
+        els = order[ptr[c]:ptr[c + 1]]
+        out.append(len(np.unique(lnods[els].ravel())))
+    return np.array(out)
+
+
+def cpu_baseline(prob, budget_s):
+    """Time the FP64 CPU oracle (OpenMP over the elements of each colour) on
+    the first steps of the same workload."""
+    from oracle import oracle_py as O
+    from alya_sma_amd import _lib  # noqa: F401  (colouring helper)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    lnods = prob["lnods"]
+    color = np.zeros(lnods.shape[0], dtype=np.int32)
+    nc = _lib.lib().sld_color_elements(lnods.shape[0], prob["coord"].shape[0],
+                                       lnods.ctypes.data, color.ctypes.data)
+    order = np.argsort(color, kind="stable").astype(np.int32)
+    ptr = np.concatenate([[0], np.cumsum(np.bincount(color, minlength=nc))]).astype(np.int32)
+    orc = O.StepOracle(prob, elem_order=order, color_ptr=ptr, keep_stress=False)
+    dt = prob["run"]["dt"]
+    orc.step(dt, scheme=0)                     # first step outside the sample (page-in)
+    n, t0 = 0, time.perf_counter()
+    while True:
+        orc.step(dt, scheme=0)
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or n >= 200:
+            break
+    ngp = prob["lnods"].shape[0] * 8
+    cpu = "unknown"
+    try:
+        cpu = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
+        cpu = [l.split(":", 1)[1].strip() for l in cpu.splitlines() if l.startswith("Model name")][0]
+    except Exception:
+        pass
+    return {"value": ngp * n / el, "unit": "GP-updates/s", "cores": threads, "kind": "port",
+            "sample": f"steps 2..{n + 1} of the same cantilever workload, FP64 C oracle "
+                      f"(oracle/sld_step.c, gcc -O3, OpenMP over the elements of each colour), "
+                      f"{el:.1f} s wall on {threads} threads of {cpu}"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU "
+              "sharding of this path is described in DESIGN.md and not run here.", flush=True)
+        sys.exit(2)
+    import torch
+    from alya_sma_amd import problems
+    from alya_sma_amd.solid import Solidz
+
+    if not torch.cuda.is_available():
+        sys.exit("bench.py needs a HIP device")
+    prob = problems.cantilever_problem()
+    r = prob["run"]
+    dt = r["dt"]
+    nelem = prob["lnods"].shape[0]
+    ngp = nelem * 8
+    K, W = args.steps, args.warmup
+    modes = [m for m in args.modes.split(",") if m]
+    results, stress = {}, {}
+    roof = None
+    for mode in modes:
+        dev = Solidz(prob)
+        dev.set_mode({"fp64": 0, "fp32": 1, "surrogate": 2}[mode])
+        if args.epb:
+            dev.set_tile(args.epb)
+        dev.step(dt, nsteps=W)
+        dev.sync()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        dev.step(dt, nsteps=K)
+        dev.sync()
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        ms = 1e3 * el / K
+        # HIP-event timing of each launch over a few further (eager) steps
+        tk = dev.time_kernels(dt, nsteps=4)
+        dev.set_store(True)
+        dev.step(dt, nsteps=1)
+        stress[mode] = dev.get("stress")
+        cnt = dev.counters()
+        results[mode] = {"value": ngp / (ms * 1e-3), "ms_per_step": ms,
+                         "kernel_ms": {k: round(v, 5) for k, v in tk.items()},
+                         "transforming_gp_updates": cnt["kf_AS"] + cnt["kf_SA"]}
+        if mode == "fp64":
+            ptr, order = dev.coloring()
+            npc = nodes_per_colour(prob["lnods"], order, ptr)
+            ne_pc = np.diff(ptr)
+            # algorithmic bytes of one element launch (average over colours):
+            # history 48 B/GP + connectivity 4 B/GP + E 1 B/GP, and per node
+            # touched: coords 24 + u 24 + f_int read-modify-write 48
+            bytes_launch = float(np.mean(ne_pc * 8 * 53 + npc * 96))
+            roof = {"bound": "hbm", "achieved": bytes_launch / (tk["element_ms"] * 1e-3) / 1e9,
+                    "peak": PEAK_HBM_GBS, "unit": "GB/s", "traffic": None,
+                    "kernel": "sld_element<double,32,false>",
+                    "bytes_per_launch": bytes_launch, "launch_ms": tk["element_ms"]}
+            roof["frac"] = roof["achieved"] / roof["peak"]
+        dev.close()
+        print(f"# {mode}: {ms:.3f} ms/step, {results[mode]['value']:.3e} GP/s, kernels {tk}",
+              file=sys.stderr, flush=True)
+    ref = stress.get("fp64")
+    for mode in modes:
+        if ref is not None:
+            results[mode]["stress_rel_l2_vs_fp64"] = float(np.linalg.norm(stress[mode] - ref) /
+                                                           max(np.linalg.norm(ref), 1e-300))
+    ok = [m for m in modes if results[m].get("stress_rel_l2_vs_fp64", 0.0) <= 1e-3]
+    best = max(ok, key=lambda m: results[m]["value"]) if ok else modes[0]
+    cpu = None if args.no_cpu else cpu_baseline(prob, args.cpu_seconds)
+    out = {
+        "metric": METRIC, "value": results[best]["value"], "unit": "GP-updates/s",
+        "n_gpus": 1, "steps": K, "warmup": W, "ms_per_step": results[best]["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": {"fp64": "f64", "fp32": "f32+f64", "surrogate": "bf16+f32+f64"}[best],
+        "data": "synthetic",
+        "config": {"workload": "SMA cantilever hex8 400x20x20 (160000 elements, 1280000 Gauss "
+                               "points), 50x2.5x2.5 mm, tip traction load-unload, E +-2% noise, "
+                               "central difference", "mode": best, "dt": dt,
+                   "gauss_points": ngp, "nodes": int(prob["coord"].shape[0])},
+        "modes": results, "roofline": roof, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

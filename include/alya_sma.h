@@ -127,6 +127,14 @@ int sld_dev_set_store(sld_dev *dev, int32_t store);
  * Asynchronous on the handle's stream; steps are replayed from a hipGraph. */
 int sld_dev_step(sld_dev *dev, double dt, int32_t scheme, double phi, int64_t nsteps);
 
+/* Runs nsteps steps eagerly with HIP events around every launch on the
+ * handle's stream (the reference brackets the same phases with its
+ * times(3..7) timers, sld_element_operations.f90:125-256).  out[6]:
+ * ms per launch of the predictor, the element kernel and the corrector,
+ * element launches per step, summed kernel ms per step, stream ms per step. */
+int sld_dev_time_kernels(sld_dev *dev, double dt, int32_t scheme, double phi,
+                         int32_t nsteps, double *out);
+
 /* Critical time step min_e h_min(e)/c(e) (sld_updtss.f90:119-162). */
 double sld_dev_critical_dt(const sld_dev *dev);
 

@@ -1,0 +1,76 @@
+"""The Fortran host layer (alya_sma_amd/fortran, iso_c_binding over the C
+ABI) on the GPU: the drop-in eval_sld_stress_model_160 against the
+reference's golden vectors, and the Fortran bar driver against the oracle."""
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from alya_sma_amd import problems
+from oracle import oracle_py as O
+
+pytestmark = pytest.mark.gpu
+FDIR = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                    "alya_sma_amd", "fortran")
+
+
+def _need(exe):
+    path = os.path.join(FDIR, exe)
+    if not os.path.exists(path):
+        pytest.fail(f"{path} not built (alya_sma_amd.build.build_fortran)")
+    return path
+
+
+def test_fortran_dropin_law_golden(golden_sm160):
+    g = golden_sm160
+    exe = _need("sld_law_driver")
+    # the drop-in takes one material per call, as parco_sld(1:18, pmate)
+    for m in range(g["mats"].shape[0]):
+        idx = np.where((g["mat"] == m) & np.isfinite(g["S"]).all(axis=0))[0]
+        n = len(idx)
+        with tempfile.TemporaryDirectory() as td:
+            fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+            with open(fin, "wb") as f:
+                np.int64(n).tofile(f)
+                np.tile(g["mats"][m], (n, 1)).tofile(f)
+                np.ascontiguousarray(g["F"][:, idx].T).tofile(f)
+                np.ascontiguousarray(g["sdv_old"][:, idx].T).tofile(f)
+            subprocess.run([exe, fin, fout], check=True, timeout=120)
+            raw = np.fromfile(fout)
+        sn = raw[:3 * n].reshape(n, 3).T
+        S = raw[3 * n:].reshape(n, 9).T
+        Sg = g["S"][:, idx]
+        err = np.max(np.abs(S - Sg)) / np.max(np.abs(Sg))
+        print(f"fortran drop-in, material {m}: {n} points, stress err {err:.1e}", flush=True)
+        assert err < 1e-10
+        np.testing.assert_array_equal(sn[0], g["sdv_new"][0, idx])
+
+
+@pytest.mark.parametrize("scheme", [0, 1])
+def test_fortran_bar_driver_matches_oracle(scheme):
+    exe = _need("sld_bar_driver")
+    prob = problems.bar_problem()
+    dt = prob["run"]["dt"]
+    with tempfile.TemporaryDirectory() as td:
+        out = os.path.join(td, "bar.bin")
+        r = subprocess.run([exe, "2000", repr(dt), str(scheme), out], check=True, timeout=300,
+                           capture_output=True, text=True)
+        print(r.stdout.splitlines()[-1], flush=True)
+        raw = np.fromfile(out)
+    npoin, nelem = prob["coord"].shape[0], prob["lnods"].shape[0]
+    u = raw[:3 * npoin].reshape(npoin, 3)
+    sdv = raw[3 * npoin:3 * npoin + 24 * nelem].reshape(nelem, 8, 3)
+    f = raw[3 * npoin + 24 * nelem:].reshape(npoin, 3)
+    # the oracle in the device's colour order
+    from alya_sma_amd.solid import Solidz
+    with Solidz(prob) as dev:
+        ptr, order = dev.coloring()
+    orc = O.StepOracle(prob, elem_order=order, color_ptr=ptr)
+    orc.step(dt, scheme=scheme, n=2000)
+    eu = np.max(np.abs(u - orc.u)) / np.max(np.abs(orc.u))
+    ef = np.max(np.abs(f - orc.fint)) / np.max(np.abs(orc.fint))
+    print(f"fortran bar driver scheme {scheme}: u err {eu:.1e}, fint err {ef:.1e}", flush=True)
+    assert eu < 1e-10 and ef < 1e-10
+    assert np.max(np.abs(sdv[..., 0] - orc.sdv[0, ..., 0])) < 1e-9
