@@ -81,15 +81,16 @@ def lib():
     L.sld_mesh_box.argtypes = [i32, i32, i32, d, d, d, p, p]
     L.sld_color_elements.argtypes = [i64, i64, p, p]
     L.sld_color_elements.restype = i32
-    if hasattr(L, "sma_mlp_create"):
-        L.sma_mlp_create.argtypes = [C.POINTER(p)]
-        L.sma_mlp_destroy.argtypes = [p]
-        L.sma_mlp_set_params.argtypes = [p, p, p, p, p]
-        L.sma_mlp_get_params.argtypes = [p, p, p, p, p]
-        L.sma_mlp_forward_device.argtypes = [p, i64, p, p, p]
-        L.sma_mlp_train_step_device.argtypes = [p, i64, p, p, d, p, p]
-        L.sma_mlp_grad_device.argtypes = [p, i64, p, p, p, p]
-        L.sma_surrogate_eval_device.argtypes = [p, i64, i32, p, p, p, p, p, p, p, p, d, p]
+    L.sma_mlp_create.argtypes = [C.c_uint64, C.POINTER(p)]
+    L.sma_mlp_destroy.argtypes = [p]
+    L.sma_mlp_set_params.argtypes = [p, p, p]
+    L.sma_mlp_get_params.argtypes = [p, p, p]
+    L.sma_mlp_set_loss_weights.argtypes = [p, p]
+    L.sma_mlp_forward_device.argtypes = [p, i64, p, p, p]
+    L.sma_mlp_grad_device.argtypes = [p, i64, p, p, p, p, p, p]
+    L.sma_mlp_train_step_device.argtypes = [p, i64, p, p, d, p, p]
+    L.sma_mlp_gen_samples_device.argtypes = [i64, C.c_uint64, p, d, d, d, p, p, p, p]
+    if hasattr(L, "sld_dev_set_surrogate"):
         L.sld_dev_set_surrogate.argtypes = [p, p, d]
     _lib = L
     return L

@@ -14,23 +14,13 @@
 #include <vector>
 
 #include "alya_sma.h"
+#include "common.hpp"
 #include "sld_kernels.hpp"
 
 namespace {
 
-thread_local std::string g_err;
-
-int fail(int code, const std::string& msg) {
-  g_err = msg;
-  return code;
-}
-
-#define HIPCHK(x)                                                                \
-  do {                                                                           \
-    hipError_t e_ = (x);                                                         \
-    if (e_ != hipSuccess)                                                        \
-      return fail(SMA_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_));  \
-  } while (0)
+using sma_abi::fail;
+#define HIPCHK(x) SMA_HIPCHK(x)
 
 template <typename T>
 int dalloc(T** p, size_t count) {
@@ -102,7 +92,20 @@ struct sld_dev {
   double dtcrit = 0.0;
   std::map<GraphKey, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
   std::vector<void*> allocs;
-  void* sur = nullptr;               // attached surrogate (mode 2)
+  struct Sur {
+    sma_mlp* mlp = nullptr;
+    double tol = 1e-4;
+    double T = 310.0;
+    int64_t cap = 0;
+    uint8_t* status = nullptr;
+    double* x = nullptr;
+    float* feat = nullptr;
+    int32_t* gp = nullptr;
+    double* aux = nullptr;
+    unsigned* count = nullptr;
+    float* Y = nullptr;
+  };
+  Sur* sur = nullptr;                // attached surrogate (mode 2)
 };
 
 namespace {
@@ -137,26 +140,30 @@ sld::NodalArgs nodal_args(sld_dev* d) {
   return A;
 }
 
-template <typename R, int EPB, bool STORE>
+template <typename R, int EPB, bool STORE, int SUR>
 void launch_element(const sld::ElemArgs& A, hipStream_t s) {
   const int ne = A.e_end - A.e_begin;
   if (ne <= 0) return;
-  hipLaunchKernelGGL((sld::sld_element<R, EPB, STORE>), dim3(nblocks(ne, EPB)), dim3(EPB * 8), 0,
+  hipLaunchKernelGGL((sld::sld_element<R, EPB, STORE, SUR>), dim3(nblocks(ne, EPB)), dim3(EPB * 8), 0,
                      s, A);
 }
 
-template <typename R, bool STORE>
+template <typename R, bool STORE, int SUR = 0>
 void launch_element_epb(int epb, const sld::ElemArgs& A, hipStream_t s) {
-  if (epb == 8) launch_element<R, 8, STORE>(A, s);
-  else if (epb == 16) launch_element<R, 16, STORE>(A, s);
-  else launch_element<R, 32, STORE>(A, s);
+  if (epb == 8) launch_element<R, 8, STORE, SUR>(A, s);
+  else if (epb == 16) launch_element<R, 16, STORE, SUR>(A, s);
+  else launch_element<R, 32, STORE, SUR>(A, s);
 }
 
-bool surrogate_attached(const sld_dev* d) { return d->sur != nullptr; }
-int surrogate_enqueue_elements(sld_dev*, double, int) {
-  return fail(SMA_ERR_STATE, "surrogate mode is not available in this build");
+bool surrogate_attached(const sld_dev* d) { return d->sur != nullptr && d->sur->mlp != nullptr; }
+
+void surrogate_detach(sld_dev* d) {
+  if (!d->sur) return;
+  void* ps[] = {d->sur->status, d->sur->x, d->sur->feat, d->sur->gp, d->sur->aux, d->sur->count, d->sur->Y};
+  for (void* p : ps) if (p) (void)hipFree(p);
+  delete d->sur;
+  d->sur = nullptr;
 }
-void surrogate_detach(sld_dev* d) { d->sur = nullptr; }
 
 // Optional per-launch HIP-event timing (eager launches only, never captured).
 struct KernelTimer {
@@ -199,13 +206,11 @@ int enqueue_step(sld_dev* d, double dt, int scheme, double phi, int tn, KernelTi
   else hipLaunchKernelGGL((sld::sld_predict<1>), dim3(nb), dim3(256), 0, s, N, dt, phi);
   TEND();
   TBEGIN(3);
-  hipLaunchKernelGGL(sld::sld_clock, dim3(1), dim3(64), 0, s, d->clock, dt);
+  hipLaunchKernelGGL(sld::sld_clock, dim3(1), dim3(64), 0, s, d->clock, dt,
+                     d->sur ? d->sur->count : (unsigned*)nullptr);
   TEND();
 
-  if (d->mode == SLD_MODE_SURROGATE) {
-    int rc = surrogate_enqueue_elements(d, dt, tn);
-    if (rc) return rc;
-  } else {
+  {
     sld::ElemArgs E{};
     E.nelem = d->nelem;
     E.lnods = d->lnods;
@@ -223,6 +228,44 @@ int enqueue_step(sld_dev* d, double dt, int scheme, double phi, int tn, KernelTi
     E.flags_out = d->flags_out;
     E.counters = d->counters;
     E.xcd_remap = d->xcd;
+    const bool sur = d->mode == SLD_MODE_SURROGATE;
+    if (sur) {
+      // prepass over all elements: flags and features of transforming points
+      sld_dev::Sur* S = d->sur;
+      E.sur_status = S->status;
+      E.sur_x = S->x;
+      E.sur_feat = S->feat;
+      E.sur_gp = S->gp;
+      E.sur_aux = S->aux;
+      E.sur_count = S->count;
+      E.T_feat = S->T;
+      E.e_begin = 0;
+      E.e_end = (int32_t)d->nelem;
+      TBEGIN(3);
+      launch_element_epb<float, false, 1>(d->epb, E, s);
+      TEND();
+      TBEGIN(3);
+      int rc = sma_mlp_infer_counted(S->mlp, S->feat, S->count, S->cap, S->Y, s);
+      TEND();
+      if (rc) return rc;
+      sld::SurCheckArgs C{};
+      C.count = S->count;
+      C.gp_of_slot = S->gp;
+      C.aux = S->aux;
+      C.feat = S->feat;
+      C.Y = S->Y;
+      for (int k = 0; k < 3; ++k) C.sdv_n[k] = d->sdv[tn][k];
+      C.props = d->props;
+      C.mat_of_elem = d->mat_of_elem;
+      C.E_of_elem = d->E_of_elem;
+      C.status = S->status;
+      C.x_out = S->x;
+      C.tol = S->tol;
+      C.cap = S->cap;
+      TBEGIN(3);
+      hipLaunchKernelGGL(sld::sld_sur_check, dim3(nblocks(S->cap, 256)), dim3(256), 0, s, C);
+      TEND();
+    }
     for (int c = 0; c < d->ncolor; ++c) {
       E.e_begin = d->color_ptr[c];
       E.e_end = d->color_ptr[c + 1];
@@ -232,6 +275,9 @@ int enqueue_step(sld_dev* d, double dt, int scheme, double phi, int tn, KernelTi
       if (d->mode == SLD_MODE_FP64) {
         if (st) launch_element_epb<double, true>(d->epb, E, s);
         else launch_element_epb<double, false>(d->epb, E, s);
+      } else if (sur) {
+        if (st) launch_element_epb<float, true, 2>(d->epb, E, s);
+        else launch_element_epb<float, false, 2>(d->epb, E, s);
       } else {
         if (st) launch_element_epb<float, true>(d->epb, E, s);
         else launch_element_epb<float, false>(d->epb, E, s);
@@ -301,7 +347,7 @@ double host_critical_dt(const double* coord, const int32_t* lnods, int64_t nelem
 
 extern "C" {
 
-const char* sma_last_error(void) { return g_err.c_str(); }
+const char* sma_last_error(void) { return sma_abi::last_error.c_str(); }
 
 int32_t sld_color_elements(int64_t nelem, int64_t npoin, const int32_t* lnods, int32_t* color) {
   if (nelem < 0 || npoin <= 0 || (!lnods && nelem) || (!color && nelem))
@@ -504,6 +550,37 @@ int sld_dev_create(const sld_mesh_desc* mesh, const sld_material_desc* mat,
                                mat->mat_of_elem, mat->E_of_elem);
 #undef TRY
   *out = d;
+  return SMA_OK;
+}
+
+int sld_dev_set_surrogate(sld_dev* d, sma_mlp* mlp, double tol) {
+  if (!d) return fail(SMA_ERR_ARG, "null handle");
+  HIPCHK(hipStreamSynchronize(d->stream));
+  free_graphs(d);
+  if (!mlp) {
+    surrogate_detach(d);
+    if (d->mode == SLD_MODE_SURROGATE) d->mode = SLD_MODE_FP32;
+    return SMA_OK;
+  }
+  if (!(tol > 0.0)) return fail(SMA_ERR_ARG, "tol must be positive");
+  if (!d->sur) {
+    auto* S = new sld_dev::Sur();
+    d->sur = S;
+    const int64_t ngp = d->nelem * 8;
+    S->cap = ngp;
+    int rc = 0;
+    if ((rc = dalloc(&S->status, ngp)) || (rc = dalloc(&S->x, ngp)) ||
+        (rc = dalloc(&S->feat, ngp * 16)) || (rc = dalloc(&S->gp, ngp)) ||
+        (rc = dalloc(&S->aux, ngp * 2)) || (rc = dalloc(&S->count, 1)) ||
+        (rc = dalloc(&S->Y, ngp * 12))) {
+      surrogate_detach(d);
+      return rc;
+    }
+    HIPCHK(hipMemset(S->count, 0, sizeof(unsigned)));
+    HIPCHK(hipMemset(S->status, 0, ngp));
+  }
+  d->sur->mlp = mlp;
+  d->sur->tol = tol;
   return SMA_OK;
 }
 

@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include "sma_law.hpp"
+#include "surrogate_features.hpp"
 
 namespace sld {
 
@@ -137,8 +138,13 @@ __global__ void __launch_bounds__(256) sld_predict(NodalArgs A, double dt, doubl
   }
 }
 
-__global__ void sld_clock(double* clock, double dt) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) clock[0] = clock[0] + dt;
+// Advances the device clock; also clears the surrogate's compact-list
+// counter for the step (a kernel, not a memset, so the step can be captured).
+__global__ void sld_clock(double* clock, double dt, unsigned* sur_count) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    clock[0] = clock[0] + dt;
+    if (sur_count) sur_count[0] = 0u;
+  }
 }
 
 template <int SCHEME>
@@ -189,6 +195,15 @@ struct ElemArgs {
   uint32_t* flags_out;          // optional [ngp]
   unsigned long long* counters; // [8] per-flag Gauss-point counts
   int32_t xcd_remap;            // 1 = spread consecutive tiles per XCD
+  // surrogate (mode 2): per-GP status (0 exact, 1 accepted, 2 fallback) and
+  // accepted xS; prepass compact list of flagged points
+  uint8_t* sur_status;
+  const double* sur_x;
+  float* sur_feat;              // [cap][16]
+  int32_t* sur_gp;              // [cap]
+  double* sur_aux;              // [cap][2] theta, e_n
+  unsigned* sur_count;
+  double T_feat;                // temperature fed to the surrogate (inert in the law)
 };
 
 template <typename R>
@@ -213,7 +228,10 @@ __device__ __forceinline__ void count_flags(unsigned long long* ctr, uint32_t fl
 
 // EPB elements per block, 8 threads (one per Gauss point) per element.
 // R = kinematics/stress precision; the return map is always FP64.
-template <typename R, int EPB, bool STORE>
+// SUR = 0 plain; 1 surrogate prepass (flags + features of transforming
+// points, no stress, no scatter); 2 element step using accepted surrogate
+// answers (exact return map otherwise).
+template <typename R, int EPB, bool STORE, int SUR = 0>
 __global__ void __launch_bounds__(EPB * 8) sld_element(ElemArgs A) {
   __shared__ double s_abs[EPB][8][6];     // gathered absolute x and u
   __shared__ R s_x[EPB][8][3];            // x_a - x_0
@@ -307,12 +325,41 @@ __global__ void __launch_bounds__(EPB * 8) sld_element(ElemArgs A) {
 #pragma unroll
       for (int q = 0; q < 9; ++q) Hf[q] = H[q];
       sma::Spectral<float> sp = sma::spectral_fp32(Hf);
-      rm = sma::return_map(m, sp.theta, sp.e_n, xS0, FAS0, FSA0);
-      sma::kirchhoff<float>(m, sp, rm.xS, tau);
+      if constexpr (SUR == 1) {
+        bool kf_AS, kf_SA;
+        double FAS, FSA;
+        const bool flagged = sma::flags_only(m, sp.theta, sp.e_n, xS0, FAS0, FSA0, kf_AS, kf_SA, FAS, FSA);
+        uint8_t st = 0;
+        if (flagged) {
+          st = 2;
+          if (kf_AS != kf_SA) {
+            const unsigned slot = atomicAdd(A.sur_count, 1u);
+            mlp::make_features(pr, E, A.T_feat, sp.e[0], sp.e[1], sp.e[2], sp.theta, xS0, FAS0,
+                               FSA0, FAS, A.sur_feat + (int64_t)slot * mlp::NIN);
+            A.sur_gp[slot] = (int32_t)gp;
+            A.sur_aux[2 * (int64_t)slot] = sp.theta;
+            A.sur_aux[2 * (int64_t)slot + 1] = sp.e_n;
+          }
+        }
+        A.sur_status[gp] = st;
+      } else if constexpr (SUR == 2) {
+        const uint8_t st = A.sur_status[gp];
+        if (st == 1) {
+          rm = sma::return_map_given(m, sp.theta, sp.e_n, xS0, FAS0, FSA0, A.sur_x[gp]);
+          if (rm.flags & 3u) rm.flags |= sma::FLAG_SURR_USED;
+        } else {
+          rm = sma::return_map(m, sp.theta, sp.e_n, xS0, FAS0, FSA0);
+          if (st == 2 && (rm.flags & 3u)) rm.flags |= sma::FLAG_SURR_FALLBACK;
+        }
+      } else {
+        rm = sma::return_map(m, sp.theta, sp.e_n, xS0, FAS0, FSA0);
+      }
+      if constexpr (SUR != 1) sma::kirchhoff<float>(m, sp, rm.xS, tau);
       if (!sp.eig_ok) rm.flags |= sma::FLAG_EIGEN_FAIL;
 #pragma unroll
       for (int q = 0; q < 9; ++q) Fm[q] = H[q] + ((q % 4 == 0) ? R(1) : R(0));
     }
+    if constexpr (SUR != 1) {
     A.sdv_k[0][gp] = rm.xS;
     A.sdv_k[1][gp] = rm.F_AS;
     A.sdv_k[2][gp] = rm.F_SA;
@@ -341,7 +388,9 @@ __global__ void __launch_bounds__(EPB * 8) sld_element(ElemArgs A) {
       for (int q = 0; q < 9; ++q) A.S_out[q * ngp + gp] = (double)S[q];
       A.flags_out[gp] = fl;
     }
+    }  // SUR != 1
   }
+  if constexpr (SUR == 1) return;   // uniform exit: the prepass has no scatter
   count_flags(A.counters, fl, valid);
   __syncthreads();
 
@@ -366,6 +415,46 @@ __global__ void __launch_bounds__(EPB * 8) sld_element(ElemArgs A) {
     fy[node] += f1;
     fz[node] += f2;
   }
+}
+
+// ----------------------------------------------------------------------------
+// Surrogate residual check on the compact list: accept the network's
+// transformation update when the consistency residual is within tol and the
+// inputs lie in the trained range; the element kernel uses accepted values.
+// ----------------------------------------------------------------------------
+struct SurCheckArgs {
+  const unsigned* count;
+  const int32_t* gp_of_slot;
+  const double* aux;            // theta, e_n
+  const float* feat;            // [slot][16]
+  const float* Y;               // [slot][12] network outputs
+  const double* sdv_n[3];
+  const double* props;
+  const int32_t* mat_of_elem;
+  const double* E_of_elem;
+  uint8_t* status;
+  double* x_out;
+  double tol;
+  int64_t cap;
+};
+
+__global__ void __launch_bounds__(256) sld_sur_check(SurCheckArgs A) {
+  const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot >= A.cap || slot >= (int64_t)A.count[0]) return;
+  const int64_t gp = A.gp_of_slot[slot];
+  const int64_t e = gp >> 3;
+  const int mat = A.mat_of_elem ? A.mat_of_elem[e] : 0;
+  const double* pr = A.props + 18 * mat;
+  const sma::Mat m = sma::make_mat(pr, A.E_of_elem ? A.E_of_elem[e] : pr[0]);
+  const double xS0 = A.sdv_n[0][gp], FAS0 = A.sdv_n[1][gp], FSA0 = A.sdv_n[2][gp];
+  const float* y = A.Y + slot * mlp::NOUT;
+  const double x = xS0 + (double)y[11] / mlp::Scale::dx;
+  bool in_range = true;
+  for (int k = 0; k < mlp::NIN; ++k) in_range &= fabsf(A.feat[slot * mlp::NIN + k]) <= 8.0f;
+  const double r = sma::consistency_residual(m, A.aux[2 * slot], A.aux[2 * slot + 1], xS0, FAS0, FSA0, x);
+  const bool ok = in_range && (r <= A.tol) && (x >= -1e-3) && (x <= 1.0 + 1e-3);
+  A.status[gp] = ok ? 1 : 2;
+  A.x_out[gp] = x;
 }
 
 // ----------------------------------------------------------------------------

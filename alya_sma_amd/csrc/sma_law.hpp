@@ -350,6 +350,51 @@ __host__ __device__ __forceinline__ ReturnMap return_map(const Mat& m, double th
   return r;
 }
 
+// Return map with the transformation update supplied by a surrogate: the
+// branch flags are evaluated exactly (:342-374); on a flagged point xS takes
+// the supplied value x_hat instead of the Newton iterate, and F_AS/F_SA are
+// the loading functions at x_hat (stored before the clamp, as :484-498).
+__host__ __device__ __forceinline__ ReturnMap return_map_given(const Mat& m, double theta, double e_n,
+                                                               double xS_0, double F_AS_0, double F_SA_0,
+                                                               double x_hat) {
+#pragma clang fp contract(off)
+  ReturnMap r;
+  bool kf_AS, kf_SA;
+  double F_AS, F_SA;
+  flags_only(m, theta, e_n, xS_0, F_AS_0, F_SA_0, kf_AS, kf_SA, F_AS, F_SA);
+  r.flags = (kf_AS ? FLAG_KF_AS : 0u) | (kf_SA ? FLAG_KF_SA : 0u);
+  double xS = xS_0;
+  if (kf_AS || kf_SA) {
+    xS = x_hat;
+    F_AS = loading(m, m.C_AS, theta, e_n, xS);
+    F_SA = loading(m, m.C_SA, theta, e_n, xS);
+  }
+  r.F_AS = F_AS;
+  r.F_SA = F_SA;
+  r.xS = fmax(fmin(1.0, xS), 0.0);
+  return r;
+}
+
+// Relative consistency residual of a candidate xS on a single-branch point:
+// the reference's Newton residual (:449-450) at dh = x - xS_0 (its one-step
+// solution makes it vanish), divided by the finish threshold R_f.  Returns
+// a large value for two-branch points (always solved exactly).
+__host__ __device__ __forceinline__ double consistency_residual(const Mat& m, double theta, double e_n,
+                                                                double xS_0, double F_AS_0, double F_SA_0,
+                                                                double x) {
+  bool kf_AS, kf_SA;
+  double F_AS, F_SA;
+  flags_only(m, theta, e_n, xS_0, F_AS_0, F_SA_0, kf_AS, kf_SA, F_AS, F_SA);
+  if (kf_AS == kf_SA) return 1e30;
+  const double d = x - xS_0;
+  if (kf_AS) {
+    const double Fx = loading(m, m.C_AS, theta, e_n, x);
+    return fabs((Fx - m.Rf_AS) * d + (Fx - F_AS_0) * (1.0 - x)) / fabs(m.Rf_AS);
+  }
+  const double Fx = loading(m, m.C_SA, theta, e_n, x);
+  return fabs((Fx - m.Rf_SA) * d - (Fx - F_SA_0) * x) / fabs(m.Rf_SA);
+}
+
 // Kirchhoff stress (:487-491) as a symmetric tensor t00 t01 t02 t11 t12 t22.
 template <typename R>
 __host__ __device__ __forceinline__ void kirchhoff(const Mat& m, const Spectral<R>& s, double xS,

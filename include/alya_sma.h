@@ -173,6 +173,50 @@ int sld_dev_sync(sld_dev *dev);
 int sld_dev_destroy(sld_dev *dev);
 
 /* ------------------------------------------------------------------------
+ * MLP surrogate of the return map (no counterpart in the reference; its
+ * closest analogue is the per-Gauss-point ANN hook ann_framework%eval,
+ * alya-ann/mod_ann_framework.f90:489-556).  16 -> 128 x4 -> 12, SiLU,
+ * BF16 weights and activations, FP32 accumulation on MFMA.
+ * Parameters in the canonical FP32 layout: W_l [n_out][n_in] for the five
+ * layers back to back (52736 floats), then the biases (524 floats).
+ * Feature and target definitions: alya_sma_amd/csrc/surrogate.hpp.
+ * ------------------------------------------------------------------------ */
+typedef struct sma_mlp sma_mlp;
+#define SMA_MLP_NIN      16
+#define SMA_MLP_NOUT     12
+#define SMA_MLP_NWEIGHT  52736
+#define SMA_MLP_NBIAS    524
+
+/* Allocate and He-initialise (seeded) on the current device. */
+int sma_mlp_create(uint64_t seed, sma_mlp **out);
+int sma_mlp_destroy(sma_mlp *mlp);
+/* Host FP32 canonical parameters in / out. */
+int sma_mlp_set_params(sma_mlp *mlp, const float *W, const float *b);
+int sma_mlp_get_params(sma_mlp *mlp, float *W, float *b);
+/* Per-output weights w_j of the loss (1/n) sum w_j (y_j - t_j)^2 (12 floats). */
+int sma_mlp_set_loss_weights(sma_mlp *mlp, const float *w12);
+/* Fused inference: X [n][16] -> Y [n][12], device pointers, async. */
+int sma_mlp_forward_device(sma_mlp *mlp, int64_t n, const float *X, float *Y, void *stream);
+/* Loss and gradient (canonical layout, device pointers) of a batch. */
+int sma_mlp_grad_device(sma_mlp *mlp, int64_t n, const float *X, const float *T,
+                        float *gW, float *gb, float *loss_out, void *stream);
+/* One Adam step on a batch (FP32 master weights, BF16 copies refreshed). */
+int sma_mlp_train_step_device(sma_mlp *mlp, int64_t n, const float *X, const float *T,
+                              double lr, float *loss_out, void *stream);
+/* Training samples from the exact FP64 return map: X [n][16], T [n][12],
+ * flags [n]; props_dev = base material [18] (device). */
+int sma_mlp_gen_samples_device(int64_t n, uint64_t seed, const double *props_dev,
+                               double strain_sd, double step_sd, double jitter,
+                               float *X, float *T, uint32_t *flags, void *stream);
+
+/* Attach a trained surrogate to a step handle (mode SLD_MODE_SURROGATE):
+ * transforming Gauss points take the network's transformation update when
+ * the relative consistency residual of the reference's Newton system is
+ * <= tol and the inputs are in the trained range; otherwise the exact return
+ * map runs (counted as SMA_FLAG_SURR_FALLBACK).  NULL detaches. */
+int sld_dev_set_surrogate(sld_dev *dev, sma_mlp *mlp, double tol);
+
+/* ------------------------------------------------------------------------
  * Host helpers (no GPU needed).
  * ------------------------------------------------------------------------ */
 /* Structured box of nx*ny*nz HEX08 over [0,lx]x[0,ly]x[0,lz]; node (i,j,k)
