@@ -73,11 +73,17 @@ def lib():
     L.sld_dev_get.argtypes = [p, i32, p, i64]
     L.sld_dev_set.argtypes = [p, i32, p, i64]
     L.sld_dev_coloring.argtypes = [p, p, p, i32, p]
-    L.sld_dev_set_tile.argtypes = [p, i32, i32]
+    L.sld_dev_set_tile.argtypes = [p, i32, i32, i32]
+    L.sld_dev_autotune.argtypes = [p, d, i32, d, i32, p, p]
     L.sld_dev_stream.argtypes = [p]
     L.sld_dev_stream.restype = p
     L.sld_dev_sync.argtypes = [p]
     L.sld_dev_destroy.argtypes = [p]
+    L.sma_device_malloc.argtypes = [C.POINTER(p), i64]
+    L.sma_device_free.argtypes = [p]
+    L.sma_memcpy.argtypes = [p, p, i64, i32]
+    L.sma_device_synchronize.argtypes = []
+    L.sma_device_count.argtypes = [C.POINTER(i32)]
     L.sld_mesh_box.argtypes = [i32, i32, i32, d, d, d, p, p]
     L.sld_color_elements.argtypes = [i64, i64, p, p]
     L.sld_color_elements.restype = i32

@@ -45,10 +45,10 @@ unsigned nblocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 struct GraphKey {
   double dt, phi;
-  int scheme, mode, store, epb, xcd, tn;
+  int scheme, mode, store, epb, xcd, persistent, tn;
   bool operator<(const GraphKey& o) const {
-    return std::tie(dt, phi, scheme, mode, store, epb, xcd, tn) <
-           std::tie(o.dt, o.phi, o.scheme, o.mode, o.store, o.epb, o.xcd, o.tn);
+    return std::tie(dt, phi, scheme, mode, store, epb, xcd, persistent, tn) <
+           std::tie(o.dt, o.phi, o.scheme, o.mode, o.store, o.epb, o.xcd, o.persistent, o.tn);
   }
 };
 
@@ -88,7 +88,7 @@ struct sld_dev {
   uint32_t* flags_out = nullptr;
   unsigned long long* counters = nullptr;
   double* clock = nullptr;
-  int mode = SLD_MODE_FP64, store = 0, epb = 32, xcd = 1, use_graph = 1;
+  int mode = SLD_MODE_FP64, store = 0, epb = 32, xcd = 1, persistent = 0, use_graph = 1;
   double dtcrit = 0.0;
   std::map<GraphKey, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
   std::vector<void*> allocs;
@@ -140,19 +140,31 @@ sld::NodalArgs nodal_args(sld_dev* d) {
   return A;
 }
 
+int g_num_cu = 0;
+
 template <typename R, int EPB, bool STORE, int SUR>
-void launch_element(const sld::ElemArgs& A, hipStream_t s) {
+void launch_element(sld::ElemArgs A, hipStream_t s, int persistent) {
   const int ne = A.e_end - A.e_begin;
   if (ne <= 0) return;
-  hipLaunchKernelGGL((sld::sld_element<R, EPB, STORE, SUR>), dim3(nblocks(ne, EPB)), dim3(EPB * 8), 0,
-                     s, A);
+  const int ntiles = (int)nblocks(ne, EPB);
+  int grid = ntiles;
+  A.tiles_per_block = 0;
+  if (persistent > 0) {
+    // persistent: `persistent` workgroups per CU, each walking a run of tiles
+    const int cap = std::max(1, g_num_cu) * persistent;
+    if (ntiles > cap) {
+      A.tiles_per_block = (ntiles + cap - 1) / cap;
+      grid = (ntiles + A.tiles_per_block - 1) / A.tiles_per_block;
+    }
+  }
+  hipLaunchKernelGGL((sld::sld_element<R, EPB, STORE, SUR>), dim3(grid), dim3(EPB * 8), 0, s, A);
 }
 
 template <typename R, bool STORE, int SUR = 0>
-void launch_element_epb(int epb, const sld::ElemArgs& A, hipStream_t s) {
-  if (epb == 8) launch_element<R, 8, STORE, SUR>(A, s);
-  else if (epb == 16) launch_element<R, 16, STORE, SUR>(A, s);
-  else launch_element<R, 32, STORE, SUR>(A, s);
+void launch_element_epb(int epb, const sld::ElemArgs& A, hipStream_t s, int persistent) {
+  if (epb == 8) launch_element<R, 8, STORE, SUR>(A, s, persistent);
+  else if (epb == 16) launch_element<R, 16, STORE, SUR>(A, s, persistent);
+  else launch_element<R, 32, STORE, SUR>(A, s, persistent);
 }
 
 bool surrogate_attached(const sld_dev* d) { return d->sur != nullptr && d->sur->mlp != nullptr; }
@@ -242,7 +254,7 @@ int enqueue_step(sld_dev* d, double dt, int scheme, double phi, int tn, KernelTi
       E.e_begin = 0;
       E.e_end = (int32_t)d->nelem;
       TBEGIN(3);
-      launch_element_epb<float, false, 1>(d->epb, E, s);
+      launch_element_epb<float, false, 1>(d->epb, E, s, d->persistent);
       TEND();
       TBEGIN(3);
       int rc = sma_mlp_infer_counted(S->mlp, S->feat, S->count, S->cap, S->Y, s);
@@ -273,14 +285,14 @@ int enqueue_step(sld_dev* d, double dt, int scheme, double phi, int tn, KernelTi
       if (E.e_end <= E.e_begin) continue;
       TBEGIN(1);
       if (d->mode == SLD_MODE_FP64) {
-        if (st) launch_element_epb<double, true>(d->epb, E, s);
-        else launch_element_epb<double, false>(d->epb, E, s);
+        if (st) launch_element_epb<double, true>(d->epb, E, s, d->persistent);
+        else launch_element_epb<double, false>(d->epb, E, s, d->persistent);
       } else if (sur) {
-        if (st) launch_element_epb<float, true, 2>(d->epb, E, s);
-        else launch_element_epb<float, false, 2>(d->epb, E, s);
+        if (st) launch_element_epb<float, true, 2>(d->epb, E, s, d->persistent);
+        else launch_element_epb<float, false, 2>(d->epb, E, s, d->persistent);
       } else {
-        if (st) launch_element_epb<float, true>(d->epb, E, s);
-        else launch_element_epb<float, false>(d->epb, E, s);
+        if (st) launch_element_epb<float, true>(d->epb, E, s, d->persistent);
+        else launch_element_epb<float, false>(d->epb, E, s, d->persistent);
       }
       TEND();
     }
@@ -453,6 +465,9 @@ int sld_dev_create(const sld_mesh_desc* mesh, const sld_material_desc* mat,
     return rc;
   };
   if (hipGetDevice(&d->device) != hipSuccess) return bail(fail(SMA_ERR_HIP, "no HIP device"));
+  if (g_num_cu == 0 &&
+      hipDeviceGetAttribute(&g_num_cu, hipDeviceAttributeMultiprocessorCount, d->device) != hipSuccess)
+    g_num_cu = 256;
   if (hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking) != hipSuccess)
     return bail(fail(SMA_ERR_HIP, "hipStreamCreate failed"));
   hipStream_t s = d->stream;
@@ -612,11 +627,13 @@ int sld_dev_set_store(sld_dev* d, int32_t store) {
   return SMA_OK;
 }
 
-int sld_dev_set_tile(sld_dev* d, int32_t epb, int32_t xcd) {
+int sld_dev_set_tile(sld_dev* d, int32_t epb, int32_t xcd, int32_t persistent) {
   if (!d) return fail(SMA_ERR_ARG, "null handle");
   if (epb != 8 && epb != 16 && epb != 32) return fail(SMA_ERR_ARG, "elems_per_block must be 8, 16 or 32");
+  if (persistent < 0 || persistent > 64) return fail(SMA_ERR_ARG, "persistent must be 0..64 workgroups per CU");
   d->epb = epb;
   d->xcd = xcd ? 1 : 0;
+  d->persistent = persistent;
   return SMA_OK;
 }
 
@@ -629,7 +646,7 @@ int sld_dev_step(sld_dev* d, double dt, int32_t scheme, double phi, int64_t nste
     return fail(SMA_ERR_STATE, "surrogate mode without a surrogate");
   int64_t left = nsteps;
   if (d->use_graph && left >= 2) {
-    GraphKey key{dt, phi, scheme, d->mode, d->store, d->epb, d->xcd, d->cur};
+    GraphKey key{dt, phi, scheme, d->mode, d->store, d->epb, d->xcd, d->persistent, d->cur};
     auto it = d->graphs.find(key);
     if (it == d->graphs.end()) {
       hipGraph_t g = nullptr;
@@ -696,6 +713,85 @@ int sld_dev_time_kernels(sld_dev* d, double dt, int32_t scheme, double phi, int3
   return SMA_OK;
 }
 
+int sld_dev_autotune(sld_dev* d, double dt, int32_t scheme, double phi, int32_t nsteps,
+                     int32_t* best, double* best_ms) {
+  if (!d) return fail(SMA_ERR_ARG, "null handle");
+  if (scheme != 0 && scheme != 1) return fail(SMA_ERR_ARG, "scheme must be 0 (CD) or 1 (TW)");
+  if (!(dt > 0.0) || nsteps <= 0) return fail(SMA_ERR_ARG, "dt > 0 and nsteps > 0 required");
+  if (d->mode == SLD_MODE_SURROGATE && !surrogate_attached(d))
+    return fail(SMA_ERR_STATE, "surrogate mode without a surrogate");
+  hipStream_t s = d->stream;
+  HIPCHK(hipStreamSynchronize(s));
+  // snapshot of everything a step writes, restored before every trial and at the end
+  const int64_t np = d->npoin, ngp = d->nelem * 8;
+  std::vector<std::pair<void*, size_t>> live;
+  for (int k = 0; k < 3; ++k) {
+    live.push_back({d->u[k], np * 8});
+    live.push_back({d->v[k], np * 8});
+    live.push_back({d->a[k], np * 8});
+    for (int sl = 0; sl < 2; ++sl) live.push_back({d->sdv[sl][k], ngp * 8});
+  }
+  live.push_back({d->clock, 8});
+  live.push_back({d->counters, 64});
+  std::vector<void*> snap(live.size(), nullptr);
+  auto cleanup = [&]() { for (void* p : snap) if (p) (void)hipFree(p); };
+  for (size_t i = 0; i < live.size(); ++i) {
+    if (hipMalloc(&snap[i], live[i].second) != hipSuccess) { cleanup(); return fail(SMA_ERR_HIP, "autotune snapshot alloc"); }
+    if (hipMemcpyAsync(snap[i], live[i].first, live[i].second, hipMemcpyDeviceToDevice, s) != hipSuccess) {
+      cleanup();
+      return fail(SMA_ERR_HIP, "autotune snapshot copy");
+    }
+  }
+  const int cur0 = d->cur;
+  auto restore = [&]() -> int {
+    for (size_t i = 0; i < live.size(); ++i)
+      HIPCHK(hipMemcpyAsync(live[i].first, snap[i], live[i].second, hipMemcpyDeviceToDevice, s));
+    d->cur = cur0;
+    return 0;
+  };
+  const int epbs[3] = {8, 16, 32}, pers[3] = {0, 2, 4};
+  int b_epb = d->epb, b_xcd = d->xcd, b_per = d->persistent;
+  double b_ms = 1e300;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  int rc = 0;
+  for (int ie = 0; ie < 3 && !rc; ++ie)
+    for (int x = 0; x < 2 && !rc; ++x)
+      for (int ip = 0; ip < 3 && !rc; ++ip) {
+        d->epb = epbs[ie];
+        d->xcd = x;
+        d->persistent = pers[ip];
+        if ((rc = restore())) break;
+        if ((rc = enqueue_step(d, dt, scheme, phi, d->cur))) break;   // warm-up
+        d->cur = 1 - d->cur;
+        if (hipEventRecord(e0, s) != hipSuccess) { rc = fail(SMA_ERR_HIP, "event"); break; }
+        for (int k = 0; k < nsteps && !rc; ++k) {
+          rc = enqueue_step(d, dt, scheme, phi, d->cur);
+          d->cur = 1 - d->cur;
+        }
+        if (rc) break;
+        float ms = 0.f;
+        if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess) { rc = fail(SMA_ERR_HIP, "event timing"); break; }
+        const double per_step = (double)ms / nsteps;
+        if (per_step < b_ms) { b_ms = per_step; b_epb = d->epb; b_xcd = d->xcd; b_per = d->persistent; }
+      }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  int rr = restore();
+  (void)hipStreamSynchronize(s);
+  cleanup();
+  d->epb = b_epb;
+  d->xcd = b_xcd;
+  d->persistent = b_per;
+  if (rc) return rc;
+  if (rr) return rr;
+  if (best) { best[0] = b_epb; best[1] = b_xcd; best[2] = b_per; }
+  if (best_ms) *best_ms = b_ms;
+  return SMA_OK;
+}
+
 double sld_dev_critical_dt(const sld_dev* d) { return d ? d->dtcrit : -1.0; }
 
 void* sld_dev_stream(sld_dev* d) { return d ? (void*)d->stream : nullptr; }
@@ -750,7 +846,7 @@ int sld_dev_get(sld_dev* d, int32_t field, void* host, int64_t count) {
     case SLD_FIELD_SDV: {
       if (need(3 * ngp)) return SMA_ERR_ARG;
       double* out = (double*)host;
-      const int k_slot = 1 - d->cur;  // ITER_K of the last step
+      const int k_slot = d->cur;  // after a step the slot just written holds TIME_N = ITER_K
       tmp.resize(ngp);
       for (int k = 0; k < 3; ++k) {
         HIPCHK(hipMemcpy(tmp.data(), d->sdv[k_slot][k], ngp * sizeof(double), hipMemcpyDeviceToHost));
@@ -830,6 +926,40 @@ int sld_dev_set(sld_dev* d, int32_t field, const void* host, int64_t count) {
     default:
       return fail(SMA_ERR_ARG, "field cannot be set");
   }
+}
+
+int sma_device_malloc(void** p, int64_t bytes) {
+  if (!p || bytes < 0) return fail(SMA_ERR_ARG, "sma_device_malloc: bad arguments");
+  *p = nullptr;
+  HIPCHK(hipMalloc(p, bytes > 0 ? (size_t)bytes : 1));
+  return SMA_OK;
+}
+
+int sma_device_free(void* p) {
+  if (p) HIPCHK(hipFree(p));
+  return SMA_OK;
+}
+
+int sma_memcpy(void* dst, const void* src, int64_t bytes, int32_t kind) {
+  if (bytes < 0 || (bytes > 0 && (!dst || !src))) return fail(SMA_ERR_ARG, "sma_memcpy: bad arguments");
+  if (bytes == 0) return SMA_OK;
+  const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : kind == 1 ? hipMemcpyDeviceToHost
+                                                             : hipMemcpyDeviceToDevice;
+  HIPCHK(hipMemcpy(dst, src, (size_t)bytes, k));
+  return SMA_OK;
+}
+
+int sma_device_synchronize(void) {
+  HIPCHK(hipDeviceSynchronize());
+  return SMA_OK;
+}
+
+int sma_device_count(int32_t* n) {
+  if (!n) return fail(SMA_ERR_ARG, "null argument");
+  int c = 0;
+  hipError_t e = hipGetDeviceCount(&c);
+  *n = (e == hipSuccess) ? c : 0;
+  return SMA_OK;
 }
 
 int sld_dev_destroy(sld_dev* d) {

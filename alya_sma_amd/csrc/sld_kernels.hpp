@@ -29,19 +29,19 @@ namespace sld {
 // 0.5 * s_ak * f(a,g,m1) * f(a,g,m2) where f = 0.5*(1 +- 0.577350269189626)
 // depending on whether node a and GP g lie on the same side in direction m
 // (mod_lagrange.f90:935-1000, mod_quadrature_gauss.f90:2320-2321,2372-2385).
-__device__ __forceinline__ int node_sign(int a, int k) {
+__host__ __device__ __forceinline__ int node_sign(int a, int k) {
   // node order (-,-,-) (+,-,-) (+,+,-) (-,+,-) (-,-,+) (+,-,+) (+,+,+) (-,+,+)
   const int sx = 0x66, sy = 0xCC, sz = 0xF0;  // bit a set = +1
   int m = (k == 0) ? sx : (k == 1) ? sy : sz;
   return ((m >> a) & 1) ? 1 : -1;
 }
-__device__ __forceinline__ int gp_sign(int g, int k) {
+__host__ __device__ __forceinline__ int gp_sign(int g, int k) {
   // GP order: xi outer, eta, zeta inner; bit = 1 means +0.5773...
   int bit = (k == 0) ? (g >> 2) & 1 : (k == 1) ? (g >> 1) & 1 : g & 1;
   return bit ? 1 : -1;
 }
 template <typename R>
-__device__ __forceinline__ R hex_deriv(int a, int g, int k) {
+__host__ __device__ __forceinline__ R hex_deriv(int a, int g, int k) {
   const double gpos = 0.577350269189626;
   const R cp = (R)(0.5 * (1.0 + gpos)), cm = (R)(0.5 * (1.0 - gpos));
   R f[3];
@@ -53,7 +53,7 @@ __device__ __forceinline__ R hex_deriv(int a, int g, int k) {
   return s * f[0] * f[1];
 }
 template <typename R>
-__device__ __forceinline__ R hex_shape(int a, int g) {
+__host__ __device__ __forceinline__ R hex_shape(int a, int g) {
   const double gpos = 0.577350269189626;
   const R cp = (R)(0.5 * (1.0 + gpos)), cm = (R)(0.5 * (1.0 - gpos));
   R f0 = (node_sign(a, 0) == gp_sign(g, 0)) ? cp : cm;
@@ -195,6 +195,7 @@ struct ElemArgs {
   uint32_t* flags_out;          // optional [ngp]
   unsigned long long* counters; // [8] per-flag Gauss-point counts
   int32_t xcd_remap;            // 1 = spread consecutive tiles per XCD
+  int32_t tiles_per_block;      // > 0: persistent launch, tiles per workgroup
   // surrogate (mode 2): per-GP status (0 exact, 1 accepted, 2 fallback) and
   // accepted xS; prepass compact list of flagged points
   uint8_t* sur_status;
@@ -241,16 +242,20 @@ __global__ void __launch_bounds__(EPB * 8) sld_element(ElemArgs A) {
 
   const int tid = threadIdx.x;
   const int le = tid >> 3, g = tid & 7;
-  int blk = blockIdx.x;
+  int bid = blockIdx.x;
   if (A.xcd_remap) {
     // workgroups are dealt round-robin to the 8 XCDs: give each XCD a
     // contiguous run of tiles so neighbouring elements share its L2
     const int nb = gridDim.x, per = (nb + 7) / 8;
-    const int xcd = blk & 7, k = blk >> 3;
-    const int cand = xcd * per + k;
+    const int xcd = bid & 7, k = bid >> 3;
     // only remap when the grid divides evenly (keeps the map a bijection)
-    if (nb % 8 == 0) blk = cand;
+    if (nb % 8 == 0) bid = xcd * per + k;
   }
+  // persistent launches give each workgroup a contiguous run of tiles
+  const int ntiles = (A.e_end - A.e_begin + EPB - 1) / EPB;
+  const int per = A.tiles_per_block > 0 ? A.tiles_per_block : 1;
+  const int t_begin = bid * per, t_end = min(ntiles, t_begin + per);
+  for (int blk = t_begin; blk < t_end; ++blk) {
   const int e = A.e_begin + blk * EPB + le;
   const bool valid = e < A.e_end;
 
@@ -390,7 +395,7 @@ __global__ void __launch_bounds__(EPB * 8) sld_element(ElemArgs A) {
     }
     }  // SUR != 1
   }
-  if constexpr (SUR == 1) return;   // uniform exit: the prepass has no scatter
+  if constexpr (SUR == 1) continue;   // uniform: the prepass has no scatter
   count_flags(A.counters, fl, valid);
   __syncthreads();
 
@@ -415,6 +420,7 @@ __global__ void __launch_bounds__(EPB * 8) sld_element(ElemArgs A) {
     fy[node] += f1;
     fz[node] += f2;
   }
+  }  // tile loop
 }
 
 // ----------------------------------------------------------------------------

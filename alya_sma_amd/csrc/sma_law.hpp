@@ -125,7 +125,8 @@ __host__ __device__ __forceinline__ bool eig3(R a00, R a01, R a02, R a11, R a12,
     R f2 = a12 * u1 + a22 * u2;          // I = 3: A(2,3)U(2) + A(3,3)U(3)
     R p2 = omega * f2;
     K = K + u2 * f2;
-    K = R(0.5) * K * (omega * omega);
+    if constexpr (sizeof(R) == 8) K = R(0.5) * K * (omega * omega);   // reference order
+    else K = R(0.5) * (K * omega) * omega;                            // no FP32 overflow
     p1 = p1 - K * u1;
     p2 = p2 - K * u2;
     w0 = a00;
@@ -252,8 +253,17 @@ __host__ __device__ __forceinline__ Spectral<float> spectral_fp32(const float (&
   float c01 = H[3] + H[1] + (H[0] * H[1] + H[3] * H[4] + H[6] * H[7]);
   float c02 = H[6] + H[2] + (H[0] * H[2] + H[3] * H[5] + H[6] * H[8]);
   float c12 = H[7] + H[5] + (H[1] * H[2] + H[4] * H[5] + H[7] * H[8]);
+  // scale to unit magnitude so no intermediate of the FP32 Householder/QL
+  // steps under- or overflows for strains from 1e-12 to 1
+  float sc = fmaxf(fmaxf(fmaxf(fabsf(c00), fabsf(c11)), fmaxf(fabsf(c22), fabsf(c01))),
+                   fmaxf(fabsf(c02), fabsf(c12)));
+  const float inv = sc > 0.f ? 1.0f / sc : 0.f;
   float mu0, mu1, mu2;
-  s.eig_ok = eig3<float>(c00, c01, c02, c11, c12, c22, mu0, mu1, mu2, s.n);
+  s.eig_ok = eig3<float>(c00 * inv, c01 * inv, c02 * inv, c11 * inv, c12 * inv, c22 * inv,
+                         mu0, mu1, mu2, s.n);
+  mu0 *= sc;
+  mu1 *= sc;
+  mu2 *= sc;
   float l0 = 0.5f * log1pf(mu0), l1 = 0.5f * log1pf(mu1), l2 = 0.5f * log1pf(mu2);
   float th = l0 + l1 + l2;
   float t3 = th * (1.0f / 3.0f);

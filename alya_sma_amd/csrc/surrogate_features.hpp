@@ -17,7 +17,7 @@ constexpr int NIN = 16, NH = 128, NOUT = 12;
 // ----------------------------------------------------------------------------
 struct Scale {
   static constexpr float e = 1.f / 0.05f, th = 1.f / 0.02f, F = 1.f / 1000.f, tau = 1.f / 1000.f;
-  static constexpr float etr = 1.f / 0.05f, dx = 10.f;
+  static constexpr float etr = 1.f / 0.05f, dx = 1.f;
 };
 
 __host__ __device__ __forceinline__ void make_features(const double* p, double E, double T, double e0,

@@ -7,7 +7,8 @@
 ! the GPU through mod_sld_dev, in place of Alya's time loop
 ! (Timste/Begste/Doiter/Endste) around sld_explicit.
 !
-! usage: sld_bar_driver NSTEPS DT SCHEME OUTFILE
+! usage: sld_bar_driver NSTEPS DT SCHEME OUTFILE [NRUN]
+!   NSTEPS sets the load cycle length, NRUN (default NSTEPS) the steps run
 ! Output (stream): real64 u(3,npoin); real64 sdv(3,8,nelem); real64 fint(3,npoin)
 ! and one line per 100 steps on stdout: step, time, tip u_x, tip reaction.
 !-----------------------------------------------------------------------
@@ -17,7 +18,7 @@ program sld_bar_driver
   implicit none
   integer(c_int32_t), parameter :: nx = 10, ny = 2, nz = 2
   real(c_double), parameter :: lx = 10.0_8, ly = 2.0_8, lz = 2.0_8, strain = 0.06_8
-  integer :: npoin, nelem, ip, istep, nsteps, uout, scheme
+  integer :: npoin, nelem, ip, istep, nsteps, nrun, uout, scheme
   real(c_double), allocatable, target :: coord(:,:), props(:), rho(:), table(:,:), u(:,:), f(:,:), sdv(:,:,:)
   integer(c_int32_t), allocatable, target :: lnods(:,:), func_of_node(:), npts(:), smooth(:)
   integer(c_int8_t), allocatable, target :: fixmask(:)
@@ -33,6 +34,10 @@ program sld_bar_driver
   call get_command_argument(1, arg); read(arg, *) nsteps
   call get_command_argument(2, arg); read(arg, *) dt
   call get_command_argument(3, arg); read(arg, *) scheme
+  nrun = nsteps
+  if (command_argument_count() >= 5) then
+     call get_command_argument(5, arg); read(arg, *) nrun
+  end if
   npoin = (nx + 1) * (ny + 1) * (nz + 1)
   nelem = nx * ny * nz
   allocate(coord(3, npoin), lnods(8, nelem), fixmask(npoin), func_of_node(npoin))
@@ -75,9 +80,9 @@ program sld_bar_driver
   call sld_dev_check(sld_dev_create(mesh, mat, bc, dev), 'sld_dev_create')
 
   allocate(u(3, npoin), f(3, npoin), sdv(3, 8, nelem))
-  do istep = 1, nsteps, 100
+  do istep = 1, nrun, 100
      call sld_dev_check(sld_dev_step(dev, dt, int(scheme, c_int32_t), 1.033_8, &
-                                     int(min(100, nsteps - istep + 1), c_int64_t)), 'sld_dev_step')
+                                     int(min(100, nrun - istep + 1), c_int64_t)), 'sld_dev_step')
      call sld_dev_check(sld_dev_get(dev, SLD_FIELD_U, c_loc(u), int(3*npoin, c_int64_t)), 'get u')
      call sld_dev_check(sld_dev_get(dev, SLD_FIELD_FINT, c_loc(f), int(3*npoin, c_int64_t)), 'get fint')
      call sld_dev_check(sld_dev_get(dev, SLD_FIELD_TIME, c_loc(t), 1_c_int64_t), 'get time')
@@ -88,7 +93,7 @@ program sld_bar_driver
            tipf = tipf + f(1, ip)
         end if
      end do
-     write(*, '(i6,3es16.8)') min(istep + 99, nsteps), t(1), tipu, tipf
+     write(*, '(i6,3es16.8)') min(istep + 99, nrun), t(1), tipu, tipf
   end do
   call sld_dev_check(sld_dev_get(dev, SLD_FIELD_SDV, c_loc(sdv), int(24*nelem, c_int64_t)), 'get sdv')
   call get_command_argument(4, arg)

@@ -120,9 +120,18 @@ class Solidz:
     def set_store(self, on=True):
         check(lib().sld_dev_set_store(self.h, 1 if on else 0), "sld_dev_set_store")
 
-    def set_tile(self, elems_per_block, xcd_remap=True):
-        check(lib().sld_dev_set_tile(self.h, int(elems_per_block), 1 if xcd_remap else 0),
-              "sld_dev_set_tile")
+    def set_tile(self, elems_per_block, xcd_remap=True, persistent=0):
+        check(lib().sld_dev_set_tile(self.h, int(elems_per_block), 1 if xcd_remap else 0,
+                                     int(persistent)), "sld_dev_set_tile")
+
+    def autotune(self, dt, nsteps=3, scheme=0, phi=1.033):
+        """Pick the fastest element-kernel launch shape; state is unchanged."""
+        best = np.zeros(3, dtype=np.int32)
+        ms = C.c_double()
+        check(lib().sld_dev_autotune(self.h, float(dt), int(scheme), float(phi), int(nsteps),
+                                     _p(best), C.byref(ms)), "sld_dev_autotune")
+        return {"elems_per_block": int(best[0]), "xcd_remap": int(best[1]),
+                "persistent": int(best[2]), "ms_per_step": ms.value}
 
     def step(self, dt, nsteps=1, scheme=0, phi=1.033):
         check(lib().sld_dev_step(self.h, float(dt), int(scheme), float(phi), int(nsteps)),

@@ -90,11 +90,11 @@ def main():
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU "
               "sharding of this path is described in DESIGN.md and not run here.", flush=True)
         sys.exit(2)
-    import torch
     from alya_sma_amd import problems
+    from alya_sma_amd.device import device_count, synchronize
     from alya_sma_amd.solid import Solidz
 
-    if not torch.cuda.is_available():
+    if device_count() < 1:
         sys.exit("bench.py needs a HIP device")
     prob = problems.cantilever_problem()
     r = prob["run"]
@@ -112,11 +112,11 @@ def main():
             dev.set_tile(args.epb)
         dev.step(dt, nsteps=W)
         dev.sync()
-        torch.cuda.synchronize()
+        synchronize()
         t0 = time.perf_counter()
         dev.step(dt, nsteps=K)
         dev.sync()
-        torch.cuda.synchronize()
+        synchronize()
         el = time.perf_counter() - t0
         ms = 1e3 * el / K
         # HIP-event timing of each launch over a few further (eager) steps

@@ -161,8 +161,18 @@ int sld_dev_set(sld_dev *dev, int32_t field, const void *host, int64_t count);
 int sld_dev_coloring(const sld_dev *dev, int32_t *ncolor, int32_t *color_ptr,
                      int32_t max_colors, int32_t *elem_order);
 
-/* Elements per block for the element kernel (8, 16 or 32; autotuned). */
-int sld_dev_set_tile(sld_dev *dev, int32_t elems_per_block, int32_t xcd_remap);
+/* Launch shape of the element kernel: elements per workgroup (8, 16 or 32),
+ * XCD-contiguous tile mapping on/off, persistent = workgroups per CU of a
+ * persistent launch (0 = one workgroup per tile).  sld_dev_autotune picks
+ * these by timing. */
+int sld_dev_set_tile(sld_dev *dev, int32_t elems_per_block, int32_t xcd_remap, int32_t persistent);
+
+/* Autotuner: times nsteps eager steps for every launch shape (elements per
+ * workgroup 8/16/32 x XCD mapping on/off x persistent 0/2/4 workgroups per
+ * CU) from the same saved state, restores the state, keeps the fastest.
+ * best[3] = (elems_per_block, xcd_remap, persistent); best_ms = ms/step. */
+int sld_dev_autotune(sld_dev *dev, double dt, int32_t scheme, double phi, int32_t nsteps,
+                     int32_t *best, double *best_ms);
 
 /* Stream the handle enqueues on (a hipStream_t). */
 void *sld_dev_stream(sld_dev *dev);
@@ -215,6 +225,17 @@ int sma_mlp_gen_samples_device(int64_t n, uint64_t seed, const double *props_dev
  * <= tol and the inputs are in the trained range; otherwise the exact return
  * map runs (counted as SMA_FLAG_SURR_FALLBACK).  NULL detaches. */
 int sld_dev_set_surrogate(sld_dev *dev, sma_mlp *mlp, double tol);
+
+/* ------------------------------------------------------------------------
+ * Device memory helpers for hosts without their own HIP allocator (the
+ * Python and Fortran hosts use these for device-pointer entry points).
+ * kind: 0 host->device, 1 device->host, 2 device->device.
+ * ------------------------------------------------------------------------ */
+int sma_device_malloc(void **p, int64_t bytes);
+int sma_device_free(void *p);
+int sma_memcpy(void *dst, const void *src, int64_t bytes, int32_t kind);
+int sma_device_synchronize(void);
+int sma_device_count(int32_t *n);
 
 /* ------------------------------------------------------------------------
  * Host helpers (no GPU needed).
