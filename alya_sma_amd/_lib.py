@@ -84,6 +84,7 @@ def lib():
     L.sma_memcpy.argtypes = [p, p, i64, i32]
     L.sma_device_synchronize.argtypes = []
     L.sma_device_count.argtypes = [C.POINTER(i32)]
+    L.sma_copy_probe.argtypes = [p, p, i64, p]
     L.sld_mesh_box.argtypes = [i32, i32, i32, d, d, d, p, p]
     L.sld_color_elements.argtypes = [i64, i64, p, p]
     L.sld_color_elements.restype = i32

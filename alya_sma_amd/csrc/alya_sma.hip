@@ -949,6 +949,22 @@ int sma_memcpy(void* dst, const void* src, int64_t bytes, int32_t kind) {
   return SMA_OK;
 }
 
+// A plain streaming copy (16 B per lane per iteration), the reference point
+// for HBM byte counters of the other kernels.
+__global__ void sma_copy_kernel(const double2* __restrict__ src, double2* __restrict__ dst, int64_t n2) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+int sma_copy_probe(const void* src, void* dst, int64_t bytes, void* stream) {
+  if (!src || !dst || bytes <= 0 || bytes % 16) return fail(SMA_ERR_ARG, "sma_copy_probe: bytes must be a positive multiple of 16");
+  const int64_t n2 = bytes / 16;
+  hipLaunchKernelGGL(sma_copy_kernel, dim3(2048), dim3(256), 0, (hipStream_t)stream,
+                     (const double2*)src, (double2*)dst, n2);
+  HIPCHK(hipGetLastError());
+  return SMA_OK;
+}
+
 int sma_device_synchronize(void) {
   HIPCHK(hipDeviceSynchronize());
   return SMA_OK;

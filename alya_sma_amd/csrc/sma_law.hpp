@@ -257,7 +257,9 @@ __host__ __device__ __forceinline__ Spectral<float> spectral_fp32(const float (&
   // steps under- or overflows for strains from 1e-12 to 1
   float sc = fmaxf(fmaxf(fmaxf(fabsf(c00), fabsf(c11)), fmaxf(fabsf(c22), fabsf(c01))),
                    fmaxf(fabsf(c02), fabsf(c12)));
-  const float inv = sc > 0.f ? 1.0f / sc : 0.f;
+  // below 1e-30 the strain is zero to FP32 precision; a denormal scale would
+  // overflow its reciprocal
+  const float inv = sc > 1e-30f ? 1.0f / sc : 0.f;
   float mu0, mu1, mu2;
   s.eig_ok = eig3<float>(c00 * inv, c01 * inv, c02 * inv, c11 * inv, c12 * inv, c22 * inv,
                          mu0, mu1, mu2, s.n);

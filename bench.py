@@ -37,6 +37,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--epb", type=int, default=0, help="elements per block (0 = default)")
+    ap.add_argument("--copy-probe-mb", type=int, default=256,
+                    help="size of the calibration copy kernel run (HBM counter reference)")
     return ap.parse_args()
 
 
@@ -144,6 +146,16 @@ def main():
         dev.close()
         print(f"# {mode}: {ms:.3f} ms/step, {results[mode]['value']:.3e} GP/s, kernels {tk}",
               file=sys.stderr, flush=True)
+    if args.copy_probe_mb > 0:
+        # one plain copy of known size: the reference for HBM byte counters
+        from alya_sma_amd._lib import check, lib
+        from alya_sma_amd.device import DeviceArray
+        nb = args.copy_probe_mb << 20
+        a, b = DeviceArray(nb // 8, np.float64), DeviceArray(nb // 8, np.float64)
+        check(lib().sma_copy_probe(a.ptr, b.ptr, nb, None), "sma_copy_probe")
+        synchronize()
+        a.free()
+        b.free()
     ref = stress.get("fp64")
     for mode in modes:
         if ref is not None:

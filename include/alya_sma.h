@@ -236,6 +236,9 @@ int sma_device_free(void *p);
 int sma_memcpy(void *dst, const void *src, int64_t bytes, int32_t kind);
 int sma_device_synchronize(void);
 int sma_device_count(int32_t *n);
+/* Streaming device-to-device copy kernel of `bytes` (multiple of 16): the
+ * calibration point for HBM byte counters (FETCH_SIZE / WRITE_SIZE). */
+int sma_copy_probe(const void *src, void *dst, int64_t bytes, void *stream);
 
 /* ------------------------------------------------------------------------
  * Host helpers (no GPU needed).
