@@ -45,10 +45,11 @@ unsigned nblocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 struct GraphKey {
   double dt, phi;
-  int scheme, mode, store, epb, xcd, persistent, tn;
+  int scheme, mode, store, epb, xcd, persistent, assembly, tn;
   bool operator<(const GraphKey& o) const {
-    return std::tie(dt, phi, scheme, mode, store, epb, xcd, persistent, tn) <
-           std::tie(o.dt, o.phi, o.scheme, o.mode, o.store, o.epb, o.xcd, o.persistent, o.tn);
+    return std::tie(dt, phi, scheme, mode, store, epb, xcd, persistent, assembly, tn) <
+           std::tie(o.dt, o.phi, o.scheme, o.mode, o.store, o.epb, o.xcd, o.persistent,
+                    o.assembly, o.tn);
   }
 };
 
@@ -67,6 +68,12 @@ struct sld_dev {
   double* a[3] = {};
   double* fint64[3] = {};
   float* fint32[3] = {};
+  // gather assembly: per-corner element forces (FP64 size, reused for FP32)
+  // and the node -> element-corner CSR in colour-sorted element order
+  void* efor[3] = {};
+  int32_t* node_ptr = nullptr;
+  int32_t* node_adj = nullptr;
+  bool repeated_nodes = false;       // an element lists a node twice: gather only
   double* mass = nullptr;
   double* rmass = nullptr;
   uint8_t* fixmask = nullptr;
@@ -83,12 +90,13 @@ struct sld_dev {
   int32_t* mat_of_elem = nullptr;
   double* E_of_elem = nullptr;
   double* sdv[2][3] = {};
-  int cur = 1;                       // index of the TIME_N history slot
+  int cur = 1;                       // index of the TIME_N history and clock slot
   double* S_out = nullptr;
   uint32_t* flags_out = nullptr;
   unsigned long long* counters = nullptr;
-  double* clock = nullptr;
+  double* clock = nullptr;           // [2] t^n in slot cur
   int mode = SLD_MODE_FP64, store = 0, epb = 32, xcd = 1, persistent = 0, use_graph = 1;
+  int assembly = SLD_ASSEMBLY_GATHER;
   double dtcrit = 0.0;
   std::map<GraphKey, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
   std::vector<void*> allocs;
@@ -116,17 +124,24 @@ int track(sld_dev* d, T* p) {
   return 0;
 }
 
-sld::NodalArgs nodal_args(sld_dev* d) {
+sld::NodalArgs nodal_args(sld_dev* d, int tn) {
   sld::NodalArgs A{};
   A.npoin = d->npoin;
   const bool f32 = d->mode != SLD_MODE_FP64;
+  const bool gather = d->assembly == SLD_ASSEMBLY_GATHER;
   for (int k = 0; k < 3; ++k) {
     A.u[k] = d->u[k];
     A.v[k] = d->v[k];
     A.a[k] = d->a[k];
     A.fint[k] = f32 ? (void*)d->fint32[k] : (void*)d->fint64[k];
     A.fext0[k] = d->fext0[k];
+    A.efor[k] = gather ? d->efor[k] : nullptr;
   }
+  A.tn = tn;
+  A.zero_fint = gather ? 0 : 1;
+  A.node_ptr = gather ? d->node_ptr : nullptr;
+  A.node_adj = gather ? d->node_adj : nullptr;
+  A.sur_count = d->sur ? d->sur->count : nullptr;
   A.fint_fp32 = f32 ? 1 : 0;
   A.rmass = d->rmass;
   A.fixmask = d->fixmask;
@@ -142,7 +157,7 @@ sld::NodalArgs nodal_args(sld_dev* d) {
 
 int g_num_cu = 0;
 
-template <typename R, int EPB, bool STORE, int SUR>
+template <typename R, int EPB, bool STORE, int SUR, bool GATHER>
 void launch_element(sld::ElemArgs A, hipStream_t s, int persistent) {
   const int ne = A.e_end - A.e_begin;
   if (ne <= 0) return;
@@ -157,14 +172,27 @@ void launch_element(sld::ElemArgs A, hipStream_t s, int persistent) {
       grid = (ntiles + A.tiles_per_block - 1) / A.tiles_per_block;
     }
   }
-  hipLaunchKernelGGL((sld::sld_element<R, EPB, STORE, SUR>), dim3(grid), dim3(EPB * 8), 0, s, A);
+  hipLaunchKernelGGL((sld::sld_element<R, EPB, STORE, SUR, GATHER>), dim3(grid), dim3(EPB * 8), 0,
+                     s, A);
 }
 
-template <typename R, bool STORE, int SUR = 0>
+template <typename R, bool STORE, int SUR, bool GATHER>
 void launch_element_epb(int epb, const sld::ElemArgs& A, hipStream_t s, int persistent) {
-  if (epb == 8) launch_element<R, 8, STORE, SUR>(A, s, persistent);
-  else if (epb == 16) launch_element<R, 16, STORE, SUR>(A, s, persistent);
-  else launch_element<R, 32, STORE, SUR>(A, s, persistent);
+  if (epb == 8) launch_element<R, 8, STORE, SUR, GATHER>(A, s, persistent);
+  else if (epb == 16) launch_element<R, 16, STORE, SUR, GATHER>(A, s, persistent);
+  else launch_element<R, 32, STORE, SUR, GATHER>(A, s, persistent);
+}
+
+template <typename R, int SUR>
+void launch_element_mode(const sld_dev* d, const sld::ElemArgs& A, hipStream_t s, bool gather) {
+  const bool st = d->store != 0;
+  if (gather) {
+    if (st) launch_element_epb<R, true, SUR, true>(d->epb, A, s, d->persistent);
+    else launch_element_epb<R, false, SUR, true>(d->epb, A, s, d->persistent);
+  } else {
+    if (st) launch_element_epb<R, true, SUR, false>(d->epb, A, s, d->persistent);
+    else launch_element_epb<R, false, SUR, false>(d->epb, A, s, d->persistent);
+  }
 }
 
 bool surrogate_attached(const sld_dev* d) { return d->sur != nullptr && d->sur->mlp != nullptr; }
@@ -177,11 +205,14 @@ void surrogate_detach(sld_dev* d) {
   d->sur = nullptr;
 }
 
-// Optional per-launch HIP-event timing (eager launches only, never captured).
+// Optional timing (eager launches only, never captured): event pairs
+// recorded around the predictor, the element phase (all its launches), the
+// corrector and the surrogate's extra launches.
 struct KernelTimer {
   std::vector<hipEvent_t> ev;
-  std::vector<int> kind;        // 0 predict, 1 element, 2 correct, 3 other
+  std::vector<int> kind;        // 0 predict, 1 element phase, 2 correct, 3 other
   size_t used = 0;
+  int element_launches = 0;
   hipStream_t s = nullptr;
   void mark(int k) {
     if (used + 2 > ev.size()) {
@@ -211,15 +242,12 @@ struct KernelTimer {
 // One explicit step; tn = history slot holding TIME_N.
 int enqueue_step(sld_dev* d, double dt, int scheme, double phi, int tn, KernelTimer* T = nullptr) {
   hipStream_t s = d->stream;
-  sld::NodalArgs N = nodal_args(d);
+  sld::NodalArgs N = nodal_args(d, tn);
+  const bool gather = d->assembly == SLD_ASSEMBLY_GATHER;
   const unsigned nb = nblocks(d->npoin, 256);
   TBEGIN(0);
   if (scheme == 0) hipLaunchKernelGGL((sld::sld_predict<0>), dim3(nb), dim3(256), 0, s, N, dt, phi);
   else hipLaunchKernelGGL((sld::sld_predict<1>), dim3(nb), dim3(256), 0, s, N, dt, phi);
-  TEND();
-  TBEGIN(3);
-  hipLaunchKernelGGL(sld::sld_clock, dim3(1), dim3(64), 0, s, d->clock, dt,
-                     d->sur ? d->sur->count : (unsigned*)nullptr);
   TEND();
 
   {
@@ -232,6 +260,7 @@ int enqueue_step(sld_dev* d, double dt, int scheme, double phi, int tn, KernelTi
       E.sdv_n[k] = d->sdv[tn][k];
       E.sdv_k[k] = d->sdv[1 - tn][k];
       E.fint[k] = N.fint[k];
+      E.efor[k] = d->efor[k];
     }
     E.props = d->props;
     E.mat_of_elem = d->mat_of_elem;
@@ -254,7 +283,7 @@ int enqueue_step(sld_dev* d, double dt, int scheme, double phi, int tn, KernelTi
       E.e_begin = 0;
       E.e_end = (int32_t)d->nelem;
       TBEGIN(3);
-      launch_element_epb<float, false, 1>(d->epb, E, s, d->persistent);
+      launch_element_epb<float, false, 1, false>(d->epb, E, s, d->persistent);
       TEND();
       TBEGIN(3);
       int rc = sma_mlp_infer_counted(S->mlp, S->feat, S->count, S->cap, S->Y, s);
@@ -278,24 +307,19 @@ int enqueue_step(sld_dev* d, double dt, int scheme, double phi, int tn, KernelTi
       hipLaunchKernelGGL(sld::sld_sur_check, dim3(nblocks(S->cap, 256)), dim3(256), 0, s, C);
       TEND();
     }
-    for (int c = 0; c < d->ncolor; ++c) {
-      E.e_begin = d->color_ptr[c];
-      E.e_end = d->color_ptr[c + 1];
-      const bool st = d->store != 0;
+    // gather: one launch over every element; scatter: one launch per colour
+    const int nlaunch = gather ? 1 : d->ncolor;
+    TBEGIN(1);
+    for (int c = 0; c < nlaunch; ++c) {
+      E.e_begin = gather ? 0 : d->color_ptr[c];
+      E.e_end = gather ? (int32_t)d->nelem : d->color_ptr[c + 1];
       if (E.e_end <= E.e_begin) continue;
-      TBEGIN(1);
-      if (d->mode == SLD_MODE_FP64) {
-        if (st) launch_element_epb<double, true>(d->epb, E, s, d->persistent);
-        else launch_element_epb<double, false>(d->epb, E, s, d->persistent);
-      } else if (sur) {
-        if (st) launch_element_epb<float, true, 2>(d->epb, E, s, d->persistent);
-        else launch_element_epb<float, false, 2>(d->epb, E, s, d->persistent);
-      } else {
-        if (st) launch_element_epb<float, true>(d->epb, E, s, d->persistent);
-        else launch_element_epb<float, false>(d->epb, E, s, d->persistent);
-      }
-      TEND();
+      if (T) T->element_launches++;
+      if (d->mode == SLD_MODE_FP64) launch_element_mode<double, 0>(d, E, s, gather);
+      else if (sur) launch_element_mode<float, 2>(d, E, s, gather);
+      else launch_element_mode<float, 0>(d, E, s, gather);
     }
+    TEND();
   }
   TBEGIN(2);
   if (scheme == 0) hipLaunchKernelGGL((sld::sld_correct<0>), dim3(nb), dim3(256), 0, s, N, dt);
@@ -443,15 +467,20 @@ int sld_dev_create(const sld_mesh_desc* mesh, const sld_material_desc* mat,
   std::vector<int32_t> color((size_t)ne);
   const int32_t ncolor = sld_color_elements(ne, np, mesh->lnods, color.data());
   if (ncolor < 0) return ncolor;
-  std::vector<int64_t> referenced((size_t)np, 0);
-  for (int64_t i = 0; i < 8 * ne; ++i) referenced[mesh->lnods[i]] = 1;
+  std::vector<int32_t> nadj((size_t)np + 1, 0);
+  for (int64_t i = 0; i < 8 * ne; ++i) nadj[mesh->lnods[i] + 1]++;
   for (int64_t p = 0; p < np; ++p)
-    if (!referenced[p]) return fail(SMA_ERR_ARG, "sld_dev_create: node not used by any element (zero mass)");
+    if (!nadj[p + 1]) return fail(SMA_ERR_ARG, "sld_dev_create: node not used by any element (zero mass)");
+  bool repeated = false;
+  for (int64_t e = 0; e < ne && !repeated; ++e)
+    for (int a = 0; a < 8; ++a)
+      for (int b = a + 1; b < 8; ++b) repeated |= mesh->lnods[8 * e + a] == mesh->lnods[8 * e + b];
 
   sld_dev* d = new sld_dev();
   d->npoin = np;
   d->nelem = ne;
   d->ncolor = ncolor;
+  d->repeated_nodes = repeated;
   d->color_ptr.assign(ncolor + 1, 0);
   for (int64_t e = 0; e < ne; ++e) d->color_ptr[color[e] + 1]++;
   for (int c = 0; c < ncolor; ++c) d->color_ptr[c + 1] += d->color_ptr[c];
@@ -480,6 +509,16 @@ int sld_dev_create(const sld_mesh_desc* mesh, const sld_material_desc* mat,
   int rc = 0;
 #define TRY(x) do { rc = (x); if (rc) return bail(rc); } while (0)
   TRY(upload(&d->lnods, ln.data(), ln.size(), s)); track(d, d->lnods);
+  {
+    // node -> element corners (8 * processed element + local node), ascending
+    // in the processed (colour-sorted) order: the gather adds in colour order
+    for (int64_t p = 0; p < np; ++p) nadj[p + 1] += nadj[p];
+    std::vector<int32_t> adj((size_t)ne * 8), fill(nadj.begin(), nadj.end() - 1);
+    for (int64_t i = 0; i < ne; ++i)
+      for (int a = 0; a < 8; ++a) adj[fill[ln[8 * i + a]]++] = (int32_t)(8 * i + a);
+    TRY(upload(&d->node_ptr, nadj.data(), nadj.size(), s)); track(d, d->node_ptr);
+    TRY(upload(&d->node_adj, adj.data(), adj.size(), s)); track(d, d->node_adj);
+  }
   std::vector<std::vector<double>> stage;
   for (int k = 0; k < 3; ++k) {
     stage.emplace_back((size_t)np);
@@ -490,6 +529,7 @@ int sld_dev_create(const sld_mesh_desc* mesh, const sld_material_desc* mat,
     TRY(dalloc(&d->a[k], np)); track(d, d->a[k]);
     TRY(dalloc(&d->fint64[k], np)); track(d, d->fint64[k]);
     TRY(dalloc(&d->fint32[k], np)); track(d, d->fint32[k]);
+    TRY(dalloc((double**)&d->efor[k], (size_t)ne * 8)); track(d, d->efor[k]);
     HIPCHK(hipMemsetAsync(d->u[k], 0, np * sizeof(double), s));
     HIPCHK(hipMemsetAsync(d->v[k], 0, np * sizeof(double), s));
     HIPCHK(hipMemsetAsync(d->a[k], 0, np * sizeof(double), s));
@@ -538,8 +578,8 @@ int sld_dev_create(const sld_mesh_desc* mesh, const sld_material_desc* mat,
     }
   TRY(dalloc(&d->counters, 8)); track(d, d->counters);
   HIPCHK(hipMemsetAsync(d->counters, 0, 8 * sizeof(unsigned long long), s));
-  TRY(dalloc(&d->clock, 1)); track(d, d->clock);
-  HIPCHK(hipMemsetAsync(d->clock, 0, sizeof(double), s));
+  TRY(dalloc(&d->clock, 2)); track(d, d->clock);
+  HIPCHK(hipMemsetAsync(d->clock, 0, 2 * sizeof(double), s));
   TRY(dalloc(&d->mass, np)); track(d, d->mass);
   TRY(dalloc(&d->rmass, np)); track(d, d->rmass);
   HIPCHK(hipMemsetAsync(d->mass, 0, np * sizeof(double), s));
@@ -637,6 +677,20 @@ int sld_dev_set_tile(sld_dev* d, int32_t epb, int32_t xcd, int32_t persistent) {
   return SMA_OK;
 }
 
+int sld_dev_set_assembly(sld_dev* d, int32_t assembly) {
+  if (!d) return fail(SMA_ERR_ARG, "null handle");
+  if (assembly != SLD_ASSEMBLY_SCATTER && assembly != SLD_ASSEMBLY_GATHER)
+    return fail(SMA_ERR_ARG, "assembly must be SLD_ASSEMBLY_SCATTER or SLD_ASSEMBLY_GATHER");
+  if (assembly == SLD_ASSEMBLY_SCATTER && d->repeated_nodes)
+    return fail(SMA_ERR_STATE, "an element lists a node twice: only the gather assembly is exact");
+  if (assembly != d->assembly) {
+    // f_int is complete after every step in both assemblies: nothing to carry
+    HIPCHK(hipStreamSynchronize(d->stream));
+    d->assembly = assembly;
+  }
+  return SMA_OK;
+}
+
 int sld_dev_step(sld_dev* d, double dt, int32_t scheme, double phi, int64_t nsteps) {
   if (!d) return fail(SMA_ERR_ARG, "null handle");
   if (scheme != 0 && scheme != 1) return fail(SMA_ERR_ARG, "scheme must be 0 (CD) or 1 (TW)");
@@ -646,7 +700,7 @@ int sld_dev_step(sld_dev* d, double dt, int32_t scheme, double phi, int64_t nste
     return fail(SMA_ERR_STATE, "surrogate mode without a surrogate");
   int64_t left = nsteps;
   if (d->use_graph && left >= 2) {
-    GraphKey key{dt, phi, scheme, d->mode, d->store, d->epb, d->xcd, d->persistent, d->cur};
+    GraphKey key{dt, phi, scheme, d->mode, d->store, d->epb, d->xcd, d->persistent, d->assembly, d->cur};
     auto it = d->graphs.find(key);
     if (it == d->graphs.end()) {
       hipGraph_t g = nullptr;
@@ -705,9 +759,11 @@ int sld_dev_time_kernels(sld_dev* d, double dt, int32_t scheme, double phi, int3
   hipEventDestroy(t0);
   hipEventDestroy(t1);
   out[0] = cnt[0] ? sum[0] / cnt[0] : 0.0;    // predictor, ms per launch
-  out[1] = cnt[1] ? sum[1] / cnt[1] : 0.0;    // element kernel, ms per launch
+  // element kernel, ms per launch: the element phase (all its launches,
+  // back to back) divided by its number of launches
+  out[1] = T.element_launches ? sum[1] / T.element_launches : 0.0;
   out[2] = cnt[2] ? sum[2] / cnt[2] : 0.0;    // corrector, ms per launch
-  out[3] = (double)cnt[1] / nsteps;          // element launches per step
+  out[3] = (double)T.element_launches / nsteps;  // element launches per step
   out[4] = (sum[0] + sum[1] + sum[2] + sum[3]) / nsteps;  // kernel ms per step
   out[5] = (double)tot / nsteps;             // wall ms per step on the stream (eager)
   return SMA_OK;
@@ -731,7 +787,7 @@ int sld_dev_autotune(sld_dev* d, double dt, int32_t scheme, double phi, int32_t 
     live.push_back({d->a[k], np * 8});
     for (int sl = 0; sl < 2; ++sl) live.push_back({d->sdv[sl][k], ngp * 8});
   }
-  live.push_back({d->clock, 8});
+  live.push_back({d->clock, 16});
   live.push_back({d->counters, 64});
   std::vector<void*> snap(live.size(), nullptr);
   auto cleanup = [&]() { for (void* p : snap) if (p) (void)hipFree(p); };
@@ -750,15 +806,18 @@ int sld_dev_autotune(sld_dev* d, double dt, int32_t scheme, double phi, int32_t 
     return 0;
   };
   const int epbs[3] = {8, 16, 32}, pers[3] = {0, 2, 4};
-  int b_epb = d->epb, b_xcd = d->xcd, b_per = d->persistent;
+  int b_epb = d->epb, b_xcd = d->xcd, b_per = d->persistent, b_asm = d->assembly;
   double b_ms = 1e300;
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
   int rc = 0;
+  const int asm_first = d->repeated_nodes ? SLD_ASSEMBLY_GATHER : SLD_ASSEMBLY_SCATTER;
+  for (int as = asm_first; as <= SLD_ASSEMBLY_GATHER && !rc; ++as)
   for (int ie = 0; ie < 3 && !rc; ++ie)
     for (int x = 0; x < 2 && !rc; ++x)
       for (int ip = 0; ip < 3 && !rc; ++ip) {
+        d->assembly = as;
         d->epb = epbs[ie];
         d->xcd = x;
         d->persistent = pers[ip];
@@ -775,7 +834,13 @@ int sld_dev_autotune(sld_dev* d, double dt, int32_t scheme, double phi, int32_t 
         if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
             hipEventElapsedTime(&ms, e0, e1) != hipSuccess) { rc = fail(SMA_ERR_HIP, "event timing"); break; }
         const double per_step = (double)ms / nsteps;
-        if (per_step < b_ms) { b_ms = per_step; b_epb = d->epb; b_xcd = d->xcd; b_per = d->persistent; }
+        if (per_step < b_ms) {
+          b_ms = per_step;
+          b_epb = d->epb;
+          b_xcd = d->xcd;
+          b_per = d->persistent;
+          b_asm = d->assembly;
+        }
       }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
@@ -785,9 +850,10 @@ int sld_dev_autotune(sld_dev* d, double dt, int32_t scheme, double phi, int32_t 
   d->epb = b_epb;
   d->xcd = b_xcd;
   d->persistent = b_per;
+  d->assembly = b_asm;
   if (rc) return rc;
   if (rr) return rr;
-  if (best) { best[0] = b_epb; best[1] = b_xcd; best[2] = b_per; }
+  if (best) { best[0] = b_epb; best[1] = b_xcd; best[2] = b_per; best[3] = b_asm; }
   if (best_ms) *best_ms = b_ms;
   return SMA_OK;
 }
@@ -879,7 +945,7 @@ int sld_dev_get(sld_dev* d, int32_t field, void* host, int64_t count) {
     }
     case SLD_FIELD_TIME:
       if (need(1)) return SMA_ERR_ARG;
-      HIPCHK(hipMemcpy(host, d->clock, sizeof(double), hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(host, d->clock + d->cur, sizeof(double), hipMemcpyDeviceToHost));
       return SMA_OK;
     case SLD_FIELD_COUNTERS:
       if (need(8)) return SMA_ERR_ARG;
@@ -921,7 +987,7 @@ int sld_dev_set(sld_dev* d, int32_t field, const void* host, int64_t count) {
     }
     case SLD_FIELD_TIME:
       if (count != 1) return fail(SMA_ERR_ARG, "count must be 1");
-      HIPCHK(hipMemcpy(d->clock, host, sizeof(double), hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(d->clock + d->cur, host, sizeof(double), hipMemcpyHostToDevice));
       return SMA_OK;
     default:
       return fail(SMA_ERR_ARG, "field cannot be set");

@@ -1,22 +1,28 @@
 // HIP kernels of the explicit solid-mechanics step (gfx950, wave64).
 //
 // Per step, on one stream (captured into a hipGraph by the host):
-//   sld_predict   nodal predictor, Dirichlet overwrite, zeroes f_int
+//   sld_predict   nodal predictor, Dirichlet overwrite at t^{n+1}
 //                 (mod_sld_solution_methods.f90:122-145 CD, :406-429 TW;
-//                 sld_updbcs.f90:79-114 + sld_funbou.f90:95-136)
-//   sld_clock     t^{n+1} = t^n + dt (setgts.f90:98)
-//   sld_element   one launch per colour: SoA gather into LDS element tiles,
-//                 reference Jacobian + deformation gradient at the 8 Gauss
-//                 points, SMA return map, Kirchhoff stress, internal force,
-//                 deterministic coloured scatter into f_int
+//                 sld_updbcs.f90:79-114 + sld_funbou.f90:95-136); clears
+//                 f_int for the coloured scatter
+//   sld_element   SoA gather into LDS element tiles, reference Jacobian +
+//                 deformation gradient at the 8 Gauss points, SMA return
+//                 map, Kirchhoff stress, internal force
 //                 (sld_element_operations.f90:119-256, elmca2.f90:105-111,
 //                 sld_elmpre.f90:149-162, sld_builtin_materials.f90:316-371,
-//                 sld_elmmat.f90:137-150, mod_matrix.f90:2327-2431)
-//   sld_correct   a = (f_ext - f_int) / m, velocity corrector / TW Dirichlet
-//                 zeroing (solric.f90:294-330; mod_sld_solution_methods.f90:
-//                 181-210 CD, :494-527 TW)
-// The history (xS, F_AS, F_SA) lives in two SoA buffers that swap roles
-// every step (sld_updunk.f90:149-186, svegm TIME_N <- ITER_K).
+//                 sld_elmmat.f90:137-150, mod_matrix.f90:2327-2431).
+//                 Two assemblies, both deterministic and bitwise equal:
+//                 coloured scatter (one launch per colour, read-modify-write
+//                 of f_int) or gather (one launch over all elements writing
+//                 per-corner forces; the corrector sums them per node in
+//                 colour order, i.e. the same additions in the same order)
+//   sld_correct   [gather f_int], a = (f_ext - f_int) / m, velocity
+//                 corrector / TW Dirichlet zeroing (solric.f90:294-330;
+//                 mod_sld_solution_methods.f90:181-210 CD, :494-527 TW);
+//                 writes t^{n+1} = t^n + dt (setgts.f90:98)
+// The history (xS, F_AS, F_SA) and the clock live in two slots that swap
+// roles every step (sld_updunk.f90:149-186, svegm TIME_N <- ITER_K): slot
+// tn holds TIME_N / t^n, the step writes slot 1 - tn.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -102,7 +108,13 @@ struct NodalArgs {
   int32_t func_maxpts;
   const double* fext0[3];      // may be null
   int32_t fext_func;           // -1 = none
-  double* clock;               // device time t^n
+  double* clock;               // [2] device time, slot tn = t^n
+  int32_t tn;
+  int32_t zero_fint;           // coloured scatter: the predictor clears f_int
+  const int32_t* node_ptr;     // gather: CSR [npoin+1] into node_adj, else null
+  const int32_t* node_adj;     // 8 * e + a per element corner at the node, ascending e
+  const void* efor[3];         // gather: per-corner element forces (double or float)
+  unsigned* sur_count;         // surrogate compact-list counter, cleared per step
 };
 
 template <int SCHEME>  // 0 = central difference, 1 = Tchamwa-Wielgosz
@@ -110,7 +122,8 @@ __global__ void __launch_bounds__(256) sld_predict(NodalArgs A, double dt, doubl
 #pragma clang fp contract(off)
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= A.npoin) return;
-  const double cutim = A.clock[0] + dt;
+  if (i == 0 && A.sur_count) A.sur_count[0] = 0u;
+  const double cutim = A.clock[A.tn] + dt;
   const double t_end = cutim + dt, t_mid = 0.5 * (cutim + t_end);
   const int fix = A.fixmask[i];
   const int fn = A.func_of_node ? A.func_of_node[i] : -1;
@@ -133,18 +146,35 @@ __global__ void __launch_bounds__(256) sld_predict(NodalArgs A, double dt, doubl
     }
     A.u[d][i] = un;
     A.v[d][i] = vn;
-    if (A.fint_fp32) ((float*)A.fint[d])[i] = 0.f;
-    else ((double*)A.fint[d])[i] = 0.0;
+    if (A.zero_fint) {
+      if (A.fint_fp32) ((float*)A.fint[d])[i] = 0.f;
+      else ((double*)A.fint[d])[i] = 0.0;
+    }
   }
 }
 
-// Advances the device clock; also clears the surrogate's compact-list
-// counter for the step (a kernel, not a memset, so the step can be captured).
-__global__ void sld_clock(double* clock, double dt, unsigned* sur_count) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    clock[0] = clock[0] + dt;
-    if (sur_count) sur_count[0] = 0u;
+// Gather assembly: f_int at node i = 0 + f_(e1) + f_(e2) + ... over the
+// element corners at the node in ascending (colour-sorted) element order,
+// which is the order in which the coloured scatter adds them.
+template <typename R>
+__device__ __forceinline__ void gather_fint(const NodalArgs& A, int64_t i, double fi[3]) {
+  const int32_t k0 = A.node_ptr[i], k1 = A.node_ptr[i + 1];
+  const R* e0 = (const R*)A.efor[0];
+  const R* e1 = (const R*)A.efor[1];
+  const R* e2 = (const R*)A.efor[2];
+  R s0 = R(0), s1 = R(0), s2 = R(0);
+  for (int32_t k = k0; k < k1; ++k) {
+    const int32_t j = A.node_adj[k];
+    s0 += e0[j];
+    s1 += e1[j];
+    s2 += e2[j];
   }
+  ((R*)A.fint[0])[i] = s0;
+  ((R*)A.fint[1])[i] = s1;
+  ((R*)A.fint[2])[i] = s2;
+  fi[0] = (double)s0;
+  fi[1] = (double)s1;
+  fi[2] = (double)s2;
 }
 
 template <int SCHEME>
@@ -152,7 +182,8 @@ __global__ void __launch_bounds__(256) sld_correct(NodalArgs A, double dt) {
 #pragma clang fp contract(off)
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= A.npoin) return;
-  const double cutim = A.clock[0];  // already advanced by sld_clock
+  const double cutim = A.clock[A.tn] + dt;  // t^{n+1}
+  if (i == 0) A.clock[1 - A.tn] = cutim;
   const double t_end = cutim + dt, t_mid = 0.5 * (cutim + t_end);
   double fx = 0.0;
   if (A.fext_func >= 0)
@@ -160,9 +191,18 @@ __global__ void __launch_bounds__(256) sld_correct(NodalArgs A, double dt) {
                 A.func_npts[A.fext_func], A.func_smooth[A.fext_func], 0, cutim);
   const double rm = A.rmass[i];
   const int fix = A.fixmask[i];
+  double fg[3];
+  if (A.node_ptr) {
+    if (A.fint_fp32) gather_fint<float>(A, i, fg);
+    else gather_fint<double>(A, i, fg);
+  } else {
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      fg[d] = A.fint_fp32 ? (double)((const float*)A.fint[d])[i] : ((const double*)A.fint[d])[i];
+  }
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    double fi = A.fint_fp32 ? (double)((const float*)A.fint[d])[i] : ((const double*)A.fint[d])[i];
+    double fi = fg[d];
     double r = 0.0 - fi;
     if (A.fext_func >= 0) r = r + fx * A.fext0[d][i];
     double acc = 0.0 + r * rm;
@@ -190,7 +230,8 @@ struct ElemArgs {
   const double* E_of_elem;      // [nelem] or null
   const double* sdv_n[3];       // TIME_N history, SoA over GP = 8*e + g
   double* sdv_k[3];             // ITER_K history
-  void* fint[3];                // nodal internal force (double or float)
+  void* fint[3];                // scatter: nodal internal force (double or float)
+  void* efor[3];                // gather: per-corner forces [8 * e + a] (double or float)
   double* S_out;                // optional PK2 [9][ngp]
   uint32_t* flags_out;          // optional [ngp]
   unsigned long long* counters; // [8] per-flag Gauss-point counts
@@ -232,7 +273,9 @@ __device__ __forceinline__ void count_flags(unsigned long long* ctr, uint32_t fl
 // SUR = 0 plain; 1 surrogate prepass (flags + features of transforming
 // points, no stress, no scatter); 2 element step using accepted surrogate
 // answers (exact return map otherwise).
-template <typename R, int EPB, bool STORE, int SUR = 0>
+// GATHER = false: read-modify-write of f_int (the launch covers one colour);
+// true: per-corner forces to efor (the launch may cover every element).
+template <typename R, int EPB, bool STORE, int SUR = 0, bool GATHER = false>
 __global__ void __launch_bounds__(EPB * 8) sld_element(ElemArgs A) {
   __shared__ double s_abs[EPB][8][6];     // gathered absolute x and u
   __shared__ R s_x[EPB][8][3];            // x_a - x_0
@@ -412,13 +455,20 @@ __global__ void __launch_bounds__(EPB * 8) sld_element(ElemArgs A) {
       f1 += q[3] * D0 + q[4] * D1 + q[5] * D2;
       f2 += q[6] * D0 + q[7] * D1 + q[8] * D2;
     }
-    // coloured scatter: no other element of this colour touches this node
-    R* fx = (R*)A.fint[0];
-    R* fy = (R*)A.fint[1];
-    R* fz = (R*)A.fint[2];
-    fx[node] += f0;
-    fy[node] += f1;
-    fz[node] += f2;
+    if constexpr (GATHER) {
+      const int64_t c = (int64_t)e * 8 + a;
+      ((R*)A.efor[0])[c] = f0;
+      ((R*)A.efor[1])[c] = f1;
+      ((R*)A.efor[2])[c] = f2;
+    } else {
+      // coloured scatter: no other element of this colour touches this node
+      R* fx = (R*)A.fint[0];
+      R* fy = (R*)A.fint[1];
+      R* fz = (R*)A.fint[2];
+      fx[node] += f0;
+      fy[node] += f1;
+      fz[node] += f2;
+    }
   }
   }  // tile loop
 }

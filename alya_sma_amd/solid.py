@@ -124,14 +124,22 @@ class Solidz:
         check(lib().sld_dev_set_tile(self.h, int(elems_per_block), 1 if xcd_remap else 0,
                                      int(persistent)), "sld_dev_set_tile")
 
+    def set_assembly(self, assembly):
+        """"scatter" (one element launch per colour) or "gather" (one launch,
+        per-node sum in the corrector); both give bitwise equal results."""
+        a = {"scatter": 0, "gather": 1}.get(assembly, assembly)
+        check(lib().sld_dev_set_assembly(self.h, int(a)), "sld_dev_set_assembly")
+
     def autotune(self, dt, nsteps=3, scheme=0, phi=1.033):
-        """Pick the fastest element-kernel launch shape; state is unchanged."""
-        best = np.zeros(3, dtype=np.int32)
+        """Pick the fastest assembly and element-kernel launch shape; state
+        is unchanged."""
+        best = np.zeros(4, dtype=np.int32)
         ms = C.c_double()
         check(lib().sld_dev_autotune(self.h, float(dt), int(scheme), float(phi), int(nsteps),
                                      _p(best), C.byref(ms)), "sld_dev_autotune")
         return {"elems_per_block": int(best[0]), "xcd_remap": int(best[1]),
-                "persistent": int(best[2]), "ms_per_step": ms.value}
+                "persistent": int(best[2]), "assembly": ("scatter", "gather")[int(best[3])],
+                "ms_per_step": ms.value}
 
     def step(self, dt, nsteps=1, scheme=0, phi=1.033):
         check(lib().sld_dev_step(self.h, float(dt), int(scheme), float(phi), int(nsteps)),
@@ -144,7 +152,8 @@ class Solidz:
         return lib().sld_dev_stream(self.h)
 
     def time_kernels(self, dt, nsteps=4, scheme=0, phi=1.033):
-        """HIP-event timing of every launch over nsteps eager steps."""
+        """HIP-event timing of the predictor, element phase and corrector
+        over nsteps eager steps."""
         out = np.zeros(6)
         check(lib().sld_dev_time_kernels(self.h, float(dt), int(scheme), float(phi), int(nsteps),
                                          _p(out)), "sld_dev_time_kernels")

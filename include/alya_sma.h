@@ -127,11 +127,13 @@ int sld_dev_set_store(sld_dev *dev, int32_t store);
  * Asynchronous on the handle's stream; steps are replayed from a hipGraph. */
 int sld_dev_step(sld_dev *dev, double dt, int32_t scheme, double phi, int64_t nsteps);
 
-/* Runs nsteps steps eagerly with HIP events around every launch on the
- * handle's stream (the reference brackets the same phases with its
- * times(3..7) timers, sld_element_operations.f90:125-256).  out[6]:
- * ms per launch of the predictor, the element kernel and the corrector,
- * element launches per step, summed kernel ms per step, stream ms per step. */
+/* Runs nsteps steps eagerly with HIP events around the predictor, the
+ * element phase and the corrector on the handle's stream (the reference
+ * brackets the same phases with its times(3..7) timers,
+ * sld_element_operations.f90:125-256).  out[6]: ms per launch of the
+ * predictor, the element kernel (element phase / its launches) and the
+ * corrector, element launches per step, summed bracketed ms per step,
+ * stream ms per step. */
 int sld_dev_time_kernels(sld_dev *dev, double dt, int32_t scheme, double phi,
                          int32_t nsteps, double *out);
 
@@ -167,10 +169,22 @@ int sld_dev_coloring(const sld_dev *dev, int32_t *ncolor, int32_t *color_ptr,
  * these by timing. */
 int sld_dev_set_tile(sld_dev *dev, int32_t elems_per_block, int32_t xcd_remap, int32_t persistent);
 
-/* Autotuner: times nsteps eager steps for every launch shape (elements per
- * workgroup 8/16/32 x XCD mapping on/off x persistent 0/2/4 workgroups per
- * CU) from the same saved state, restores the state, keeps the fastest.
- * best[3] = (elems_per_block, xcd_remap, persistent); best_ms = ms/step. */
+/* Assembly of the internal force (both deterministic, bitwise equal):
+ * SLD_ASSEMBLY_SCATTER = one element launch per colour adding into f_int
+ * (the coloured element loop of sld_elmope_all, sld_element_operations.f90
+ * with mod_matrix.f90:2327-2431 assembly); SLD_ASSEMBLY_GATHER (default)
+ * = one element launch over all elements writing per-corner forces, summed
+ * per node in colour order by the corrector.  Meshes with an element that
+ * lists a node twice only accept GATHER. */
+#define SLD_ASSEMBLY_SCATTER 0
+#define SLD_ASSEMBLY_GATHER  1
+int sld_dev_set_assembly(sld_dev *dev, int32_t assembly);
+
+/* Autotuner: times nsteps eager steps for every launch shape (assembly
+ * scatter/gather x elements per workgroup 8/16/32 x XCD mapping on/off x
+ * persistent 0/2/4 workgroups per CU) from the same saved state, restores
+ * the state, keeps the fastest.  best[4] = (elems_per_block, xcd_remap,
+ * persistent, assembly); best_ms = ms/step. */
 int sld_dev_autotune(sld_dev *dev, double dt, int32_t scheme, double phi, int32_t nsteps,
                      int32_t *best, double *best_ms);
 

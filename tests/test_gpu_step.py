@@ -150,19 +150,69 @@ def test_odd_step_counts_and_eager_path_agree():
         assert np.array_equal(a.get("sdv"), b.get("sdv"))
 
 
+@pytest.mark.parametrize("assembly", ["gather", "scatter"])
 @pytest.mark.parametrize("epb,persistent", [(8, 0), (16, 0), (32, 0), (8, 1), (32, 2)])
-def test_tiles_agree(epb, persistent):
+def test_tiles_agree(epb, persistent, assembly):
     # 40x8x8 = 2560 elements: persistent launches with 1-2 workgroups per CU
     # still walk several tiles per workgroup on 256 CUs only for epb 8
     prob = problems.cantilever_problem(nx=40, ny=8, nz=8, lx=10.0, ly=2.0, lz=2.0, nsteps=100,
                                        traction=700.0)
     dt = prob["run"]["dt"]
     with Solidz(prob) as a, Solidz(prob) as b:
+        b.set_assembly(assembly)
         b.set_tile(epb, xcd_remap=(epb != 16), persistent=persistent)
         a.step(dt, nsteps=100)
         b.step(dt, nsteps=100)
         assert np.array_equal(a.get("u"), b.get("u"))
         assert np.array_equal(a.get("sdv"), b.get("sdv"))
+
+
+@pytest.mark.parametrize("mode,scheme", [(0, 0), (0, 1), (1, 0), (1, 1)])
+def test_gather_and_scatter_assembly_bitwise_equal(mode, scheme):
+    """The gather sums each node's element corners in colour order, the
+    coloured scatter adds them colour by colour: the same additions."""
+    prob = problems.cantilever_problem(nx=24, ny=4, nz=4, lx=6.0, ly=1.0, lz=1.0, nsteps=300,
+                                       traction=700.0)
+    dt = prob["run"]["dt"]
+    print(f"assembly gather vs scatter, mode {mode} scheme {scheme}", flush=True)
+    with Solidz(prob) as g, Solidz(prob) as s:
+        g.set_assembly("gather")
+        s.set_assembly("scatter")
+        for d in (g, s):
+            d.set_mode(mode)
+            d.step(dt, nsteps=150, scheme=scheme, phi=1.033)
+        for f in ("u", "v", "a", "fint", "sdv", "time"):
+            assert np.array_equal(g.get(f), s.get(f)), f
+        # switching assembly mid-run keeps the trajectory
+        g.set_assembly("scatter")
+        s.set_assembly("gather")
+        for d in (g, s):
+            d.step(dt, nsteps=150, scheme=scheme, phi=1.033)
+        for f in ("u", "v", "fint", "sdv"):
+            assert np.array_equal(g.get(f), s.get(f)), f
+        assert np.any(s.get("sdv")[..., 0] > 0)
+
+
+def test_element_with_repeated_node_gather_matches_oracle():
+    """A hex whose corner 3 sits on corner 2 (a collapsed edge): two lanes
+    of one element add to the same node, which only the gather assembly
+    orders; the scatter assembly is refused for such meshes."""
+    prob = problems.cantilever_problem(nx=8, ny=2, nz=2, lx=4.0, ly=1.0, lz=1.0, nsteps=200,
+                                       traction=500.0)
+    ln = prob["lnods"].copy()
+    ln[5, 3] = ln[5, 2]
+    prob["lnods"] = ln
+    _, gpvol, _, _ = O.hex8_kinematics(prob["coord"][ln[5]], np.zeros((8, 3)))
+    assert np.all(gpvol > 0)
+    dt = prob["run"]["dt"]
+    from alya_sma_amd._lib import HipError
+    dev, orc = _pair(prob)
+    with dev:
+        with pytest.raises(HipError, match="node twice"):
+            dev.set_assembly("scatter")
+        dev.step(dt, nsteps=200)
+        orc.step(dt, n=200)
+        _compare(dev, orc, "repeated node 200")
 
 
 def test_autotune_leaves_state_unchanged():
@@ -175,6 +225,7 @@ def test_autotune_leaves_state_unchanged():
         best = b.autotune(dt, nsteps=2)
         print("autotune:", best, flush=True)
         assert best["elems_per_block"] in (8, 16, 32) and best["ms_per_step"] > 0
+        assert best["assembly"] in ("gather", "scatter")
         a.step(dt, nsteps=50)
         b.step(dt, nsteps=50)
         assert np.array_equal(a.get("u"), b.get("u"))
