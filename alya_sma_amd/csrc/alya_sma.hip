@@ -68,11 +68,11 @@ struct sld_dev {
   double* a[3] = {};
   double* fint64[3] = {};
   float* fint32[3] = {};
-  // gather assembly: per-corner element forces (FP64 size, reused for FP32)
-  // and the node -> element-corner CSR in colour-sorted element order
-  void* efor[3] = {};
+  // gather assembly: corner forces in node-sorted slots (FP64 size, reused
+  // for FP32), slot ranges per node, slot of every element corner
+  void* efor = nullptr;
   int32_t* node_ptr = nullptr;
-  int32_t* node_adj = nullptr;
+  int32_t* corner_slot = nullptr;
   bool repeated_nodes = false;       // an element lists a node twice: gather only
   double* mass = nullptr;
   double* rmass = nullptr;
@@ -135,12 +135,11 @@ sld::NodalArgs nodal_args(sld_dev* d, int tn) {
     A.a[k] = d->a[k];
     A.fint[k] = f32 ? (void*)d->fint32[k] : (void*)d->fint64[k];
     A.fext0[k] = d->fext0[k];
-    A.efor[k] = gather ? d->efor[k] : nullptr;
   }
+  A.efor = gather ? d->efor : nullptr;
   A.tn = tn;
   A.zero_fint = gather ? 0 : 1;
   A.node_ptr = gather ? d->node_ptr : nullptr;
-  A.node_adj = gather ? d->node_adj : nullptr;
   A.sur_count = d->sur ? d->sur->count : nullptr;
   A.fint_fp32 = f32 ? 1 : 0;
   A.rmass = d->rmass;
@@ -260,8 +259,9 @@ int enqueue_step(sld_dev* d, double dt, int scheme, double phi, int tn, KernelTi
       E.sdv_n[k] = d->sdv[tn][k];
       E.sdv_k[k] = d->sdv[1 - tn][k];
       E.fint[k] = N.fint[k];
-      E.efor[k] = d->efor[k];
     }
+    E.efor = d->efor;
+    E.corner_slot = d->corner_slot;
     E.props = d->props;
     E.mat_of_elem = d->mat_of_elem;
     E.E_of_elem = d->E_of_elem;
@@ -510,14 +510,14 @@ int sld_dev_create(const sld_mesh_desc* mesh, const sld_material_desc* mat,
 #define TRY(x) do { rc = (x); if (rc) return bail(rc); } while (0)
   TRY(upload(&d->lnods, ln.data(), ln.size(), s)); track(d, d->lnods);
   {
-    // node -> element corners (8 * processed element + local node), ascending
-    // in the processed (colour-sorted) order: the gather adds in colour order
+    // slots: the corners at each node, ascending in the processed (colour-
+    // sorted) element order, so the gather adds in colour order
     for (int64_t p = 0; p < np; ++p) nadj[p + 1] += nadj[p];
-    std::vector<int32_t> adj((size_t)ne * 8), fill(nadj.begin(), nadj.end() - 1);
+    std::vector<int32_t> slot((size_t)ne * 8), fill(nadj.begin(), nadj.end() - 1);
     for (int64_t i = 0; i < ne; ++i)
-      for (int a = 0; a < 8; ++a) adj[fill[ln[8 * i + a]]++] = (int32_t)(8 * i + a);
+      for (int a = 0; a < 8; ++a) slot[8 * i + a] = fill[ln[8 * i + a]]++;
     TRY(upload(&d->node_ptr, nadj.data(), nadj.size(), s)); track(d, d->node_ptr);
-    TRY(upload(&d->node_adj, adj.data(), adj.size(), s)); track(d, d->node_adj);
+    TRY(upload(&d->corner_slot, slot.data(), slot.size(), s)); track(d, d->corner_slot);
   }
   std::vector<std::vector<double>> stage;
   for (int k = 0; k < 3; ++k) {
@@ -529,7 +529,6 @@ int sld_dev_create(const sld_mesh_desc* mesh, const sld_material_desc* mat,
     TRY(dalloc(&d->a[k], np)); track(d, d->a[k]);
     TRY(dalloc(&d->fint64[k], np)); track(d, d->fint64[k]);
     TRY(dalloc(&d->fint32[k], np)); track(d, d->fint32[k]);
-    TRY(dalloc((double**)&d->efor[k], (size_t)ne * 8)); track(d, d->efor[k]);
     HIPCHK(hipMemsetAsync(d->u[k], 0, np * sizeof(double), s));
     HIPCHK(hipMemsetAsync(d->v[k], 0, np * sizeof(double), s));
     HIPCHK(hipMemsetAsync(d->a[k], 0, np * sizeof(double), s));
@@ -571,6 +570,7 @@ int sld_dev_create(const sld_mesh_desc* mesh, const sld_material_desc* mat,
     TRY(upload(&d->E_of_elem, Ee.data(), Ee.size(), s)); track(d, d->E_of_elem);
   }
   const int64_t ngp = ne * 8;
+  TRY(dalloc((double**)&d->efor, (size_t)ngp * 3)); track(d, d->efor);
   for (int sl = 0; sl < 2; ++sl)
     for (int k = 0; k < 3; ++k) {
       TRY(dalloc(&d->sdv[sl][k], ngp)); track(d, d->sdv[sl][k]);

@@ -14,8 +14,9 @@
 //                 Two assemblies, both deterministic and bitwise equal:
 //                 coloured scatter (one launch per colour, read-modify-write
 //                 of f_int) or gather (one launch over all elements writing
-//                 per-corner forces; the corrector sums them per node in
-//                 colour order, i.e. the same additions in the same order)
+//                 each corner force into its node-sorted slot; the
+//                 corrector sums each node's contiguous slots in colour
+//                 order, i.e. the same additions in the same order)
 //   sld_correct   [gather f_int], a = (f_ext - f_int) / m, velocity
 //                 corrector / TW Dirichlet zeroing (solric.f90:294-330;
 //                 mod_sld_solution_methods.f90:181-210 CD, :494-527 TW);
@@ -111,9 +112,9 @@ struct NodalArgs {
   double* clock;               // [2] device time, slot tn = t^n
   int32_t tn;
   int32_t zero_fint;           // coloured scatter: the predictor clears f_int
-  const int32_t* node_ptr;     // gather: CSR [npoin+1] into node_adj, else null
-  const int32_t* node_adj;     // 8 * e + a per element corner at the node, ascending e
-  const void* efor[3];         // gather: per-corner element forces (double or float)
+  const int32_t* node_ptr;     // gather: [npoin+1] slot ranges per node, else null
+  const void* efor;            // gather: corner forces [slot][3] (double or float),
+                               // slots of a node in ascending element order
   unsigned* sur_count;         // surrogate compact-list counter, cleared per step
 };
 
@@ -159,15 +160,12 @@ __global__ void __launch_bounds__(256) sld_predict(NodalArgs A, double dt, doubl
 template <typename R>
 __device__ __forceinline__ void gather_fint(const NodalArgs& A, int64_t i, double fi[3]) {
   const int32_t k0 = A.node_ptr[i], k1 = A.node_ptr[i + 1];
-  const R* e0 = (const R*)A.efor[0];
-  const R* e1 = (const R*)A.efor[1];
-  const R* e2 = (const R*)A.efor[2];
+  const R* ef = (const R*)A.efor;
   R s0 = R(0), s1 = R(0), s2 = R(0);
   for (int32_t k = k0; k < k1; ++k) {
-    const int32_t j = A.node_adj[k];
-    s0 += e0[j];
-    s1 += e1[j];
-    s2 += e2[j];
+    s0 += ef[3 * (int64_t)k];
+    s1 += ef[3 * (int64_t)k + 1];
+    s2 += ef[3 * (int64_t)k + 2];
   }
   ((R*)A.fint[0])[i] = s0;
   ((R*)A.fint[1])[i] = s1;
@@ -231,7 +229,8 @@ struct ElemArgs {
   const double* sdv_n[3];       // TIME_N history, SoA over GP = 8*e + g
   double* sdv_k[3];             // ITER_K history
   void* fint[3];                // scatter: nodal internal force (double or float)
-  void* efor[3];                // gather: per-corner forces [8 * e + a] (double or float)
+  void* efor;                   // gather: corner forces [slot][3] (double or float)
+  const int32_t* corner_slot;   // gather: [8 * e + a] -> slot of the corner at its node
   double* S_out;                // optional PK2 [9][ngp]
   uint32_t* flags_out;          // optional [ngp]
   unsigned long long* counters; // [8] per-flag Gauss-point counts
@@ -275,13 +274,30 @@ __device__ __forceinline__ void count_flags(unsigned long long* ctr, uint32_t fl
 // answers (exact return map otherwise).
 // GATHER = false: read-modify-write of f_int (the launch covers one colour);
 // true: per-corner forces to efor (the launch may cover every element).
+#ifndef SLD_WAVES_F64
+#define SLD_WAVES_F64 1
+#endif
+#ifndef SLD_WAVES_F32
+#define SLD_WAVES_F32 1
+#endif
+template <typename R>
+struct ElemOccupancy { static constexpr int waves = sizeof(R) == 8 ? SLD_WAVES_F64 : SLD_WAVES_F32; };
+
+// The value of x, opaque to the optimiser: quantities derived from the lane
+// index (shape-function derivatives at the lane's Gauss point or node) are
+// then recomputed where they are used instead of being hoisted out of the
+// tile loop and held in registers through the constitutive update.
+__device__ __forceinline__ int opaque(int x) {
+  asm volatile("" : "+v"(x));
+  return x;
+}
+
 template <typename R, int EPB, bool STORE, int SUR = 0, bool GATHER = false>
-__global__ void __launch_bounds__(EPB * 8) sld_element(ElemArgs A) {
+__global__ void __launch_bounds__(EPB * 8, ElemOccupancy<R>::waves) sld_element(ElemArgs A) {
   __shared__ double s_abs[EPB][8][6];     // gathered absolute x and u
   __shared__ R s_x[EPB][8][3];            // x_a - x_0
   __shared__ R s_u[EPB][8][3];            // u_a - u_0
   __shared__ R s_q[EPB][8][9];            // Q = detJ tau j^-T per Gauss point
-  __shared__ int s_node[EPB][8];
 
   const int tid = threadIdx.x;
   const int le = tid >> 3, g = tid & 7;
@@ -313,7 +329,6 @@ __global__ void __launch_bounds__(EPB * 8) sld_element(ElemArgs A) {
     s_abs[le][g][4] = A.u[1][node];
     s_abs[le][g][5] = A.u[2][node];
   }
-  s_node[le][g] = node;
   __syncthreads();
   if (valid) {
 #pragma unroll
@@ -327,10 +342,11 @@ __global__ void __launch_bounds__(EPB * 8) sld_element(ElemArgs A) {
   // -- phase 2: Gauss point g of element e
   uint32_t fl = 0;
   if (valid) {
+    const int gk = opaque(g);
     R J[9] = {0}, Gu[9] = {0};  // col-major: J(i,k) = J[i + 3k]
 #pragma unroll
     for (int a = 0; a < 8; ++a) {
-      R D0 = hex_deriv<R>(a, g, 0), D1 = hex_deriv<R>(a, g, 1), D2 = hex_deriv<R>(a, g, 2);
+      R D0 = hex_deriv<R>(a, gk, 0), D1 = hex_deriv<R>(a, gk, 1), D2 = hex_deriv<R>(a, gk, 2);
 #pragma unroll
       for (int i = 0; i < 3; ++i) {
         R x = s_x[le][a][i], uu = s_u[le][a][i];
@@ -347,6 +363,16 @@ __global__ void __launch_bounds__(EPB * 8) sld_element(ElemArgs A) {
 #pragma unroll
       for (int j = 0; j < 3; ++j)
         H[i + 3 * j] = Gu[i] * Ji[3 * j] + Gu[i + 3] * Ji[1 + 3 * j] + Gu[i + 6] * Ji[2 + 3 * j];
+    // current Jacobian j = J + Gu = F J and its inverse, formed before the
+    // constitutive update so that J and Gu are dead through it
+    R jci[9];
+    R detj;
+    {
+      R jc[9];
+#pragma unroll
+      for (int q = 0; q < 9; ++q) jc[q] = J[q] + Gu[q];
+      detj = sma::inv3<R>(jc, jci);
+    }
 
     const int mat = A.mat_of_elem ? A.mat_of_elem[e] : 0;
     const double* pr = A.props + 18 * mat;
@@ -372,6 +398,15 @@ __global__ void __launch_bounds__(EPB * 8) sld_element(ElemArgs A) {
       float Hf[9];
 #pragma unroll
       for (int q = 0; q < 9; ++q) Hf[q] = H[q];
+      // plain FP32: Hencky strain tensor by series; the spectral form serves
+      // stretches beyond the series' range and the surrogate modes, whose
+      // features are principal strains
+      sma::Hencky32 hk{};
+      if constexpr (SUR == 0) hk = sma::hencky_fp32(Hf);
+      if (SUR == 0 && hk.ok) {
+        rm = sma::return_map(m, hk.theta, hk.e_n, xS0, FAS0, FSA0);
+        sma::kirchhoff_dev(m, hk, rm.xS, tau);
+      } else {
       sma::Spectral<float> sp = sma::spectral_fp32(Hf);
       if constexpr (SUR == 1) {
         bool kf_AS, kf_SA;
@@ -404,6 +439,7 @@ __global__ void __launch_bounds__(EPB * 8) sld_element(ElemArgs A) {
       }
       if constexpr (SUR != 1) sma::kirchhoff<float>(m, sp, rm.xS, tau);
       if (!sp.eig_ok) rm.flags |= sma::FLAG_EIGEN_FAIL;
+      }
 #pragma unroll
       for (int q = 0; q < 9; ++q) Fm[q] = H[q] + ((q % 4 == 0) ? R(1) : R(0));
     }
@@ -413,11 +449,7 @@ __global__ void __launch_bounds__(EPB * 8) sld_element(ElemArgs A) {
     A.sdv_k[2][gp] = rm.F_SA;
     fl = rm.flags;
 
-    // current Jacobian j = J + Gu = F J; Q = detJ * tau * j^-T
-    R jc[9], jci[9];
-#pragma unroll
-    for (int q = 0; q < 9; ++q) jc[q] = J[q] + Gu[q];
-    const R detj = sma::inv3<R>(jc, jci);
+    // Q = detJ * tau * j^-T
     if (!(detj / detJ > R(0))) fl |= sma::FLAG_DETF_NONPOS;
     const R T[9] = {tau[0], tau[1], tau[2], tau[1], tau[3], tau[4], tau[2], tau[4], tau[5]};
 #pragma unroll
@@ -444,7 +476,7 @@ __global__ void __launch_bounds__(EPB * 8) sld_element(ElemArgs A) {
 
   // -- phase 3: thread g now owns node a = g: f_ai = sum_g' sum_k Q_ik(g') D_ak(g')
   if (valid) {
-    const int a = g;
+    const int a = opaque(g);
     using Acc = typename ElemTraits<R>::Acc;
     Acc f0 = 0, f1 = 0, f2 = 0;
 #pragma unroll
@@ -456,10 +488,10 @@ __global__ void __launch_bounds__(EPB * 8) sld_element(ElemArgs A) {
       f2 += q[6] * D0 + q[7] * D1 + q[8] * D2;
     }
     if constexpr (GATHER) {
-      const int64_t c = (int64_t)e * 8 + a;
-      ((R*)A.efor[0])[c] = f0;
-      ((R*)A.efor[1])[c] = f1;
-      ((R*)A.efor[2])[c] = f2;
+      R* ef = (R*)A.efor + 3 * (int64_t)A.corner_slot[(int64_t)e * 8 + a];
+      ef[0] = f0;
+      ef[1] = f1;
+      ef[2] = f2;
     } else {
       // coloured scatter: no other element of this colour touches this node
       R* fx = (R*)A.fint[0];

@@ -282,6 +282,109 @@ __host__ __device__ __forceinline__ Spectral<float> spectral_fp32(const float (&
   return s;
 }
 
+// FP32 kinematics without eigenvectors.  The stress of the law needs only
+// theta = tr E, the deviatoric Hencky strain e' = dev E and |e'|, because
+// tau = c_vol I + sum_i 2G (e_i - eL xS e_i/|e'|) n_i n_i^T
+//     = c_vol I + 2G (1 - eL xS/|e'|) e'          (compute_kirchoff_stress
+// :293-332 with d_i = e_i/|e'|).  E = 1/2 log b is formed as a tensor:
+// 1/2 log b = artanh(Z) = sum_k Z^(2k+1)/(2k+1), Z = (b - I)(b + I)^-1, whose
+// terms all commute with b; with |Z|_F <= 0.6 the terms fall by >= 0.36 each.
+struct Hencky32 {
+  double theta, e_n;   // FP64 for the return map
+  float ed[6];         // e' (00, 01, 02, 11, 12, 22)
+  bool ok;             // false: |Z|_F > 0.6, use the spectral path
+};
+
+// c = a b for symmetric a, b that commute (the product is symmetric).
+__host__ __device__ __forceinline__ void sym_mul_commuting(const float (&a)[6], const float (&b)[6],
+                                                          float (&c)[6]) {
+  c[0] = a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+  c[1] = a[0] * b[1] + a[1] * b[3] + a[2] * b[4];
+  c[2] = a[0] * b[2] + a[1] * b[4] + a[2] * b[5];
+  c[3] = a[1] * b[1] + a[3] * b[3] + a[4] * b[4];
+  c[4] = a[1] * b[2] + a[3] * b[4] + a[4] * b[5];
+  c[5] = a[2] * b[2] + a[4] * b[4] + a[5] * b[5];
+}
+
+__host__ __device__ __forceinline__ Hencky32 hencky_fp32(const float (&H)[9]) {
+  Hencky32 h;
+  // C = b - I from H = F - I without cancellation
+  float c[6];
+  c[0] = 2.f * H[0] + (H[0] * H[0] + H[3] * H[3] + H[6] * H[6]);
+  c[3] = 2.f * H[4] + (H[1] * H[1] + H[4] * H[4] + H[7] * H[7]);
+  c[5] = 2.f * H[8] + (H[2] * H[2] + H[5] * H[5] + H[8] * H[8]);
+  c[1] = H[3] + H[1] + (H[0] * H[1] + H[3] * H[4] + H[6] * H[7]);
+  c[2] = H[6] + H[2] + (H[0] * H[2] + H[3] * H[5] + H[6] * H[8]);
+  c[4] = H[7] + H[5] + (H[1] * H[2] + H[4] * H[5] + H[7] * H[8]);
+  // (b + I)^-1 = (2I + C)^-1 by cofactors (det(b + I) >= 1)
+  const float m00 = 2.f + c[0], m11 = 2.f + c[3], m22 = 2.f + c[5];
+  float a[6];
+  a[0] = m11 * m22 - c[4] * c[4];
+  a[1] = c[2] * c[4] - c[1] * m22;
+  a[2] = c[1] * c[4] - c[2] * m11;
+  a[3] = m00 * m22 - c[2] * c[2];
+  a[4] = c[1] * c[2] - m00 * c[4];
+  a[5] = m00 * m11 - c[1] * c[1];
+  const float r = 1.0f / (m00 * a[0] + c[1] * a[1] + c[2] * a[2]);
+#pragma unroll
+  for (int q = 0; q < 6; ++q) a[q] *= r;
+  float z[6];
+  sym_mul_commuting(c, a, z);
+  const float zf2 = z[0] * z[0] + z[3] * z[3] + z[5] * z[5] +
+                    2.f * (z[1] * z[1] + z[2] * z[2] + z[4] * z[4]);
+  h.ok = zf2 <= 0.36f;
+  float z2[6], t[6], e[6];
+  sym_mul_commuting(z, z, z2);
+#pragma unroll
+  for (int q = 0; q < 6; ++q) t[q] = e[q] = z[q];
+  if (h.ok) {
+    // |Z^(2k+1)|/(2k+1) below 2^-26 of |E|: at most 18 terms at |Z|_F = 0.6
+    for (int k = 1; k < 24; ++k) {
+      float tn[6];
+      sym_mul_commuting(t, z2, tn);
+      const float w = 1.0f / (float)(2 * k + 1);
+      float tmax = 0.f, emax = 0.f;
+#pragma unroll
+      for (int q = 0; q < 6; ++q) {
+        t[q] = tn[q];
+        e[q] += w * tn[q];
+        tmax = fmaxf(tmax, fabsf(tn[q]));
+        emax = fmaxf(emax, fabsf(e[q]));
+      }
+      if (w * tmax <= 1.4901161e-08f * emax) break;
+    }
+  }
+  const float th = e[0] + e[3] + e[5];
+  const float t3 = th * (1.0f / 3.0f);
+  h.ed[0] = e[0] - t3;
+  h.ed[1] = e[1];
+  h.ed[2] = e[2];
+  h.ed[3] = e[3] - t3;
+  h.ed[4] = e[4];
+  h.ed[5] = e[5] - t3;
+  const float en = sqrtf(h.ed[0] * h.ed[0] + h.ed[3] * h.ed[3] + h.ed[5] * h.ed[5] +
+                         2.f * (h.ed[1] * h.ed[1] + h.ed[2] * h.ed[2] + h.ed[4] * h.ed[4]));
+  h.theta = (double)th;
+  h.e_n = (double)en;
+  return h;
+}
+
+// tau = c_vol I + 2G (1 - eL xS/|e'|) e'; for |e'| below FP32 epsilon the
+// direction term is dropped, as d_i = 0 in the spectral form.
+__host__ __device__ __forceinline__ void kirchhoff_dev(const Mat& m, const Hencky32& h, double xS,
+                                                      float (&t)[6]) {
+  const float cvol = (float)(m.K * (h.theta - 3.0 * m.alfa * m.eL * xS));
+  const float twoG = (float)(2.0 * m.G);
+  const float en = (float)h.e_n;
+  const float coef = en > 1.1920929e-07f ? twoG * (1.0f - (float)(m.eL * xS) / en) : twoG;
+  t[0] = cvol + coef * h.ed[0];
+  t[1] = coef * h.ed[1];
+  t[2] = coef * h.ed[2];
+  t[3] = cvol + coef * h.ed[3];
+  t[4] = coef * h.ed[4];
+  t[5] = cvol + coef * h.ed[5];
+}
+
 // eval_loading_function :513-537 (beta = 1 per :554, temperature inert).
 __host__ __device__ __forceinline__ double loading(const Mat& m, double C, double theta,
                                           double enor, double xS) {
