@@ -58,8 +58,10 @@ struct sld_dev {
   hipStream_t stream = nullptr;
   int64_t npoin = 0, nelem = 0;
   int32_t ncolor = 0;
-  std::vector<int32_t> color_ptr;   // [ncolor+1]
-  std::vector<int32_t> order;       // processed position -> caller element
+  std::vector<int32_t> color_ptr;   // [ncolor+1] ranges of color_list
+  std::vector<int32_t> color_list;  // stored elements sorted by colour (stable)
+  std::vector<int32_t> order;       // stored position -> caller element
+  int32_t* color_list_dev = nullptr;
   // device arrays
   int32_t* lnods = nullptr;
   double* coord[3] = {};
@@ -282,6 +284,7 @@ int enqueue_step(sld_dev* d, double dt, int scheme, double phi, int tn, KernelTi
       E.T_feat = S->T;
       E.e_begin = 0;
       E.e_end = (int32_t)d->nelem;
+      E.elem_list = nullptr;
       TBEGIN(3);
       launch_element_epb<float, false, 1, false>(d->epb, E, s, d->persistent);
       TEND();
@@ -313,6 +316,7 @@ int enqueue_step(sld_dev* d, double dt, int scheme, double phi, int tn, KernelTi
     for (int c = 0; c < nlaunch; ++c) {
       E.e_begin = gather ? 0 : d->color_ptr[c];
       E.e_end = gather ? (int32_t)d->nelem : d->color_ptr[c + 1];
+      E.elem_list = gather ? nullptr : d->color_list_dev;
       if (E.e_end <= E.e_begin) continue;
       if (T) T->element_launches++;
       if (d->mode == SLD_MODE_FP64) launch_element_mode<double, 0>(d, E, s, gather);
@@ -484,10 +488,14 @@ int sld_dev_create(const sld_mesh_desc* mesh, const sld_material_desc* mat,
   d->color_ptr.assign(ncolor + 1, 0);
   for (int64_t e = 0; e < ne; ++e) d->color_ptr[color[e] + 1]++;
   for (int c = 0; c < ncolor; ++c) d->color_ptr[c + 1] += d->color_ptr[c];
+  // elements are stored in the caller's order (neighbours stay close for
+  // the gather launch); the coloured scatter walks them through color_list
   d->order.resize(ne);
+  for (int64_t e = 0; e < ne; ++e) d->order[e] = (int32_t)e;
+  d->color_list.resize(ne);
   {
     std::vector<int32_t> fill(d->color_ptr.begin(), d->color_ptr.end() - 1);
-    for (int64_t e = 0; e < ne; ++e) d->order[fill[color[e]]++] = (int32_t)e;
+    for (int64_t e = 0; e < ne; ++e) d->color_list[fill[color[e]]++] = (int32_t)e;
   }
   auto bail = [&](int rc) {
     sld_dev_destroy(d);
@@ -509,13 +517,17 @@ int sld_dev_create(const sld_mesh_desc* mesh, const sld_material_desc* mat,
   int rc = 0;
 #define TRY(x) do { rc = (x); if (rc) return bail(rc); } while (0)
   TRY(upload(&d->lnods, ln.data(), ln.size(), s)); track(d, d->lnods);
+  TRY(upload(&d->color_list_dev, d->color_list.data(), d->color_list.size(), s));
+  track(d, d->color_list_dev);
   {
-    // slots: the corners at each node, ascending in the processed (colour-
-    // sorted) element order, so the gather adds in colour order
+    // slots: the corners at each node in colour order (color_list), so the
+    // gather adds them in the order of the coloured scatter
     for (int64_t p = 0; p < np; ++p) nadj[p + 1] += nadj[p];
     std::vector<int32_t> slot((size_t)ne * 8), fill(nadj.begin(), nadj.end() - 1);
-    for (int64_t i = 0; i < ne; ++i)
-      for (int a = 0; a < 8; ++a) slot[8 * i + a] = fill[ln[8 * i + a]]++;
+    for (int64_t i = 0; i < ne; ++i) {
+      const int64_t e = d->color_list[i];
+      for (int a = 0; a < 8; ++a) slot[8 * e + a] = fill[ln[8 * e + a]]++;
+    }
     TRY(upload(&d->node_ptr, nadj.data(), nadj.size(), s)); track(d, d->node_ptr);
     TRY(upload(&d->corner_slot, slot.data(), slot.size(), s)); track(d, d->corner_slot);
   }
@@ -589,6 +601,7 @@ int sld_dev_create(const sld_mesh_desc* mesh, const sld_material_desc* mat,
     sld::MassArgs M{};
     M.e_begin = d->color_ptr[c];
     M.e_end = d->color_ptr[c + 1];
+    M.elem_list = d->color_list_dev;
     M.lnods = d->lnods;
     for (int k = 0; k < 3; ++k) M.coord[k] = d->coord[k];
     M.rho = d->rho;
@@ -876,7 +889,8 @@ int sld_dev_coloring(const sld_dev* d, int32_t* ncolor, int32_t* color_ptr, int3
     if (max_colors < d->ncolor) return fail(SMA_ERR_ARG, "color_ptr too small");
     std::memcpy(color_ptr, d->color_ptr.data(), (d->ncolor + 1) * sizeof(int32_t));
   }
-  if (elem_order) std::memcpy(elem_order, d->order.data(), d->nelem * sizeof(int32_t));
+  if (elem_order)
+    for (int64_t i = 0; i < d->nelem; ++i) elem_order[i] = d->order[d->color_list[i]];
   return SMA_OK;
 }
 

@@ -218,7 +218,8 @@ __global__ void __launch_bounds__(256) sld_correct(NodalArgs A, double dt) {
 // Element kernel
 // ----------------------------------------------------------------------------
 struct ElemArgs {
-  int32_t e_begin, e_end;       // colour range in the colour-sorted element order
+  int32_t e_begin, e_end;       // range of the launch in elem_list (or of elements)
+  const int32_t* elem_list;     // scatter: elements sorted by colour; gather: null (identity)
   int64_t nelem;
   const int32_t* lnods;         // [nelem][8]
   const double* coord[3];       // SoA reference coordinates
@@ -315,8 +316,9 @@ __global__ void __launch_bounds__(EPB * 8, ElemOccupancy<R>::waves) sld_element(
   const int per = A.tiles_per_block > 0 ? A.tiles_per_block : 1;
   const int t_begin = bid * per, t_end = min(ntiles, t_begin + per);
   for (int blk = t_begin; blk < t_end; ++blk) {
-  const int e = A.e_begin + blk * EPB + le;
-  const bool valid = e < A.e_end;
+  const int i_el = A.e_begin + blk * EPB + le;
+  const bool valid = i_el < A.e_end;
+  const int e = !valid ? 0 : A.elem_list ? A.elem_list[i_el] : i_el;
 
   // -- phase 1: gather the element's 8 nodes into LDS (thread g loads node g)
   int node = 0;
@@ -549,7 +551,8 @@ __global__ void __launch_bounds__(256) sld_sur_check(SurCheckArgs A) {
 // Setup: lumped mass (sld_elmmat.f90:199-209), one thread per element, per colour.
 // ----------------------------------------------------------------------------
 struct MassArgs {
-  int32_t e_begin, e_end;
+  int32_t e_begin, e_end;       // colour range in elem_list
+  const int32_t* elem_list;
   const int32_t* lnods;
   const double* coord[3];
   const double* rho;
@@ -559,8 +562,9 @@ struct MassArgs {
 
 __global__ void sld_mass(MassArgs A) {
 #pragma clang fp contract(off)
-  const int e = A.e_begin + blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= A.e_end) return;
+  const int i_el = A.e_begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i_el >= A.e_end) return;
+  const int e = A.elem_list[i_el];
   double x[8][3];
   int nodes[8];
   for (int a = 0; a < 8; ++a) {
