@@ -91,6 +91,7 @@ struct sld_dev {
   double* rho = nullptr;
   int32_t* mat_of_elem = nullptr;
   double* E_of_elem = nullptr;
+  double* kg = nullptr;              // [nelem][2] bulk, shear modulus
   double* sdv[2][3] = {};
   int cur = 1;                       // index of the TIME_N history and clock slot
   double* S_out = nullptr;
@@ -267,6 +268,7 @@ int enqueue_step(sld_dev* d, double dt, int scheme, double phi, int tn, KernelTi
     E.props = d->props;
     E.mat_of_elem = d->mat_of_elem;
     E.E_of_elem = d->E_of_elem;
+    E.kg = d->kg;
     E.S_out = d->S_out;
     E.flags_out = d->flags_out;
     E.counters = d->counters;
@@ -583,6 +585,9 @@ int sld_dev_create(const sld_mesh_desc* mesh, const sld_material_desc* mat,
   }
   const int64_t ngp = ne * 8;
   TRY(dalloc((double**)&d->efor, (size_t)ngp * 3)); track(d, d->efor);
+  TRY(dalloc(&d->kg, (size_t)ne * 2)); track(d, d->kg);
+  hipLaunchKernelGGL(sld::sld_elem_moduli, dim3(nblocks(ne, 256)), dim3(256), 0, s, ne, d->props,
+                     d->mat_of_elem, d->E_of_elem, d->kg);
   for (int sl = 0; sl < 2; ++sl)
     for (int k = 0; k < 3; ++k) {
       TRY(dalloc(&d->sdv[sl][k], ngp)); track(d, d->sdv[sl][k]);

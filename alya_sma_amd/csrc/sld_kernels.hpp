@@ -118,35 +118,44 @@ struct NodalArgs {
   unsigned* sur_count;         // surrogate compact-list counter, cleared per step
 };
 
-template <int SCHEME>  // 0 = central difference, 1 = Tchamwa-Wielgosz
-__global__ void __launch_bounds__(256) sld_predict(NodalArgs A, double dt, double phi) {
+// Predictor of node i from the step-n state (u, v, a): u^{n+1} with the
+// Dirichlet values at t^{n+1} = cutim, and v^{n+1/2} (CD) or v^{n+1} (TW).
+// scheme: 0 = central difference, 1 = Tchamwa-Wielgosz.
+__device__ __forceinline__ void predict_node(const NodalArgs& A, int64_t i, int scheme, double dt,
+                                             double phi, double cutim, double un[3], double vn[3]) {
 #pragma clang fp contract(off)
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= A.npoin) return;
-  if (i == 0 && A.sur_count) A.sur_count[0] = 0u;
-  const double cutim = A.clock[A.tn] + dt;
   const double t_end = cutim + dt, t_mid = 0.5 * (cutim + t_end);
   const int fix = A.fixmask[i];
-  const int fn = A.func_of_node ? A.func_of_node[i] : -1;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    double u = A.u[d][i], v = A.v[d][i], a = A.a[d][i];
-    double un, vn;
-    if (SCHEME == 0) {
-      vn = v + (t_mid - cutim) * a;
-      un = u + dt * vn;
+    const double u = A.u[d][i], v = A.v[d][i], a = A.a[d][i];
+    if (scheme == 0) {
+      vn[d] = v + (t_mid - cutim) * a;
+      un[d] = u + dt * vn[d];
     } else {
-      vn = v + dt * a;
-      un = u + dt * v + phi * (dt * dt) * a;
+      vn[d] = v + dt * a;
+      un[d] = u + dt * v + phi * (dt * dt) * a;
     }
     if ((fix >> d) & 1) {
-      un = 0.0;
+      un[d] = 0.0;
+      const int fn = A.func_of_node ? A.func_of_node[i] : -1;
       if (fn >= 0)
-        un = funbou(A.func_table + (size_t)fn * A.func_maxpts * 4, A.func_npts[fn],
-                    A.func_smooth[fn], d, cutim);
+        un[d] = funbou(A.func_table + (size_t)fn * A.func_maxpts * 4, A.func_npts[fn],
+                       A.func_smooth[fn], d, cutim);
     }
-    A.u[d][i] = un;
-    A.v[d][i] = vn;
+  }
+}
+
+template <int SCHEME>  // 0 = central difference, 1 = Tchamwa-Wielgosz
+__global__ void __launch_bounds__(256) sld_predict(NodalArgs A, double dt, double phi) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.npoin) return;
+  double un[3], vn[3];
+  predict_node(A, i, SCHEME, dt, phi, A.clock[A.tn] + dt, un, vn);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    A.u[d][i] = un[d];
+    A.v[d][i] = vn[d];
     if (A.zero_fint) {
       if (A.fint_fp32) ((float*)A.fint[d])[i] = 0.f;
       else ((double*)A.fint[d])[i] = 0.0;
@@ -181,7 +190,10 @@ __global__ void __launch_bounds__(256) sld_correct(NodalArgs A, double dt) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= A.npoin) return;
   const double cutim = A.clock[A.tn] + dt;  // t^{n+1}
-  if (i == 0) A.clock[1 - A.tn] = cutim;
+  if (i == 0) {
+    A.clock[1 - A.tn] = cutim;
+    if (A.sur_count) A.sur_count[0] = 0u;   // the next step's compact list
+  }
   const double t_end = cutim + dt, t_mid = 0.5 * (cutim + t_end);
   double fx = 0.0;
   if (A.fext_func >= 0)
@@ -227,6 +239,7 @@ struct ElemArgs {
   const double* props;          // [nmat][18]
   const int32_t* mat_of_elem;   // [nelem] or null
   const double* E_of_elem;      // [nelem] or null
+  const double* kg;             // [nelem][2] bulk and shear modulus per element
   const double* sdv_n[3];       // TIME_N history, SoA over GP = 8*e + g
   double* sdv_k[3];             // ITER_K history
   void* fint[3];                // scatter: nodal internal force (double or float)
@@ -276,10 +289,10 @@ __device__ __forceinline__ void count_flags(unsigned long long* ctr, uint32_t fl
 // GATHER = false: read-modify-write of f_int (the launch covers one colour);
 // true: per-corner forces to efor (the launch may cover every element).
 #ifndef SLD_WAVES_F64
-#define SLD_WAVES_F64 1
+#define SLD_WAVES_F64 2
 #endif
 #ifndef SLD_WAVES_F32
-#define SLD_WAVES_F32 1
+#define SLD_WAVES_F32 4
 #endif
 template <typename R>
 struct ElemOccupancy { static constexpr int waves = sizeof(R) == 8 ? SLD_WAVES_F64 : SLD_WAVES_F32; };
@@ -346,14 +359,15 @@ __global__ void __launch_bounds__(EPB * 8, ElemOccupancy<R>::waves) sld_element(
   if (valid) {
     const int gk = opaque(g);
     R J[9] = {0}, Gu[9] = {0};  // col-major: J(i,k) = J[i + 3k]
+    // node 0 is the origin of the relative coordinates: its terms vanish
 #pragma unroll
-    for (int a = 0; a < 8; ++a) {
-      R D0 = hex_deriv<R>(a, gk, 0), D1 = hex_deriv<R>(a, gk, 1), D2 = hex_deriv<R>(a, gk, 2);
+    for (int a = 1; a < 8; ++a) {
+      const R D[3] = {hex_deriv<R>(a, gk, 0), hex_deriv<R>(a, gk, 1), hex_deriv<R>(a, gk, 2)};
 #pragma unroll
       for (int i = 0; i < 3; ++i) {
         R x = s_x[le][a][i], uu = s_u[le][a][i];
-        J[i] += x * D0; J[i + 3] += x * D1; J[i + 6] += x * D2;
-        Gu[i] += uu * D0; Gu[i + 3] += uu * D1; Gu[i + 6] += uu * D2;
+        J[i] += x * D[0]; J[i + 3] += x * D[1]; J[i + 6] += x * D[2];
+        Gu[i] += uu * D[0]; Gu[i + 3] += uu * D[1]; Gu[i + 6] += uu * D[2];
       }
     }
     R Ji[9];
@@ -378,8 +392,7 @@ __global__ void __launch_bounds__(EPB * 8, ElemOccupancy<R>::waves) sld_element(
 
     const int mat = A.mat_of_elem ? A.mat_of_elem[e] : 0;
     const double* pr = A.props + 18 * mat;
-    const double E = A.E_of_elem ? A.E_of_elem[e] : pr[0];
-    const sma::Mat m = sma::make_mat(pr, E);
+    const sma::Mat m = sma::make_mat_kg(pr, A.kg[2 * (int64_t)e], A.kg[2 * (int64_t)e + 1]);
     const int64_t gp = (int64_t)e * 8 + g;
     const double xS0 = A.sdv_n[0][gp], FAS0 = A.sdv_n[1][gp], FSA0 = A.sdv_n[2][gp];
 
@@ -419,6 +432,7 @@ __global__ void __launch_bounds__(EPB * 8, ElemOccupancy<R>::waves) sld_element(
           st = 2;
           if (kf_AS != kf_SA) {
             const unsigned slot = atomicAdd(A.sur_count, 1u);
+            const double E = A.E_of_elem ? A.E_of_elem[e] : pr[0];
             mlp::make_features(pr, E, A.T_feat, sp.e[0], sp.e[1], sp.e[2], sp.theta, xS0, FAS0,
                                FSA0, FAS, A.sur_feat + (int64_t)slot * mlp::NIN);
             A.sur_gp[slot] = (int32_t)gp;
@@ -590,6 +604,16 @@ __global__ void sld_mass(MassArgs A) {
         elmuu[a] = elmuu[a] + rho * vol * hex_shape<double>(a, g) * hex_shape<double>(b, g);
   }
   for (int a = 0; a < 8; ++a) A.mass[nodes[a]] += elmuu[a];
+}
+
+// Setup: bulk and shear modulus per element (the expressions of make_mat).
+__global__ void sld_elem_moduli(int64_t ne, const double* __restrict__ props,
+                                const int32_t* __restrict__ mat_of_elem,
+                                const double* __restrict__ E_of_elem, double* __restrict__ kg) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  const double* pr = props + 18 * (mat_of_elem ? mat_of_elem[e] : 0);
+  sma::elastic_moduli(pr, E_of_elem ? E_of_elem[e] : pr[0], kg[2 * e], kg[2 * e + 1]);
 }
 
 __global__ void sld_reciprocal(const double* __restrict__ m, double* __restrict__ rm, int64_t n) {

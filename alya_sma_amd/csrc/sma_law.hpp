@@ -39,16 +39,25 @@ struct Mat {
   double K, G, eL, alfa, Rf_AS, Rs_AS, Rf_SA, Rs_SA, G_1, C_AS, C_SA;
 };
 
-__host__ __device__ __forceinline__ Mat make_mat(const double* __restrict__ p, double E) {
+// Bulk and shear modulus from Young's modulus E and props(2) = nu.
+__host__ __device__ __forceinline__ void elastic_moduli(const double* __restrict__ p, double E,
+                                                       double& K, double& G) {
+#pragma clang fp contract(off)
+  const double v = p[1];
+  K = E / (3.0 - 6.0 * v);
+  G = E / (2.0 + 2.0 * v);
+}
+
+// Material constants with K, G given (precomputed per element).
+__host__ __device__ __forceinline__ Mat make_mat_kg(const double* __restrict__ p, double K, double G) {
 #pragma clang fp contract(off)
   Mat m;
-  double v = p[1];
   m.eL = p[2];
   m.alfa = p[14];
   m.C_AS = p[5];
   m.C_SA = p[11];
-  m.K = E / (3.0 - 6.0 * v);
-  m.G = E / (2.0 + 2.0 * v);
+  m.K = K;
+  m.G = G;
   double aux = sqrt(2.0 / 3.0) + m.alfa;
   m.Rf_AS = p[4] * aux + p[5] * p[6];
   m.Rs_AS = p[3] * aux + p[5] * p[7];
@@ -56,6 +65,21 @@ __host__ __device__ __forceinline__ Mat make_mat(const double* __restrict__ p, d
   m.Rs_SA = p[9] * aux + p[11] * p[13];
   m.G_1 = (2.0 * m.G + 9.0 * (m.alfa * m.alfa) * m.K) * m.eL;
   return m;
+}
+
+__host__ __device__ __forceinline__ Mat make_mat(const double* __restrict__ p, double E) {
+  double K, G;
+  elastic_moduli(p, E, K, G);
+  return make_mat_kg(p, K, G);
+}
+
+// 1/x: the hardware reciprocal (1 ulp) on the device, for FP32-only paths.
+__host__ __device__ __forceinline__ float fast_rcp(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_rcpf(x);
+#else
+  return 1.0f / x;
+#endif
 }
 
 // One Givens step of the implicit-shift QL sweep (dsyevq3.f90:99-129) on
@@ -325,7 +349,7 @@ __host__ __device__ __forceinline__ Hencky32 hencky_fp32(const float (&H)[9]) {
   a[3] = m00 * m22 - c[2] * c[2];
   a[4] = c[1] * c[2] - m00 * c[4];
   a[5] = m00 * m11 - c[1] * c[1];
-  const float r = 1.0f / (m00 * a[0] + c[1] * a[1] + c[2] * a[2]);
+  const float r = fast_rcp(m00 * a[0] + c[1] * a[1] + c[2] * a[2]);
 #pragma unroll
   for (int q = 0; q < 6; ++q) a[q] *= r;
   float z[6];
@@ -333,25 +357,26 @@ __host__ __device__ __forceinline__ Hencky32 hencky_fp32(const float (&H)[9]) {
   const float zf2 = z[0] * z[0] + z[3] * z[3] + z[5] * z[5] +
                     2.f * (z[1] * z[1] + z[2] * z[2] + z[4] * z[4]);
   h.ok = zf2 <= 0.36f;
+  // terms after Z^(2n+1) sum to below 2^-24 of |E| ~ |Z| when
+  // |Z|^(2n+2) / ((2n+3)(1 - |Z|^2)) <= 2^-24 (|Z| <= |Z|_F)
+  const int nterms = zf2 <= 1.7e-7f ? 0 : zf2 <= 5.4e-4f ? 1 : zf2 <= 7.3e-3f ? 2
+                   : zf2 <= 2.6e-2f ? 3 : zf2 <= 5.2e-2f ? 4 : zf2 <= 8.4e-2f ? 5
+                   : zf2 <= 0.115f ? 6 : zf2 <= 0.17f ? 8 : zf2 <= 0.22f ? 10
+                   : zf2 <= 0.28f ? 14 : 20;
   float z2[6], t[6], e[6];
   sym_mul_commuting(z, z, z2);
 #pragma unroll
   for (int q = 0; q < 6; ++q) t[q] = e[q] = z[q];
   if (h.ok) {
-    // |Z^(2k+1)|/(2k+1) below 2^-26 of |E|: at most 18 terms at |Z|_F = 0.6
-    for (int k = 1; k < 24; ++k) {
+    for (int k = 1; k <= nterms; ++k) {
       float tn[6];
       sym_mul_commuting(t, z2, tn);
-      const float w = 1.0f / (float)(2 * k + 1);
-      float tmax = 0.f, emax = 0.f;
+      const float w = fast_rcp((float)(2 * k + 1));
 #pragma unroll
       for (int q = 0; q < 6; ++q) {
         t[q] = tn[q];
         e[q] += w * tn[q];
-        tmax = fmaxf(tmax, fabsf(tn[q]));
-        emax = fmaxf(emax, fabsf(e[q]));
       }
-      if (w * tmax <= 1.4901161e-08f * emax) break;
     }
   }
   const float th = e[0] + e[3] + e[5];
@@ -376,7 +401,7 @@ __host__ __device__ __forceinline__ void kirchhoff_dev(const Mat& m, const Henck
   const float cvol = (float)(m.K * (h.theta - 3.0 * m.alfa * m.eL * xS));
   const float twoG = (float)(2.0 * m.G);
   const float en = (float)h.e_n;
-  const float coef = en > 1.1920929e-07f ? twoG * (1.0f - (float)(m.eL * xS) / en) : twoG;
+  const float coef = en > 1.1920929e-07f ? twoG * (1.0f - (float)(m.eL * xS) * fast_rcp(en)) : twoG;
   t[0] = cvol + coef * h.ed[0];
   t[1] = coef * h.ed[1];
   t[2] = coef * h.ed[2];
@@ -529,14 +554,17 @@ __host__ __device__ __forceinline__ void kirchhoff(const Mat& m, const Spectral<
   t[4] = c0 * n[1] * n[2] + c1 * n[4] * n[5] + c2 * n[7] * n[8];
 }
 
-// Column-major 3x3 inverse by cofactors (one reciprocal); returns det.
+// Column-major 3x3 inverse by cofactors (one reciprocal); returns det.  In
+// FP32 the reciprocal is the hardware one (1 ulp).
 template <typename R>
 __host__ __device__ __forceinline__ R inv3(const R (&M)[9], R (&Mi)[9]) {
   R c00 = M[4] * M[8] - M[5] * M[7];
   R c01 = M[2] * M[7] - M[1] * M[8];
   R c02 = M[1] * M[5] - M[2] * M[4];
   R det = M[0] * c00 + M[3] * c01 + M[6] * c02;
-  R r = R(1) / det;
+  R r;
+  if constexpr (sizeof(R) == 4) r = fast_rcp(det);
+  else r = R(1) / det;
   Mi[0] = c00 * r;
   Mi[1] = c01 * r;
   Mi[2] = c02 * r;
