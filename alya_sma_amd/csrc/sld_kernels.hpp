@@ -247,7 +247,7 @@ struct ElemArgs {
   const int32_t* corner_slot;   // gather: [8 * e + a] -> slot of the corner at its node
   double* S_out;                // optional PK2 [9][ngp]
   uint32_t* flags_out;          // optional [ngp]
-  unsigned long long* counters; // [8] per-flag Gauss-point counts
+  unsigned long long* counters; // [COUNTER_SLOTS][8] per-flag Gauss-point counts
   int32_t xcd_remap;            // 1 = spread consecutive tiles per XCD
   int32_t tiles_per_block;      // > 0: persistent launch, tiles per workgroup
   // surrogate (mode 2): per-GP status (0 exact, 1 accepted, 2 fallback) and
@@ -268,8 +268,15 @@ struct ElemTraits<double> { using Acc = double; };
 template <>
 struct ElemTraits<float> { using Acc = float; };
 
+// Flag counters live in COUNTER_SLOTS slots of 8 (one 64-byte line each);
+// workgroup b adds into slot b % COUNTER_SLOTS, so the per-wave atomics of a
+// launch spread over many addresses instead of serialising on one.  The host
+// sums the slots.
+constexpr int COUNTER_SLOTS = 256;
+
 // Wave-level flag counters: one 64-bit atomic per wave and per set bit type.
 __device__ __forceinline__ void count_flags(unsigned long long* ctr, uint32_t fl, bool valid) {
+  ctr += 8 * (blockIdx.x % COUNTER_SLOTS);
   const uint32_t bits[7] = {sma::FLAG_KF_AS, sma::FLAG_KF_SA, sma::FLAG_NEWTON_FAIL,
                             sma::FLAG_EIGEN_FAIL, sma::FLAG_DETF_NONPOS,
                             sma::FLAG_SURR_USED, sma::FLAG_SURR_FALLBACK};
@@ -427,11 +434,19 @@ __global__ void __launch_bounds__(EPB * 8, ElemOccupancy<R>::waves) sld_element(
         bool kf_AS, kf_SA;
         double FAS, FSA;
         const bool flagged = sma::flags_only(m, sp.theta, sp.e_n, xS0, FAS0, FSA0, kf_AS, kf_SA, FAS, FSA);
+        // single-branch points go to the compact list: one atomic per wave
+        const bool want = flagged && (kf_AS != kf_SA);
+        const unsigned long long wm = __ballot(want);
+        const int lane = threadIdx.x & 63;
+        const int leader = wm ? __ffsll((long long)wm) - 1 : 0;
+        unsigned base = 0;
+        if (wm && lane == leader) base = atomicAdd(A.sur_count, (unsigned)__popcll(wm));
+        base = __shfl(base, leader);
         uint8_t st = 0;
         if (flagged) {
           st = 2;
-          if (kf_AS != kf_SA) {
-            const unsigned slot = atomicAdd(A.sur_count, 1u);
+          if (want) {
+            const unsigned slot = base + (unsigned)__popcll(wm & ((1ull << lane) - 1ull));
             const double E = A.E_of_elem ? A.E_of_elem[e] : pr[0];
             mlp::make_features(pr, E, A.T_feat, sp.e[0], sp.e[1], sp.e[2], sp.theta, xS0, FAS0,
                                FSA0, FAS, A.sur_feat + (int64_t)slot * mlp::NIN);

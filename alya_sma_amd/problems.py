@@ -102,10 +102,11 @@ def bar_problem(nx=10, ny=2, nz=2, lx=10.0, ly=2.0, lz=2.0, strain=0.06,
 
 def cantilever_problem(nx=400, ny=20, nz=20, lx=50.0, ly=2.5, lz=2.5, traction=500.0,
                        nsteps=5000, safety=0.5, seed=1, props=NITINOL, rho=RHO_NITINOL,
-                       E_noise=0.02, use_lib=False):
-    """Cantilever clamped at x = 0 with a tip traction in y on x = lx that is
-    ramped up and released (DISCRETE,SMOOTH 0 -> 1 -> 0 over nsteps).  The
-    per-element Young's modulus carries seeded uniform noise of +-E_noise."""
+                       E_noise=0.02, use_lib=False, direction=1):
+    """Cantilever clamped at x = 0 with a tip traction on x = lx in direction
+    `direction` (0 = x, 1 = y) that is ramped up and released
+    (DISCRETE,SMOOTH 0 -> 1 -> 0 over nsteps).  The per-element Young's
+    modulus carries seeded uniform noise of +-E_noise."""
     coord, lnods = (box_mesh if use_lib else box_mesh_np)(nx, ny, nz, lx, ly, lz)
     npoin, nelem = coord.shape[0], lnods.shape[0]
     tol = 1e-9 * lx
@@ -118,7 +119,7 @@ def cantilever_problem(nx=400, ny=20, nz=20, lx=50.0, ly=2.5, lz=2.5, traction=5
     wy = np.where((np.abs(coord[:, 1]) < tol) | (np.abs(coord[:, 1] - ly) < tol), 0.5, 1.0)
     wz = np.where((np.abs(coord[:, 2]) < tol) | (np.abs(coord[:, 2] - lz) < tol), 0.5, 1.0)
     fext0 = np.zeros((npoin, 3))
-    fext0[tip, 1] = traction * hy * hz * (wy * wz)[tip]
+    fext0[tip, direction] = traction * hy * hz * (wy * wz)[tip]
     rng = np.random.default_rng(seed)
     P = np.asarray(props, np.float64).reshape(-1, 18)
     E_of_elem = P[0, 0] * (1.0 + E_noise * rng.uniform(-1.0, 1.0, nelem))

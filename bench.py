@@ -218,7 +218,7 @@ def main():
         if args.epb:
             dev.set_tile(args.epb)
         dev.step(dt, nsteps=W)
-        cfg = {"assembly": args.assembly or "gather", "elems_per_block": args.epb or 32,
+        cfg = {"assembly": args.assembly or "gather", "elems_per_block": args.epb or 8,
                "xcd_remap": 1, "persistent": 0}
         if tune:
             cfg = dev.autotune(dt, nsteps=3)

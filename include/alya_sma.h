@@ -130,10 +130,12 @@ int sld_dev_step(sld_dev *dev, double dt, int32_t scheme, double phi, int64_t ns
 /* Runs nsteps steps eagerly with HIP events around the predictor, the
  * element phase and the corrector on the handle's stream (the reference
  * brackets the same phases with its times(3..7) timers,
- * sld_element_operations.f90:125-256).  out[6]: ms per launch of the
- * predictor, the element kernel (element phase / its launches) and the
- * corrector, element launches per step, summed bracketed ms per step,
- * stream ms per step. */
+ * sld_element_operations.f90:125-256).  With the gather assembly the
+ * element launch is repeated 8 times back to back inside the bracket (same
+ * inputs, same outputs; flag counts of the repeats are discarded).  out[6]:
+ * ms per launch of the predictor, the element kernel (bracket / launches
+ * in it) and the corrector, element launches per step, kernel ms per step,
+ * eager stream ms per step (repeats included). */
 int sld_dev_time_kernels(sld_dev *dev, double dt, int32_t scheme, double phi,
                          int32_t nsteps, double *out);
 
@@ -165,10 +167,11 @@ int sld_dev_set(sld_dev *dev, int32_t field, const void *host, int64_t count);
 int sld_dev_coloring(const sld_dev *dev, int32_t *ncolor, int32_t *color_ptr,
                      int32_t max_colors, int32_t *elem_order);
 
-/* Launch shape of the element kernel: elements per workgroup (8, 16 or 32),
- * XCD-contiguous tile mapping on/off, persistent = workgroups per CU of a
- * persistent launch (0 = one workgroup per tile).  sld_dev_autotune picks
- * these by timing. */
+/* Launch shape of the element kernel: elements per workgroup (8, 16 or 32;
+ * default 8 = one wave per workgroup), XCD-contiguous tile mapping on/off
+ * (default on), persistent = workgroups per CU of a persistent launch
+ * (0 = one workgroup per tile, default).  sld_dev_autotune picks these by
+ * timing. */
 int sld_dev_set_tile(sld_dev *dev, int32_t elems_per_block, int32_t xcd_remap, int32_t persistent);
 
 /* Assembly of the internal force (both deterministic, bitwise equal):
