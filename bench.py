@@ -18,6 +18,11 @@ next to the same difference between two FP64 runs whose initial states
 differ by 1e-13 (relative, in u at step 20): transformation fronts make the
 undamped load-unload cycle sensitive to any rounding, so the end-of-run
 difference measures that sensitivity rather than the precision.
+
+The K timed steps run in ten chunks; two extra eager steps between chunks
+(outside the timed sum) time each kernel with HIP events, so `kernel_ms`
+and the roofline's `launch_ms` are averages over the whole run.  Every
+mode therefore advances W + K + 20 steps.
 """
 import argparse
 import json
@@ -67,22 +72,23 @@ def kernel_name(mode, cfg):
 def algorithmic_bytes(prob, dev, mode, assembly):
     """Bytes one element launch must move at least (average over launches).
 
-    Per Gauss point: history read + written 48 B, connectivity 4 B, Young's
-    modulus 1 B.  Per node the launch touches: coordinates 24 B and
-    displacement 24 B read; coloured scatter: f_int read-modify-write
-    (2 x 3 x 8 B in FP64, 2 x 3 x 4 B in FP32); gather: instead per element
-    corner the force written (3 x 8 B or 3 x 4 B)."""
+    Per Gauss point: history read + written 48 B, connectivity 4 B, the
+    element's bulk and shear moduli 2 B.  Gather assembly: per Gauss point
+    the corner's slot index 4 B and its force written (3 x 8 B in FP64, 3 x 4
+    B in FP32), per node coordinates + displacement read 48 B.  Coloured
+    scatter: per node the launch touches coordinates + displacement 48 B and
+    f_int read-modify-write (2 x 3 x 8 B or 2 x 3 x 4 B)."""
     fb = 8 if mode == "fp64" else 4
     lnods = prob["lnods"]
     if assembly == "gather":
         ne, npn = lnods.shape[0], prob["coord"].shape[0]
-        return float(ne * 8 * (53 + 3 * fb) + npn * 48)
+        return float(ne * 8 * (54 + 4 + 3 * fb) + npn * 48)
     ptr, order = dev.coloring()
     per = []
     for c in range(len(ptr) - 1):
         els = order[ptr[c]:ptr[c + 1]]
         npc = len(np.unique(lnods[els].ravel()))
-        per.append(len(els) * 8 * 53 + npc * (48 + 6 * fb))
+        per.append(len(els) * 8 * 54 + npc * (48 + 6 * fb))
     return float(np.mean(per))
 
 
@@ -224,16 +230,24 @@ def main():
             cfg = dev.autotune(dt, nsteps=3)
             cfg.pop("ms_per_step")
         cfgs[mode] = cfg
-        dev.sync()
-        synchronize()
-        t0 = time.perf_counter()
-        dev.step(dt, nsteps=K)
-        dev.sync()
-        synchronize()
-        el = time.perf_counter() - t0
+        # K timed steps in ten chunks, each bracketed by synchronisations;
+        # between chunks (outside the timed sum) one eager step with HIP
+        # events samples the per-launch kernel times, so their average
+        # covers the elastic and the transforming phases of the run
+        el, samples = 0.0, []
+        nchunk = min(10, K)
+        for c in range(nchunk):
+            n = K // nchunk + (1 if c < K % nchunk else 0)
+            dev.sync()
+            synchronize()
+            t0 = time.perf_counter()
+            dev.step(dt, nsteps=n)
+            dev.sync()
+            synchronize()
+            el += time.perf_counter() - t0
+            samples.append(dev.time_kernels(dt, nsteps=2))
         ms = 1e3 * el / K
-        # HIP-event timing of the phases over a few further (eager) steps
-        tk = dev.time_kernels(dt, nsteps=4)
+        tk = {k: float(np.mean([x[k] for x in samples])) for k in samples[0]}
         cnt = dev.counters()
         kname = kernel_name(mode, cfg)
         nbytes = algorithmic_bytes(prob, dev, mode, cfg["assembly"])
@@ -261,7 +275,7 @@ def main():
     sens = None
     if not args.no_accuracy:
         t0 = time.perf_counter()
-        acc, sens = accuracy(prob, modes, K + W, cfgs)
+        acc, sens = accuracy(prob, modes, K + W + 2 * min(10, K), cfgs)
         for m in modes:
             results[m].update(acc[m])
         print(f"# accuracy pass {time.perf_counter() - t0:.1f} s: {acc}, fp64 self-sensitivity "

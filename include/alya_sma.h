@@ -135,7 +135,9 @@ int sld_dev_step(sld_dev *dev, double dt, int32_t scheme, double phi, int64_t ns
  * inputs, same outputs; flag counts of the repeats are discarded).  out[6]:
  * ms per launch of the predictor, the element kernel (bracket / launches
  * in it) and the corrector, element launches per step, kernel ms per step,
- * eager stream ms per step (repeats included). */
+ * eager stream ms per step (repeats included).  The steps are enqueued back
+ * to back; with nsteps > 1 the first bracket (opened on an idle stream) is
+ * left out of the averages. */
 int sld_dev_time_kernels(sld_dev *dev, double dt, int32_t scheme, double phi,
                          int32_t nsteps, double *out);
 
