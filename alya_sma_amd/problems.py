@@ -137,3 +137,85 @@ def cantilever_problem(nx=400, ny=20, nz=20, lx=50.0, ly=2.5, lz=2.5, traction=5
     prob["run"] = dict(dt=dt, nsteps=nsteps, safety=safety, scheme=0, phi=1.033)
     prob["name"] = f"cantilever {nx}x{ny}x{nz} hex8 over {lx}x{ly}x{lz} mm"
     return prob
+
+
+def batched_specimens(n=256, seed=3, median=8000, sigma=0.7, lo=2000, hi=60000, nsteps=2000,
+                      width=1.0, safety=0.5, props=NITINOL, rho=RHO_NITINOL):
+    """Many uniaxial SMA specimens of uneven size stepped as one mesh.
+
+    Specimen s is a bar of cubic hex8 elements, n_e ~ LogNormal(ln median,
+    sigma) truncated to [lo, hi] by redrawing, aspect ratio U[5, 40], cross
+    section width x width mm; its own material row (E and the four
+    transformation stresses scaled by factors U[0.9, 1.1]) and its own
+    DISCRETE,SMOOTH function 0 -> amp*L -> 0 with amp ~ U[3%, 7%], prescribed
+    on u_x at x = L, with the symmetry planes of the uniaxial bar (u_x = 0 on
+    x = 0, u_y = 0 on y = 0, u_z = 0 on z = 0).  The specimens share no node;
+    one time step (the smallest critical step times `safety`) serves all.
+    Specimens are laid side by side in y (2 x width apart); that changes no
+    result, only where a plot would put them."""
+    rng = np.random.default_rng(seed)
+    sizes = np.empty(n, dtype=np.int64)
+    for s in range(n):
+        while True:
+            v = rng.lognormal(np.log(median), sigma)
+            if lo <= v <= hi:
+                break
+        sizes[s] = int(round(v))
+    aspect = rng.uniform(5.0, 40.0, n)
+    fE = rng.uniform(0.9, 1.1, n)
+    fS = rng.uniform(0.9, 1.1, n)
+    amp = rng.uniform(0.03, 0.07, n)
+    P0 = np.asarray(props, np.float64).reshape(18)
+    coords, lnods, fixes, funcs, mats, dims = [], [], [], [], [], []
+    P = np.zeros((n, 18))
+    rhos = np.full(n, rho, dtype=np.float64)
+    hmin_over_c = np.inf
+    node0 = 0
+    for s in range(n):
+        ny = max(1, int(round((sizes[s] / aspect[s]) ** (1.0 / 3.0))))
+        nx = max(1, int(round(sizes[s] / (ny * ny))))
+        h = width / ny
+        L = nx * h
+        c, ln = box_mesh_np(nx, ny, ny, L, width, width)
+        tol = 1e-9 * L
+        fx = np.zeros(c.shape[0], dtype=np.uint8)
+        fx[np.abs(c[:, 0]) < tol] |= 1
+        fx[np.abs(c[:, 1]) < tol] |= 2
+        fx[np.abs(c[:, 2]) < tol] |= 4
+        tip = np.abs(c[:, 0] - L) < tol
+        fx[tip] |= 1
+        fn = np.full(c.shape[0], -1, dtype=np.int32)
+        fn[tip] = s
+        c = c.copy()
+        c[:, 1] += 2.0 * width * s
+        coords.append(c)
+        lnods.append(ln + node0)
+        fixes.append(fx)
+        funcs.append(fn)
+        mats.append(np.full(ln.shape[0], s, dtype=np.int32))
+        dims.append((nx, ny, ny, L))
+        node0 += c.shape[0]
+        P[s] = P0
+        P[s, 0] = P0[0] * fE[s]
+        for k in (3, 4, 9, 10):
+            P[s, k] = P0[k] * fS[s]
+        E, nu = P[s, 0], P[s, 1]
+        lam = nu * E / ((1 + nu) * (1 - 2 * nu))
+        mu = E / (2 * (1 + nu))
+        hmin_over_c = min(hmin_over_c, h / np.sqrt((lam + 2 * mu) / rho))
+    dt = safety * hmin_over_c
+    t_end = nsteps * dt
+    tabs = np.stack([smooth_table(t_end, [[0, 0, 0], [amp[s] * dims[s][3], 0, 0], [0, 0, 0]])
+                     for s in range(n)])
+    prob = dict(coord=np.concatenate(coords), lnods=np.concatenate(lnods).astype(np.int32),
+                props=P, rho=rhos, mat_of_elem=np.concatenate(mats), fixmask=np.concatenate(fixes),
+                func_of_node=np.concatenate(funcs), func_table=tabs,
+                func_npts=np.full(n, 3, np.int32), func_smooth=np.ones(n, np.int32),
+                fext0=None, fext_func=-1)
+    prob["run"] = dict(dt=dt, nsteps=nsteps, safety=safety, scheme=0, phi=1.033)
+    prob["specimens"] = dict(elements=np.array([d[0] * d[1] * d[2] for d in dims]),
+                             dims=np.array([d[:3] for d in dims]), length=np.array([d[3] for d in dims]),
+                             amplitude=amp, E_factor=fE, stress_factor=fS)
+    prob["name"] = (f"{n} SMA specimens, {prob['lnods'].shape[0]} hex8 in total "
+                    f"({int(prob['specimens']['elements'].min())}-{int(prob['specimens']['elements'].max())} each)")
+    return prob

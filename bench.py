@@ -6,7 +6,10 @@ ramped up and released over 5000 central-difference steps, per-element
 Young's modulus with seeded +-2% noise.  Inputs are resident in HBM before
 the timed region; one step = predictor, element kernel(s), corrector.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--workload cantilever|specimens]
+
+`--workload specimens` runs the batch of 256 uneven SMA specimens (2.7 M
+hex8 in one mesh, 2000 steps) instead of the cantilever.
 
 Prints one JSON line.  `value` is the throughput of the fastest precision
 mode whose stress error stays within 1e-3.  The error of a mode is the
@@ -46,7 +49,8 @@ MODES = {"fp64": 0, "fp32": 1, "surrogate": 2}
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5000)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default: the workload's run)")
+    ap.add_argument("--workload", choices=["cantilever", "specimens"], default="cantilever")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--modes", default="fp64,fp32")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -92,15 +96,17 @@ def algorithmic_bytes(prob, dev, mode, assembly):
     return float(np.mean(per))
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
-    summary (FETCH_SIZE and WRITE_SIZE scaled by a plain copy kernel)."""
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    rocprofv3 PMC summary (FETCH_SIZE and WRITE_SIZE scaled by a plain copy
+    kernel), or None when that pair was not measured."""
     try:
         with open(TRAFFIC_FILE) as f:
             for line in f:
-                m = re.match(r"traffic_bytes_per_launch\s+(.+?)\s+=\s+([0-9.eE+]+)\s*$", line.strip())
-                if m and m.group(1) == kernel:
-                    return float(m.group(2))
+                m = re.match(r"traffic_bytes_per_launch\s+(\S+)\s+(.+?)\s+=\s+([0-9.eE+]+)\s*$",
+                             line.strip())
+                if m and m.group(1) == workload and m.group(2) == kernel:
+                    return float(m.group(3))
     except OSError:
         pass
     return None
@@ -137,7 +143,7 @@ def cpu_baseline(prob, budget_s):
     except Exception:
         pass
     return {"value": ngp * n / el, "unit": "GP-updates/s", "cores": threads, "kind": "port",
-            "sample": f"steps 2..{n + 1} of the same cantilever workload, FP64 C oracle "
+            "sample": f"steps 2..{n + 1} of the same workload, FP64 C oracle "
                       f"(oracle/sld_step.c, gcc -O3, OpenMP over the elements of each colour), "
                       f"{el:.1f} s wall on {threads} threads of {cpu}"}
 
@@ -208,11 +214,19 @@ def main():
 
     if device_count() < 1:
         sys.exit("bench.py needs a HIP device")
-    prob = problems.cantilever_problem()
+    if args.workload == "cantilever":
+        prob = problems.cantilever_problem()
+        workload = ("SMA cantilever hex8 400x20x20 (160000 elements, 1280000 Gauss points), "
+                    "50x2.5x2.5 mm, tip traction load-unload, E +-2% noise, central difference")
+    else:
+        prob = problems.batched_specimens()
+        workload = (prob["name"] + ", lognormal sizes (median 8000, sigma 0.7, 2k-60k), aspect "
+                    "5-40, props +-10%, displacement amplitude 3-7%, central difference")
     dt = prob["run"]["dt"]
     nelem = prob["lnods"].shape[0]
     ngp = nelem * 8
-    K, W = args.steps, args.warmup
+    K = args.steps if args.steps is not None else prob["run"]["nsteps"]
+    W = args.warmup
     modes = [m for m in args.modes.split(",") if m]
     tune = args.tune == "auto" and not args.assembly and not args.epb
     results, cfgs, roofs = {}, {}, {}
@@ -253,7 +267,8 @@ def main():
         nbytes = algorithmic_bytes(prob, dev, mode, cfg["assembly"])
         achieved = nbytes / (tk["element_ms"] * 1e-3) / 1e9
         roofs[mode] = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                       "frac": achieved / PEAK_HBM_GBS, "traffic": measured_traffic(kname),
+                       "frac": achieved / PEAK_HBM_GBS,
+                       "traffic": measured_traffic(args.workload, kname),
                        "kernel": kname, "algorithmic_bytes_per_launch": nbytes,
                        "launch_ms": tk["element_ms"]}
         results[mode] = {"value": ngp / (ms * 1e-3), "ms_per_step": ms, "launch": cfg,
@@ -289,9 +304,7 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": {"fp64": "f64", "fp32": "f32+f64", "surrogate": "bf16+f32+f64"}[best],
         "data": "synthetic",
-        "config": {"workload": "SMA cantilever hex8 400x20x20 (160000 elements, 1280000 Gauss "
-                               "points), 50x2.5x2.5 mm, tip traction load-unload, E +-2% noise, "
-                               "central difference", "mode": best, "dt": dt,
+        "config": {"workload": workload, "mode": best, "dt": dt,
                    "gauss_points": ngp, "nodes": int(prob["coord"].shape[0])},
         "modes": results,
         "fp64_self_sensitivity_trajectory_end": sens,

@@ -248,3 +248,34 @@ def test_fp32_mode_error_vs_fp64_oracle():
         print(f"fp32 mode, bar 1000 steps: stress rel L2 {eS:.2e}, u rel L2 {eu:.2e}, "
               f"xS max abs {exs:.2e}", flush=True)
         assert eS < 1e-3 and eu < 1e-3
+
+
+def test_batched_uneven_specimens_match_oracle():
+    """Config 3 in small: six bars of uneven size, each with its own material
+    row and load function, stepped as one mesh (one element launch) against
+    the oracle, restarted from the oracle's state every 25 steps."""
+    prob = problems.batched_specimens(n=6, median=80, lo=30, hi=300, nsteps=400)
+    dt = prob["run"]["dt"]
+    print("batched specimens:", prob["name"], flush=True)
+    dev, orc = _pair(prob)
+    with dev:
+        for chunk in range(16):
+            if chunk:
+                _sync_from_oracle(dev, orc)
+            dev.step(dt, nsteps=25)
+            orc.step(dt, n=25)
+            _compare(dev, orc, f"batch step {25 * (chunk + 1)}", quiet=chunk % 8 != 7)
+        xs = dev.get("sdv")[..., 0].max(axis=1)
+        mat = prob["mat_of_elem"]
+        assert all(np.any(xs[mat == s] > 0) for s in range(6))   # every specimen transformed
+
+
+def test_batched_specimens_persistent_launch_bitwise():
+    prob = problems.batched_specimens(n=12, median=200, lo=50, hi=900, nsteps=200)
+    dt = prob["run"]["dt"]
+    with Solidz(prob) as a, Solidz(prob) as b:
+        b.set_tile(8, xcd_remap=True, persistent=2)
+        a.step(dt, nsteps=120)
+        b.step(dt, nsteps=120)
+        for f in ("u", "v", "sdv"):
+            assert np.array_equal(a.get(f), b.get(f)), f
