@@ -537,9 +537,10 @@ __global__ void __launch_bounds__(EPB * 8, ElemOccupancy<R>::waves) sld_element(
 }
 
 // ----------------------------------------------------------------------------
-// Surrogate residual check on the compact list: accept the network's
-// transformation update when the consistency residual is within tol and the
-// inputs lie in the trained range; the element kernel uses accepted values.
+// Surrogate check on the compact list: accept the network's transformation
+// update when it lies within tol (in xS) of the return map's solution and
+// the inputs lie in the trained range; the element kernel uses accepted
+// values.
 // ----------------------------------------------------------------------------
 struct SurCheckArgs {
   const unsigned* count;

@@ -515,10 +515,14 @@ __host__ __device__ __forceinline__ ReturnMap return_map_given(const Mat& m, dou
   return r;
 }
 
-// Relative consistency residual of a candidate xS on a single-branch point:
-// the reference's Newton residual (:449-450) at dh = x - xS_0 (its one-step
-// solution makes it vanish), divided by the finish threshold R_f.  Returns
-// a large value for two-branch points (always solved exactly).
+// Distance, in xS, from a candidate x to the return map's solution on a
+// single-branch point.  With one branch active the reference's residual
+// (:449-450) is linear in the increment dh = x - xS_0 (the dh^2 terms of
+// (F(x) - R_f) dh and (F(x) - F_0)(1 - x) cancel because F is linear in x),
+// so R(x) / R'(x) is exactly x minus the root, i.e. the size of the Newton
+// correction the reference's first iteration would apply from x.  Returns a
+// large value for two-branch points (always solved exactly) and when the
+// slope vanishes.
 __host__ __device__ __forceinline__ double consistency_residual(const Mat& m, double theta, double e_n,
                                                                 double xS_0, double F_AS_0, double F_SA_0,
                                                                 double x) {
@@ -527,12 +531,20 @@ __host__ __device__ __forceinline__ double consistency_residual(const Mat& m, do
   flags_only(m, theta, e_n, xS_0, F_AS_0, F_SA_0, kf_AS, kf_SA, F_AS, F_SA);
   if (kf_AS == kf_SA) return 1e30;
   const double d = x - xS_0;
+  // dF/dx of the loading function (eval_loading_function, beta = 1)
+  const double k = -(2.0 * m.G * m.eL + 9.0 * m.alfa * m.alfa * m.K * m.eL);
+  double R, dR;
   if (kf_AS) {
     const double Fx = loading(m, m.C_AS, theta, e_n, x);
-    return fabs((Fx - m.Rf_AS) * d + (Fx - F_AS_0) * (1.0 - x)) / fabs(m.Rf_AS);
+    R = (Fx - m.Rf_AS) * d + (Fx - F_AS_0) * (1.0 - x);
+    dR = k * d + (Fx - m.Rf_AS) + k * (1.0 - x) - (Fx - F_AS_0);
+  } else {
+    const double Fx = loading(m, m.C_SA, theta, e_n, x);
+    R = (Fx - m.Rf_SA) * d - (Fx - F_SA_0) * x;
+    dR = k * d + (Fx - m.Rf_SA) - k * x - (Fx - F_SA_0);
   }
-  const double Fx = loading(m, m.C_SA, theta, e_n, x);
-  return fabs((Fx - m.Rf_SA) * d - (Fx - F_SA_0) * x) / fabs(m.Rf_SA);
+  if (!(fabs(dR) > 0.0)) return 1e30;
+  return fabs(R / dR);
 }
 
 // Kirchhoff stress (:487-491) as a symmetric tensor t00 t01 t02 t11 t12 t22.

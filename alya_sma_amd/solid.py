@@ -124,6 +124,13 @@ class Solidz:
         check(lib().sld_dev_set_tile(self.h, int(elems_per_block), 1 if xcd_remap else 0,
                                      int(persistent)), "sld_dev_set_tile")
 
+    def set_surrogate(self, surrogate, tol=1e-4):
+        """Attach a trained alya_sma_amd.surrogate.Surrogate: mode 2 then
+        takes its xS update on single-branch transforming points that lie
+        within tol (in xS) of the return map's solution; None detaches."""
+        h = surrogate.h if surrogate is not None else None
+        check(lib().sld_dev_set_surrogate(self.h, h, float(tol)), "sld_dev_set_surrogate")
+
     def set_assembly(self, assembly):
         """"scatter" (one element launch per colour) or "gather" (one launch,
         per-node sum in the corrector); both give bitwise equal results."""

@@ -52,7 +52,9 @@ def parse():
     ap.add_argument("--steps", type=int, default=None, help="timed steps (default: the workload's run)")
     ap.add_argument("--workload", choices=["cantilever", "specimens"], default="cantilever")
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--modes", default="fp64,fp32")
+    ap.add_argument("--modes", default="fp64,fp32,surrogate")
+    ap.add_argument("--sur-tol", type=float, default=1e-4,
+                    help="surrogate acceptance: distance in xS to the return map's solution")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the error pass")
@@ -148,15 +150,34 @@ def cpu_baseline(prob, budget_s):
                       f"{el:.1f} s wall on {threads} threads of {cpu}"}
 
 
-def accuracy(prob, modes, K, cfgs):
+def surrogate_network():
+    """The committed trained network, or one trained here (not timed) when
+    the file is absent."""
+    from alya_sma_amd import surrogate as S
+    s = S.load_default()
+    if s is None:
+        print("# training the surrogate (no committed weights)", file=sys.stderr, flush=True)
+        s = S.train_default(n_samples=4_000_000, steps=6000,
+                            log=lambda m: print("# " + m, file=sys.stderr, flush=True))
+    return s
+
+
+def make_device(prob, mode, sur=None, tol=1e-4):
+    from alya_sma_amd.solid import Solidz
+    d = Solidz(prob)
+    if mode == "surrogate":
+        d.set_surrogate(sur, tol)
+    d.set_mode(MODES[mode])
+    return d
+
+
+def accuracy(prob, modes, K, cfgs, sur=None, tol=1e-4):
     """One-step stress error at checkpoints, trajectory error at the end,
     FP64 self-sensitivity (see the module docstring)."""
-    from alya_sma_amd.solid import Solidz
     dt = prob["run"]["dt"]
 
     def make(mode, store):
-        d = Solidz(prob)
-        d.set_mode(MODES[mode])
+        d = make_device(prob, mode, sur, tol)
         c = cfgs[mode]
         d.set_assembly(c["assembly"])
         d.set_tile(c["elems_per_block"], c["xcd_remap"], c["persistent"])
@@ -230,9 +251,9 @@ def main():
     modes = [m for m in args.modes.split(",") if m]
     tune = args.tune == "auto" and not args.assembly and not args.epb
     results, cfgs, roofs = {}, {}, {}
+    sur = surrogate_network() if "surrogate" in modes else None
     for mode in modes:
-        dev = Solidz(prob)
-        dev.set_mode(MODES[mode])
+        dev = make_device(prob, mode, sur, args.sur_tol)
         if args.assembly:
             dev.set_assembly(args.assembly)
         if args.epb:
@@ -274,6 +295,9 @@ def main():
         results[mode] = {"value": ngp / (ms * 1e-3), "ms_per_step": ms, "launch": cfg,
                          "kernel_ms": {k: round(v, 5) for k, v in tk.items()},
                          "transforming_gp_updates": cnt["kf_AS"] + cnt["kf_SA"]}
+        if mode == "surrogate":
+            results[mode]["surrogate_accepted"] = cnt["surrogate_used"]
+            results[mode]["surrogate_fallback"] = cnt["surrogate_fallback"]
         dev.close()
         print(f"# {mode}: {ms:.4f} ms/step, {results[mode]['value']:.4e} GP/s, {cfg}, kernels "
               f"{results[mode]['kernel_ms']}", file=sys.stderr, flush=True)
@@ -290,7 +314,7 @@ def main():
     sens = None
     if not args.no_accuracy:
         t0 = time.perf_counter()
-        acc, sens = accuracy(prob, modes, K + W + 2 * min(10, K), cfgs)
+        acc, sens = accuracy(prob, modes, K + W + 2 * min(10, K), cfgs, sur, args.sur_tol)
         for m in modes:
             results[m].update(acc[m])
         print(f"# accuracy pass {time.perf_counter() - t0:.1f} s: {acc}, fp64 self-sensitivity "

@@ -241,10 +241,12 @@ int sma_mlp_gen_samples_device(int64_t n, uint64_t seed, const double *props_dev
                                float *X, float *T, uint32_t *flags, void *stream);
 
 /* Attach a trained surrogate to a step handle (mode SLD_MODE_SURROGATE):
- * transforming Gauss points take the network's transformation update when
- * the relative consistency residual of the reference's Newton system is
- * <= tol and the inputs are in the trained range; otherwise the exact return
- * map runs (counted as SMA_FLAG_SURR_FALLBACK).  NULL detaches. */
+ * single-branch transforming Gauss points take the network's transformation
+ * update x when it lies within tol (in xS) of the return map's solution --
+ * |R(x)/R'(x)| of the reference's Newton residual, exact because that
+ * residual is linear in the increment on one branch -- and the inputs are
+ * in the trained range; otherwise the exact return map runs (counted as
+ * SMA_FLAG_SURR_FALLBACK).  NULL detaches. */
 int sld_dev_set_surrogate(sld_dev *dev, sma_mlp *mlp, double tol);
 
 /* ------------------------------------------------------------------------

@@ -126,3 +126,76 @@ def test_training_lowers_loss():
     print(f"training: loss {l0:.3e} -> {l1:.3e}", flush=True)
     assert l1 < 0.7 * l0
     s.close()
+
+
+def test_surrogate_in_step_gain():
+    """What the trained network gains in the step (uniaxial bar, config 0):
+    from the FP64 trajectory's state at transforming steps, one step in FP32
+    mode (exact return map) and one in surrogate mode.  Accepted points must
+    lie within tol of the exact xS (the gate is exact: the single-branch
+    residual is linear in the increment), the stress error stays within
+    1e-3, and the trained network replaces more Newton solves than an
+    untrained one.  Times per step are printed, not asserted: the exact
+    return map costs one update here, which the network does not beat."""
+    from alya_sma_amd import problems, surrogate as S
+    from alya_sma_amd.solid import Solidz
+    trained = S.load_default()
+    assert trained is not None, "trained weights missing: python -m alya_sma_amd.surrogate"
+    untrained = S.Surrogate(seed=3)
+    tol = 1e-4
+    prob = problems.bar_problem()
+    dt = prob["run"]["dt"]
+    rel = lambda a, b: float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+    R = Solidz(prob)
+    R.set_store(True)
+    A = Solidz(prob)
+    A.set_mode(1)
+    A.set_store(True)
+    B = {}
+    for name, net in (("trained", trained), ("untrained", untrained)):
+        B[name] = Solidz(prob)
+        B[name].set_surrogate(net, tol)
+        B[name].set_mode(2)
+        B[name].set_store(True)
+    done, acc = 0, {"trained": [0, 0], "untrained": [0, 0]}
+    for target in (300, 350, 400, 450, 500, 1300, 1400, 1500):
+        R.step(dt, nsteps=target - done)
+        done = target
+        st = {f: R.get(f) for f in ("u", "v", "a", "sdv", "time")}
+        R.step(dt, nsteps=1)
+        done += 1
+        SR = R.get("stress")
+        for f, v in st.items():
+            A.set(f, v)
+        A.step(dt, nsteps=1)
+        xa = A.get("sdv")[..., 0]
+        assert rel(A.get("stress"), SR) < 1e-3
+        for name, d in B.items():
+            for f, v in st.items():
+                d.set(f, v)
+            c0 = d.counters()
+            d.step(dt, nsteps=1)
+            c1 = d.counters()
+            fl = d.get("flags")
+            used = (fl & 0x20) != 0
+            acc[name][0] += int(c1["surrogate_used"] - c0["surrogate_used"])
+            acc[name][1] += int(c1["surrogate_fallback"] - c0["surrogate_fallback"] +
+                                c1["surrogate_used"] - c0["surrogate_used"])
+            xb = d.get("sdv")[..., 0]
+            # the surrogate modes form the FP32 strain spectrally (their
+            # features are principal strains), the FP32 mode by series: the
+            # two differ by FP32 rounding (1e-5 in xS bounds it here)
+            assert np.all(np.abs(xb[used] - xa[used]) <= tol + 1e-5)
+            assert np.all(np.abs(xb[~used] - xa[~used]) <= 1e-5)      # exact map elsewhere
+            assert rel(d.get("stress"), SR) < 1e-3
+    ta = A.time_kernels(dt, nsteps=2)["kernel_ms_per_step"]
+    tb = B["trained"].time_kernels(dt, nsteps=2)["kernel_ms_per_step"]
+    print(f"surrogate in the step: trained accepts {acc['trained'][0]}/{acc['trained'][1]} "
+          f"transforming points, untrained {acc['untrained'][0]}/{acc['untrained'][1]}; "
+          f"kernel ms/step exact {ta:.4f}, surrogate {tb:.4f}", flush=True)
+    assert acc["trained"][1] > 0
+    assert acc["trained"][0] > acc["untrained"][0]
+    for d in [R, A] + list(B.values()):
+        d.close()
+    untrained.close()
+    trained.close()
