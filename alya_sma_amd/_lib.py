@@ -32,7 +32,7 @@ class SldMeshDesc(C.Structure):
 
 class SldMaterialDesc(C.Structure):
     _fields_ = [("nmat", C.c_int32), ("props", C.c_void_p), ("rho", C.c_void_p),
-                ("mat_of_elem", C.c_void_p), ("E_of_elem", C.c_void_p)]
+                ("mat_of_elem", C.c_void_p), ("E_of_elem", C.c_void_p), ("law", C.c_void_p)]
 
 
 class SldBcDesc(C.Structure):

@@ -92,6 +92,7 @@ struct sld_dev {
   int32_t* mat_of_elem = nullptr;
   double* E_of_elem = nullptr;
   double* kg = nullptr;              // [nelem][2] bulk, shear modulus
+  uint8_t* law_of_elem = nullptr;    // [nelem] 1 = neo-Hookean; null when all SMA
   double* sdv[2][3] = {};
   int cur = 1;                       // index of the TIME_N history and clock slot
   double* S_out = nullptr;
@@ -274,6 +275,7 @@ int enqueue_step(sld_dev* d, double dt, int scheme, double phi, int tn, KernelTi
     E.mat_of_elem = d->mat_of_elem;
     E.E_of_elem = d->E_of_elem;
     E.kg = d->kg;
+    E.law_of_elem = d->law_of_elem;
     E.S_out = d->S_out;
     E.flags_out = d->flags_out;
     E.counters = d->counters;
@@ -351,7 +353,7 @@ void free_graphs(sld_dev* d) {
 
 double host_critical_dt(const double* coord, const int32_t* lnods, int64_t nelem,
                         const double* props, const double* rho, const int32_t* mat,
-                        const double* Eel) {
+                        const double* Eel, const int32_t* law) {
   // sld_updtss.f90:119-162 with elmlen.f90 (ndime==3, hnatu = 2 for HEX08)
   static const int sg[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
                                {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
@@ -380,11 +382,17 @@ double host_critical_dt(const double* coord, const int32_t* lnods, int64_t nelem
       hmin = std::min(hmin, 2.0 / std::sqrt(tr[i][0] * tr[i][0] + tr[i][1] * tr[i][1] +
                                             tr[i][2] * tr[i][2]));
     const int m = mat ? mat[e] : 0;
-    const double E = Eel ? Eel[e] : props[18 * m];
-    const double nu = props[18 * m + 1];
-    const double lam = nu * E / ((1.0 + nu) * (1.0 - 2.0 * nu));
-    const double mu = E / (2.0 * (1.0 + nu));
-    const double c = std::sqrt((lam + 2.0 * mu) / rho[m]);
+    double c;
+    if (law && law[m] == SLD_LAW_NEOHOOKEAN102) {
+      // sld_velsnd.f90:75-78: sqrt((K0 + 4 mu0 / 3) / rho)
+      c = std::sqrt((props[18 * m + 1] + 4.0 * props[18 * m] / 3.0) / rho[m]);
+    } else {
+      const double E = Eel ? Eel[e] : props[18 * m];
+      const double nu = props[18 * m + 1];
+      const double lam = nu * E / ((1.0 + nu) * (1.0 - 2.0 * nu));
+      const double mu = E / (2.0 * (1.0 + nu));
+      c = std::sqrt((lam + 2.0 * mu) / rho[m]);
+    }
     dtmin = std::min(dtmin, hmin / c);
   }
   return dtmin;
@@ -462,6 +470,13 @@ int sld_dev_create(const sld_mesh_desc* mesh, const sld_material_desc* mat,
     for (int64_t e = 0; e < ne; ++e)
       if (mat->mat_of_elem[e] < 0 || mat->mat_of_elem[e] >= mat->nmat)
         return fail(SMA_ERR_ARG, "sld_dev_create: material index out of range");
+  bool any_neo = false;
+  if (mat->law)
+    for (int k = 0; k < mat->nmat; ++k) {
+      if (mat->law[k] != SLD_LAW_SMA160 && mat->law[k] != SLD_LAW_NEOHOOKEAN102)
+        return fail(SMA_ERR_ARG, "sld_dev_create: law must be SLD_LAW_SMA160 or SLD_LAW_NEOHOOKEAN102");
+      any_neo |= mat->law[k] == SLD_LAW_NEOHOOKEAN102;
+    }
   if (bc->nfunc < 0 || (bc->nfunc > 0 && (!bc->func_table || !bc->func_npts || bc->func_maxpts < 1)))
     return fail(SMA_ERR_ARG, "sld_dev_create: bad load-function tables");
   for (int f = 0; f < bc->nfunc; ++f)
@@ -592,6 +607,12 @@ int sld_dev_create(const sld_mesh_desc* mesh, const sld_material_desc* mat,
   const int64_t ngp = ne * 8;
   TRY(dalloc((double**)&d->efor, (size_t)ngp * 3)); track(d, d->efor);
   TRY(dalloc(&d->kg, (size_t)ne * 2)); track(d, d->kg);
+  if (any_neo) {
+    std::vector<uint8_t> le((size_t)ne);
+    for (int64_t i = 0; i < ne; ++i)
+      le[i] = mat->law[mat->mat_of_elem ? mat->mat_of_elem[d->order[i]] : 0] == SLD_LAW_NEOHOOKEAN102;
+    TRY(upload(&d->law_of_elem, le.data(), le.size(), s)); track(d, d->law_of_elem);
+  }
   hipLaunchKernelGGL(sld::sld_elem_moduli, dim3(nblocks(ne, 256)), dim3(256), 0, s, ne, d->props,
                      d->mat_of_elem, d->E_of_elem, d->kg);
   for (int sl = 0; sl < 2; ++sl)
@@ -626,7 +647,7 @@ int sld_dev_create(const sld_mesh_desc* mesh, const sld_material_desc* mat,
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(s));
   d->dtcrit = host_critical_dt(mesh->coord, mesh->lnods, ne, mat->props, mat->rho,
-                               mat->mat_of_elem, mat->E_of_elem);
+                               mat->mat_of_elem, mat->E_of_elem, mat->law);
 #undef TRY
   *out = d;
   return SMA_OK;

@@ -240,6 +240,7 @@ struct ElemArgs {
   const int32_t* mat_of_elem;   // [nelem] or null
   const double* E_of_elem;      // [nelem] or null
   const double* kg;             // [nelem][2] bulk and shear modulus per element
+  const uint8_t* law_of_elem;   // [nelem] 1 = neo-Hookean (model 102), 0 = SMA; null = all SMA
   const double* sdv_n[3];       // TIME_N history, SoA over GP = 8*e + g
   double* sdv_k[3];             // ITER_K history
   void* fint[3];                // scatter: nodal internal force (double or float)
@@ -406,7 +407,19 @@ __global__ void __launch_bounds__(EPB * 8, ElemOccupancy<R>::waves) sld_element(
     R tau[6];
     R Fm[9];
     sma::ReturnMap rm;
-    if constexpr (sizeof(R) == 8) {
+    const bool neo = A.law_of_elem && A.law_of_elem[e];
+    if (neo) {
+      // neo-Hookean element: no history (TIME_N carries over), no flags
+      R jm1;
+      sma::neo_hookean_tau<R>(pr[0], pr[1], H, tau, jm1);
+      rm.xS = xS0;
+      rm.F_AS = FAS0;
+      rm.F_SA = FSA0;
+      rm.flags = 0;
+#pragma unroll
+      for (int q = 0; q < 9; ++q) Fm[q] = H[q] + ((q % 4 == 0) ? R(1) : R(0));
+      if constexpr (SUR == 1) A.sur_status[gp] = 0;
+    } else if constexpr (sizeof(R) == 8) {
       double F[9];
 #pragma unroll
       for (int q = 0; q < 9; ++q) F[q] = H[q] + ((q % 4 == 0) ? 1.0 : 0.0);

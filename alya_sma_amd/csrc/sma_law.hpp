@@ -410,6 +410,43 @@ __host__ __device__ __forceinline__ void kirchhoff_dev(const Mat& m, const Henck
   t[5] = cvol + coef * h.ed[5];
 }
 
+// Neo-Hookean (Ansys form) Kirchhoff stress, sld_stress_model_102.f90:116-119
+// pushed forward: tau = F S F^T with S = ((-mu/3) J^(-2/3) tr C + J K (J-1))
+// C^-1 + mu J^(-2/3) I gives tau = mu J^(-2/3) dev(b) + J K (J-1) I.  Formed
+// from H = F - I: dev(b) = dev(b - I) and J - 1 = tr H + I2(H) + det H, so
+// small strains lose nothing to cancellation (in FP32 as in FP64).
+template <typename R>
+__host__ __device__ __forceinline__ void neo_hookean_tau(double mu, double K, const R (&H)[9],
+                                                        R (&t)[6], R& jm1_out) {
+  R c[6];   // b - I = H + H^T + H H^T
+  c[0] = R(2) * H[0] + (H[0] * H[0] + H[3] * H[3] + H[6] * H[6]);
+  c[3] = R(2) * H[4] + (H[1] * H[1] + H[4] * H[4] + H[7] * H[7]);
+  c[5] = R(2) * H[8] + (H[2] * H[2] + H[5] * H[5] + H[8] * H[8]);
+  c[1] = H[3] + H[1] + (H[0] * H[1] + H[3] * H[4] + H[6] * H[7]);
+  c[2] = H[6] + H[2] + (H[0] * H[2] + H[3] * H[5] + H[6] * H[8]);
+  c[4] = H[7] + H[5] + (H[1] * H[2] + H[4] * H[5] + H[7] * H[8]);
+  // det(I + H) - 1 (H column-major: H(i,k) = H[i + 3k])
+  const R trH = H[0] + H[4] + H[8];
+  const R i2 = (H[0] * H[4] - H[3] * H[1]) + (H[0] * H[8] - H[6] * H[2]) + (H[4] * H[8] - H[7] * H[5]);
+  const R dH = H[0] * (H[4] * H[8] - H[7] * H[5]) - H[3] * (H[1] * H[8] - H[7] * H[2]) +
+               H[6] * (H[1] * H[5] - H[4] * H[2]);
+  const R jm1 = trH + i2 + dH;
+  const R J = R(1) + jm1;
+  R jm23;
+  if constexpr (sizeof(R) == 8) jm23 = pow(J, -2.0 / 3.0);
+  else jm23 = expf((-2.0f / 3.0f) * log1pf(jm1));
+  const R tr3 = (c[0] + c[3] + c[5]) * (R(1) / R(3));
+  const R m = (R)mu * jm23;
+  const R p = J * (R)K * jm1;
+  t[0] = m * (c[0] - tr3) + p;
+  t[3] = m * (c[3] - tr3) + p;
+  t[5] = m * (c[5] - tr3) + p;
+  t[1] = m * c[1];
+  t[2] = m * c[2];
+  t[4] = m * c[4];
+  jm1_out = jm1;
+}
+
 // eval_loading_function :513-537 (beta = 1 per :554, temperature inert).
 __host__ __device__ __forceinline__ double loading(const Mat& m, double C, double theta,
                                           double enor, double xS) {

@@ -22,7 +22,7 @@ module mod_sld_dev
 
   type, bind(C) :: sld_material_desc
      integer(c_int32_t) :: nmat
-     type(c_ptr)        :: props, rho, mat_of_elem, E_of_elem
+     type(c_ptr)        :: props, rho, mat_of_elem, E_of_elem, law
   end type sld_material_desc
 
   type, bind(C) :: sld_bc_desc

@@ -74,7 +74,7 @@ program sld_bar_driver
   smooth(1) = 1
 
   mesh = sld_mesh_desc(int(npoin, c_int64_t), int(nelem, c_int64_t), c_loc(lnods), c_loc(coord))
-  mat = sld_material_desc(1_c_int32_t, c_loc(props), c_loc(rho), c_null_ptr, c_null_ptr)
+  mat = sld_material_desc(1_c_int32_t, c_loc(props), c_loc(rho), c_null_ptr, c_null_ptr, c_null_ptr)
   bc = sld_bc_desc(c_loc(fixmask), c_loc(func_of_node), 1_c_int32_t, 3_c_int32_t, c_loc(npts), &
                    c_loc(smooth), c_loc(table), c_null_ptr, -1_c_int32_t)
   call sld_dev_check(sld_dev_create(mesh, mat, bc, dev), 'sld_dev_create')

@@ -219,3 +219,72 @@ def batched_specimens(n=256, seed=3, median=8000, sigma=0.7, lo=2000, hi=60000, 
     prob["name"] = (f"{n} SMA specimens, {prob['lnods'].shape[0]} hex8 in total "
                     f"({int(prob['specimens']['elements'].min())}-{int(prob['specimens']['elements'].max())} each)")
     return prob
+
+
+# Tissue of config 4: neo-Hookean (Ansys form) mu = 0.03 MPa, K = 1 MPa,
+# density 1000 kg/m^3 (props[0] = mu, props[1] = K as parco_sld for NEOHO ANSYS).
+TISSUE = np.array([0.03, 1.0] + [0.0] * 16, dtype=np.float64)
+RHO_TISSUE = 1.0e-9
+LAW_SMA, LAW_NEOHOOKEAN = 160, 102
+
+
+def cylinder_problem(ntheta=360, nz=400, nr_tube=3, nr_tissue=39, r_in=4.0, r_tube=4.3,
+                     r_out=10.0, length=40.0, crimp=0.4, nsteps=10000, safety=0.5,
+                     props=NITINOL, rho=RHO_NITINOL):
+    """Config 4: SMA tube r in [r_in, r_tube] inside neo-Hookean tissue
+    r in [r_tube, r_out], length `length`, structured hex8 in (r, theta, z)
+    (nr_tube + nr_tissue) x ntheta x nz elements (default 42 x 360 x 400 =
+    6.05 M), closed in theta.  The inner surface r = r_in is crimped by the
+    prescribed radial displacement -crimp (mm) and released (DISCRETE,SMOOTH
+    0 -> 1 -> 0 over nsteps); u_z = 0 on z = 0.  One load function per
+    circumferential station carries that station's direction (cos, sin)."""
+    nr = nr_tube + nr_tissue
+    r = np.concatenate([np.linspace(r_in, r_tube, nr_tube + 1),
+                        np.linspace(r_tube, r_out, nr_tissue + 1)[1:]])
+    th = 2.0 * np.pi * np.arange(ntheta) / ntheta
+    z = np.linspace(0.0, length, nz + 1)
+    ir, it, iz = np.meshgrid(np.arange(nr + 1), np.arange(ntheta), np.arange(nz + 1), indexing="ij")
+    node = lambda a, b, c: a + (nr + 1) * ((b % ntheta) + ntheta * c)
+    npoin = (nr + 1) * ntheta * (nz + 1)
+    coord = np.zeros((npoin, 3))
+    p = node(ir, it, iz).ravel()
+    coord[p, 0] = (r[ir] * np.cos(th[it])).ravel()
+    coord[p, 1] = (r[ir] * np.sin(th[it])).ravel()
+    coord[p, 2] = z[iz].ravel()
+    # elements numbered like the nodes (r fastest, then theta, then z), so that
+    # consecutive elements share nodes that sit close in memory
+    er, et, ez = [x.ravel(order="F") for x in np.meshgrid(np.arange(nr), np.arange(ntheta),
+                                                          np.arange(nz), indexing="ij")]
+    # reference node order with (xi, eta, zeta) = (r, theta, z): positive volume
+    lnods = np.stack([node(er, et, ez), node(er + 1, et, ez), node(er + 1, et + 1, ez),
+                      node(er, et + 1, ez), node(er, et, ez + 1), node(er + 1, et, ez + 1),
+                      node(er + 1, et + 1, ez + 1), node(er, et + 1, ez + 1)], axis=1).astype(np.int32)
+    mat_of_elem = (er >= nr_tube).astype(np.int32)          # 0 = SMA tube, 1 = tissue
+    fix = np.zeros(npoin, dtype=np.uint8)
+    fix[np.abs(coord[:, 2]) < 1e-9 * length] |= 4
+    inner = np.zeros(npoin, dtype=bool)
+    inner[node(np.zeros_like(it[0]), it[0], iz[0]).ravel()] = True
+    fix[inner] |= 3
+    func_of_node = np.full(npoin, -1, dtype=np.int32)
+    stat = np.zeros(npoin, dtype=np.int64)
+    stat[p] = it.ravel()
+    func_of_node[inner] = stat[inner]
+    P = np.stack([np.asarray(props, np.float64).reshape(18), TISSUE])
+    rhos = np.array([rho, RHO_TISSUE])
+    # critical step: the SMA tube's smallest edge at the SMA wave speed (the
+    # tissue's speed is ~100x lower)
+    E, nu = P[0, 0], P[0, 1]
+    c_sma = np.sqrt((nu * E / ((1 + nu) * (1 - 2 * nu)) + E / (1 + nu)) / rho)
+    h = min((r_tube - r_in) / nr_tube, 2 * r_in * np.sin(np.pi / ntheta), length / nz)
+    dt = safety * h / c_sma
+    t_end = nsteps * dt
+    tabs = np.stack([smooth_table(t_end, [[0, 0, 0], [-crimp * np.cos(a), -crimp * np.sin(a), 0],
+                                          [0, 0, 0]]) for a in th])
+    prob = dict(coord=coord, lnods=lnods, props=P, rho=rhos, mat_of_elem=mat_of_elem,
+                law_of_mat=np.array([LAW_SMA, LAW_NEOHOOKEAN], np.int32), fixmask=fix,
+                func_of_node=func_of_node, func_table=tabs, func_npts=np.full(ntheta, 3, np.int32),
+                func_smooth=np.ones(ntheta, np.int32), fext0=None, fext_func=-1)
+    prob["run"] = dict(dt=dt, nsteps=nsteps, safety=safety, scheme=0, phi=1.033)
+    prob["name"] = (f"SMA tube in neo-Hookean tissue, {nr}x{ntheta}x{nz} = {lnods.shape[0]} hex8 "
+                    f"(tube {nr_tube} layers, r {r_in}-{r_tube}-{r_out} mm, L {length} mm)")
+    return prob

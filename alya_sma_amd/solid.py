@@ -81,7 +81,8 @@ class Solidz:
         props = arr(np.atleast_2d(prob["props"]), np.float64)
         matd = _lib.SldMaterialDesc(props.shape[0], _p(props), _p(arr(prob["rho"], np.float64)),
                                     _p(arr(prob.get("mat_of_elem"), np.int32)),
-                                    _p(arr(prob.get("E_of_elem"), np.float64)))
+                                    _p(arr(prob.get("E_of_elem"), np.float64)),
+                                    _p(arr(prob.get("law_of_mat"), np.int32)))
         ftab = arr(prob["func_table"], np.float64)
         bc = _lib.SldBcDesc(_p(arr(prob["fixmask"], np.uint8)),
                             _p(arr(prob.get("func_of_node"), np.int32)),

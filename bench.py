@@ -9,7 +9,8 @@ the timed region; one step = predictor, element kernel(s), corrector.
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--workload cantilever|specimens]
 
 `--workload specimens` runs the batch of 256 uneven SMA specimens (2.7 M
-hex8 in one mesh, 2000 steps) instead of the cantilever.
+hex8 in one mesh, 2000 steps) instead of the cantilever; `--workload
+cylinder` the SMA tube in neo-Hookean tissue (6.05 M hex8, 10000 steps).
 
 Prints one JSON line.  `value` is the throughput of the fastest precision
 mode whose stress error stays within 1e-3.  The error of a mode is the
@@ -50,7 +51,7 @@ def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed steps (default: the workload's run)")
-    ap.add_argument("--workload", choices=["cantilever", "specimens"], default="cantilever")
+    ap.add_argument("--workload", choices=["cantilever", "specimens", "cylinder"], default="cantilever")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--modes", default="fp64,fp32,surrogate")
     ap.add_argument("--sur-tol", type=float, default=1e-4,
@@ -239,6 +240,9 @@ def main():
         prob = problems.cantilever_problem()
         workload = ("SMA cantilever hex8 400x20x20 (160000 elements, 1280000 Gauss points), "
                     "50x2.5x2.5 mm, tip traction load-unload, E +-2% noise, central difference")
+    elif args.workload == "cylinder":
+        prob = problems.cylinder_problem()
+        workload = prob["name"] + ", radial crimp and release of the inner surface, central difference"
     else:
         prob = problems.batched_specimens()
         workload = (prob["name"] + ", lognormal sizes (median 8000, sigma 0.7, 2k-60k), aspect "

@@ -90,8 +90,14 @@ typedef struct {
   const double *props;         /* [nmat][18] parco_sld(1:18, imate) */
   const double *rho;           /* [nmat] densi_sld(1, imate) */
   const int32_t *mat_of_elem;  /* [nelem] lmate_sld, or NULL */
-  const double *E_of_elem;     /* [nelem] per-element Young's modulus, or NULL */
+  const double *E_of_elem;     /* [nelem] per-element Young's modulus (SMA), or NULL */
+  const int32_t *law;          /* [nmat] lawst_sld: SLD_LAW_SMA160 or SLD_LAW_NEOHOOKEAN102;
+                                  NULL = all SMA.  Neo-Hookean rows: props[0] = mu,
+                                  props[1] = K (NEOHO ANSYS, sld_stress_model_102.f90:60-61) */
 } sld_material_desc;
+
+#define SLD_LAW_SMA160         160
+#define SLD_LAW_NEOHOOKEAN102  102
 
 typedef struct {
   const uint8_t *fixmask;      /* [npoin] bit d = kfl_fixno_sld(d+1, ipoin) == 1 */

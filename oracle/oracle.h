@@ -98,7 +98,12 @@ typedef struct {
     const int32_t *elem_order; /* [nelem] processing order or NULL           */
     const int32_t *color_ptr;  /* [ncolor+1] ranges into elem_order or NULL  */
     int32_t ncolor;
+    const int32_t *law_of_mat; /* [nmat] 160 SMA, 102 neo-Hookean; NULL = 160 */
 } orc_model;
+
+/* Neo-Hookean (Ansys) PK2 at one point: props[0] = mu, props[1] = K,
+ * F column-major, J = det F (sld_stress_model_102.f90:60-119). */
+void orc_nh102_point(const double props[18], const double F[9], double J, double S[9]);
 
 typedef struct {
     double *u, *v, *a;         /* [npoin][3] (ITER_K after a step)           */

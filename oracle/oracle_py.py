@@ -74,6 +74,18 @@ def sm160_eval(props, F, sdv_old, mat_of_gp=None, E_of_gp=None):
     return sdv_new, S, flags
 
 
+def nh102_point(props, F, J=None):
+    """Neo-Hookean (Ansys) PK2 at one point (sld_stress_model_102.f90):
+    props[0] = mu, props[1] = K; F (3, 3) row-major; J defaults to det F."""
+    F = np.asarray(F, dtype=np.float64)
+    p = np.zeros(18)
+    p[:len(np.ravel(props))] = np.ravel(props)
+    Fc = np.ascontiguousarray(F.T).ravel()
+    S = np.zeros(9)
+    lib().orc_nh102_point(_ptr(p), _ptr(Fc), C.c_double(np.linalg.det(F) if J is None else J), _ptr(S))
+    return S.reshape(3, 3).T.copy()
+
+
 def sm160_point(props, F, sdv_old):
     """One point, F as a 3x3 array.  Returns (sdv_new, S 3x3, tau 3x3, flags)."""
     props = np.ascontiguousarray(props, dtype=np.float64)
@@ -122,7 +134,8 @@ class _Model(C.Structure):
                 ("nfunc", C.c_int32), ("func_npts", C.c_void_p), ("func_smooth", C.c_void_p),
                 ("func_table", C.c_void_p), ("func_maxpts", C.c_int32),
                 ("fext0", C.c_void_p), ("fext_func", C.c_int32),
-                ("elem_order", C.c_void_p), ("color_ptr", C.c_void_p), ("ncolor", C.c_int32)]
+                ("elem_order", C.c_void_p), ("color_ptr", C.c_void_p), ("ncolor", C.c_int32),
+                ("law_of_mat", C.c_void_p)]
 
 
 class _State(C.Structure):
@@ -172,6 +185,7 @@ class StepOracle:
         m.elem_order = _ptr(arr(elem_order, np.int32))
         m.color_ptr = _ptr(arr(color_ptr, np.int32))
         m.ncolor = 0 if color_ptr is None else len(color_ptr) - 1
+        m.law_of_mat = _ptr(arr(prob.get("law_of_mat"), np.int32))
         self.m = m
         n, e = self.npoin, self.nelem
         self.u = np.zeros((n, 3)); self.v = np.zeros((n, 3)); self.a = np.zeros((n, 3))

@@ -77,6 +77,7 @@ double orc_sld_critical_dt(const orc_model *m)
         const double *pr;
         int32_t mat = m->mat_of_elem ? m->mat_of_elem[e] : 0;
         double E = elem_E(m, e, &pr), nu = pr[1];
+        const int law = m->law_of_mat ? m->law_of_mat[mat] : 160;
         gather(m, m->coord, e, elcod);
         for (int k = 0; k < 8; k++)
             for (int a = 0; a < 3; a++)
@@ -99,6 +100,8 @@ double orc_sld_critical_dt(const orc_model *m)
         double lam = nu * E / ((1.0 + nu) * (1.0 - 2.0 * nu));
         double mu = E / (2.0 * (1.0 + nu));
         double c = sqrt((lam + 2.0 * mu) / m->rho[mat]);
+        if (law == 102)   /* sld_velsnd.f90:75-78: sqrt((K0 + 4 mu0/3)/rho) */
+            c = sqrt((pr[1] + 4.0 * pr[0] / 3.0) / m->rho[mat]);
         double dt = hmin / c;
         if (dt < dtmin) dtmin = dt;
     }
@@ -148,8 +151,10 @@ static void element_loop(const orc_model *m, orc_state *s)
             double props[18];
             const double *pr;
             double E = elem_E(m, e, &pr);
+            const int32_t mat = m->mat_of_elem ? m->mat_of_elem[e] : 0;
+            const int law = m->law_of_mat ? m->law_of_mat[mat] : 160;
             memcpy(props, pr, sizeof(props));
-            props[0] = E;
+            if (law == 160) props[0] = E;
             gather(m, m->coord, e, elcod);
             gather(m, s->u, e, eldis);
             orc_hex8_kinematics(elcod, eldis, gpcar, gpvol, F, detF);
@@ -157,7 +162,14 @@ static void element_loop(const orc_model *m, orc_state *s)
                 double S[9], P[9];
                 const double *Fg = F + 9 * g;
                 double *sk = sdv_k + 24 * e + 3 * g, *sn = sdv_n + 24 * e + 3 * g;
-                uint32_t fl = orc_sm160_point(props, Fg, sn, sk, S, 0);
+                uint32_t fl = 0;
+                if (law == 102) {
+                    /* no history: TIME_N carries over to ITER_K unchanged */
+                    orc_nh102_point(props, Fg, detF[g], S);
+                    for (int q = 0; q < 3; q++) sk[q] = sn[q];
+                } else {
+                    fl = orc_sm160_point(props, Fg, sn, sk, S, 0);
+                }
                 if (!(detF[g] > 0.0)) fl |= ORC_FLAG_DETF_NONPOS;
                 if (s->flags) s->flags[8 * e + g] = fl;
                 if (s->S) memcpy(s->S + 72 * e + 9 * g, S, sizeof(S));

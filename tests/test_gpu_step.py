@@ -279,3 +279,43 @@ def test_batched_specimens_persistent_launch_bitwise():
         b.step(dt, nsteps=120)
         for f in ("u", "v", "sdv"):
             assert np.array_equal(a.get(f), b.get(f)), f
+
+
+def test_tube_in_tissue_two_materials_match_oracle():
+    """Config 4 in small: SMA tube (model 160) in neo-Hookean tissue (model
+    102), inner surface crimped radially, against the oracle (restarted from
+    its state every 25 steps), plus the critical step of the two materials."""
+    prob = problems.cylinder_problem(ntheta=16, nz=6, nr_tube=2, nr_tissue=3, nsteps=400)
+    dt = prob["run"]["dt"]
+    print("tube in tissue:", prob["name"], flush=True)
+    dev, orc = _pair(prob)
+    with dev:
+        assert abs(dev.critical_dt() - orc.critical_dt()) <= 1e-13 * orc.critical_dt()
+        for chunk in range(16):
+            if chunk:
+                _sync_from_oracle(dev, orc)
+            dev.step(dt, nsteps=25)
+            orc.step(dt, n=25)
+            _compare(dev, orc, f"tube step {25 * (chunk + 1)}", quiet=chunk % 8 != 7)
+        xs = dev.get("sdv")[..., 0]
+        mat = prob["mat_of_elem"]
+        assert xs[mat == 0].max() > 0          # the tube transforms under the crimp
+        assert np.all(xs[mat == 1] == 0)       # the tissue has no history
+
+
+def test_tube_in_tissue_fp32_one_step_error():
+    prob = problems.cylinder_problem(ntheta=24, nz=8, nr_tube=2, nr_tissue=4, nsteps=400)
+    dt = prob["run"]["dt"]
+    with Solidz(prob) as ref, Solidz(prob) as f32:
+        ref.set_store(True)
+        f32.set_mode(1)
+        f32.set_store(True)
+        ref.step(dt, nsteps=199)
+        for k in ("u", "v", "a", "sdv", "time"):
+            f32.set(k, ref.get(k))
+        ref.step(dt, nsteps=1)
+        f32.step(dt, nsteps=1)
+        S, Sr = f32.get("stress"), ref.get("stress")
+        err = np.linalg.norm(S - Sr) / np.linalg.norm(Sr)
+        print(f"tube in tissue, fp32 one-step stress error {err:.2e}", flush=True)
+        assert err < 1e-3

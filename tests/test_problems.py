@@ -35,3 +35,22 @@ def test_batched_specimens_are_disjoint_with_own_material_and_load():
     E, nu, rho = p["props"][:, 0], p["props"][:, 1], p["rho"]
     c = np.sqrt((nu * E / ((1 + nu) * (1 - 2 * nu)) + E / (1 + nu)) / rho)
     assert np.isclose(p["run"]["dt"], 0.5 * np.min(h / c), rtol=1e-12)
+
+
+def test_cylinder_problem_two_materials():
+    p = problems.cylinder_problem(ntheta=12, nz=3, nr_tube=2, nr_tissue=3, nsteps=100)
+    ln, mat = p["lnods"], p["mat_of_elem"]
+    assert ln.shape[0] == 5 * 12 * 3
+    assert np.array_equal(p["law_of_mat"], [160, 102])
+    # tube elements lie inside r = 4.3, tissue elements outside
+    rc = np.linalg.norm(p["coord"][ln][:, :, :2].mean(axis=1), axis=1)
+    assert np.all(rc[mat == 0] < 4.3) and np.all(rc[mat == 1] > 4.3)
+    # the inner surface follows one radial function per station, u_z = 0 on z = 0
+    r = np.linalg.norm(p["coord"][:, :2], axis=1)
+    inner = np.abs(r - 4.0) < 1e-9
+    assert np.all(p["fixmask"][inner] & 3 == 3) and np.all(p["func_of_node"][inner] >= 0)
+    f = p["func_of_node"][inner]
+    d = p["func_table"][f, 1, :2]                       # crimp direction at the peak
+    x = p["coord"][inner, :2] / 4.0
+    np.testing.assert_allclose(d, -0.4 * x, atol=1e-12)
+    assert np.all(p["fixmask"][np.abs(p["coord"][:, 2]) < 1e-9] & 4)
