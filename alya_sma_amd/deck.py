@@ -1,0 +1,321 @@
+"""Alya input decks for the explicit SMA step.
+
+Reads a solidz case -- `<name>.dat`, `<name>.dom.dat` (with its INCLUDEd
+`.geo.dat`, `.set.dat`, `.fix.dat`) and `<name>.sld.dat` -- into the
+problem dict of alya_sma_amd.problems, so that a case such as the
+reference's tests/uniaxial_test runs as written.  The subset read is the
+one this path uses:
+
+  .dat       TIME_INTERVAL (t0, t1), NUMBER_OF_STEPS, TIME_COUPLING
+             FROM_CRITICAL
+  .geo.dat   ELEMENTS (HEX08), COORDINATES, BOUNDARIES (face nodes + element),
+             MATERIALS
+  .fix.dat   ON_BOUNDARIES (boundary -> code)
+  .set.dat   NODE (node -> node set)
+  .sld.dat   PROPERTIES: MATERIAL, DENSITY, CONSTITUTIVE_MODEL NITINOL
+             (sld_reaphy.f90: parco(1:17)) or NEOHO ANSYS mu K
+             (lawst 102); NUMERICAL_TREATMENT: TIME_INTEGRATION
+             (TCHAMWA_WIELGOSZ with PHITW, or central difference),
+             SAFETY_FACTOR; BOUNDARY_CONDITIONS CODES, NODES rules
+             ("c1 & c2 ... fixity values [, MODULE_FUNCTION= f]") and
+             FUNCTIONS (DISCRETE[, SMOOTH] shape tables, sld_funbou.f90)
+
+A node's codes are the codes of the boundaries whose faces contain it
+(Alya's boundary-condition extrapolation to nodes); a CODES, NODES rule
+applies to the nodes whose code set equals the rule's.  Fixity digits are
+per dof (x y z); a prescribed dof takes the function's table column, or
+the given value without a function.
+
+write_node_sets() writes per-step node-set histories (DISPX..Z and the
+reactions FRXIX..Z on prescribed dofs) as text columns, in the spirit of
+Alya's node-set output (sld_outset.f90 / sld_nodset).
+"""
+import os
+import re
+
+import numpy as np
+
+
+def _lines(path, seen=None):
+    """Statements of an Alya file: '$' comments removed, '\\' continuations
+    joined, INCLUDE files expanded (relative to the including file)."""
+    seen = set() if seen is None else seen
+    path = os.path.abspath(path)
+    if path in seen:
+        raise ValueError(f"INCLUDE cycle at {path}")
+    seen.add(path)
+    out, acc = [], ""
+    with open(path) as f:
+        for raw in f:
+            line = raw.split("$", 1)[0].rstrip()
+            if line.endswith("\\"):
+                acc += line[:-1] + " "
+                continue
+            line = (acc + line).strip()
+            acc = ""
+            if not line:
+                continue
+            m = re.match(r"INCLUDE\s+(\S+)", line, re.I)
+            if m:
+                inc = os.path.join(os.path.dirname(path), m.group(1))
+                if os.path.exists(inc):
+                    out.extend(_lines(inc, seen))
+                continue
+            out.append(line)
+    return out
+
+
+def _block(lines, start, end):
+    """Statements strictly between the first line starting with `start`
+    and the next line starting with `end` (case-insensitive)."""
+    up = [l.upper() for l in lines]
+    for i, l in enumerate(up):
+        # a section header, not a "KEY = value" line of the same name
+        if l.startswith(start) and "=" not in l and (len(l) == len(start) or not l[len(start)].isalnum()
+                                                     and l[len(start)] != "_"):
+            for j in range(i + 1, len(lines)):
+                if up[j].startswith(end):
+                    return lines[i + 1:j]
+            return lines[i + 1:]
+    return []
+
+
+def _num(s):
+    return float(s.replace("D", "E").replace("d", "e"))
+
+
+def read_case(dat_path):
+    """Problem dict (alya_sma_amd.problems schema) of the case whose main
+    file is `dat_path` (…/<name>.dat), plus 'run', 'node_sets' and 'name'."""
+    base = dat_path[:-4] if dat_path.endswith(".dat") else dat_path
+    name = os.path.basename(base)
+    main = _lines(base + ".dat")
+    dom = _lines(base + ".dom.dat")
+    sld = _lines(base + ".sld.dat")
+
+    # ---- time (.dat)
+    t0, t1, nmax = 0.0, None, None
+    for l in main:
+        m = re.match(r"TIME_INTERVAL\s*[:=]\s*(\S+)\s*,\s*(\S+)", l, re.I)
+        if m:
+            t0, t1 = _num(m.group(1)), _num(m.group(2))
+        m = re.match(r"NUMBER_OF_STEPS\s*=\s*(\S+)", l, re.I)
+        if m:
+            nmax = int(_num(m.group(1)))
+
+    # ---- mesh (.geo.dat)
+    elems = {}
+    for l in _block(dom, "ELEMENTS", "END_ELEMENTS"):
+        v = l.split()
+        if len(v) >= 9:
+            elems[int(v[0])] = [int(x) for x in v[1:9]]
+    coords = {}
+    for l in _block(dom, "COORDINATES", "END_COORDINATES"):
+        v = l.split()
+        coords[int(v[0])] = [_num(x) for x in v[1:4]]
+    bounds = {}
+    for l in _block(dom, "BOUNDARIES", "END_BOUNDARIES"):
+        v = l.split()
+        bounds[int(v[0])] = [int(x) for x in v[1:5]]          # quad face nodes (element id follows)
+    mats = {}
+    for l in _block(dom, "MATERIALS", "END_MATERIALS"):
+        v = l.split()
+        mats[int(v[0])] = int(v[1])
+    if not elems or not coords:
+        raise ValueError(f"{name}: no ELEMENTS/COORDINATES read")
+    node_ids = sorted(coords)
+    nidx = {n: i for i, n in enumerate(node_ids)}
+    elem_ids = sorted(elems)
+    coord = np.array([coords[n] for n in node_ids], dtype=np.float64)
+    lnods = np.array([[nidx[n] for n in elems[e]] for e in elem_ids], dtype=np.int32)
+
+    # ---- boundary codes (.fix.dat) -> node code sets
+    bcode = {}
+    for l in _block(dom, "ON_BOUNDARIES", "END_ON_BOUNDARIES"):
+        v = l.split()
+        bcode[int(v[0])] = int(v[1])
+    node_codes = [set() for _ in node_ids]
+    for b, nodes in bounds.items():
+        if b in bcode:
+            for n in nodes:
+                node_codes[nidx[n]].add(bcode[b])
+
+    # ---- node sets (.set.dat)
+    node_sets = {}
+    for l in _block(dom, "NODE", "END_NODE"):
+        v = l.split()
+        if len(v) >= 2 and v[0].isdigit():
+            node_sets.setdefault(int(v[1]), []).append(nidx[int(v[0])])
+
+    # ---- materials (.sld.dat)
+    props, rho, law = {}, {}, {}
+    cur = 1
+    for l in _block(sld, "PROPERTIES", "END_PROPERTIES"):
+        m = re.match(r"MATERIAL\s*=\s*(\d+)", l, re.I)
+        if m:
+            cur = int(m.group(1))
+        m = re.match(r"DENSITY\s*=\s*(\S+)", l, re.I)
+        if m:
+            rho[cur] = _num(m.group(1))
+        m = re.match(r"CONSTITUTIVE_MODEL\s*:\s*(\w+)(.*)", l, re.I)
+        if m:
+            model, rest = m.group(1).upper(), m.group(2).split()
+            p = np.zeros(18)
+            if model.startswith("NITIN"):
+                vals = [_num(x) for x in rest]
+                p[:len(vals)] = vals                           # parco(1:17) in deck order
+                law[cur] = 160
+            elif model.startswith("NEOHO") and rest and rest[0].upper().startswith("ANSYS"):
+                vals = [_num(x) for x in rest[1:]]
+                p[:len(vals)] = vals                           # mu, K
+                law[cur] = 102
+            else:
+                raise ValueError(f"{name}: constitutive model {model} {' '.join(rest[:1])} not on this path")
+            props[cur] = p
+    mat_ids = sorted(props)
+    midx = {m: i for i, m in enumerate(mat_ids)}
+    mat_of_elem = np.array([midx.get(mats.get(e, mat_ids[0]), 0) for e in elem_ids], dtype=np.int32)
+
+    # ---- integrator (.sld.dat NUMERICAL_TREATMENT)
+    scheme, phi, safety = 0, 1.033, 1.0
+    for l in _block(sld, "NUMERICAL_TREATMENT", "END_NUMERICAL_TREATMENT"):
+        m = re.match(r"TIME_INTEGRATION\s*:\s*(\w+)(.*)", l, re.I)
+        if m:
+            if m.group(1).upper().startswith("TCHAM"):
+                scheme = 1
+                mm = re.search(r"PHITW\s*=\s*(\S+)", m.group(2), re.I)
+                if mm:
+                    phi = _num(mm.group(1))
+            else:
+                scheme = 0
+        m = re.match(r"SAFETY_FACTOR\s*=\s*(\S+)", l, re.I)
+        if m:
+            safety = _num(m.group(1))
+
+    # ---- functions
+    funcs = {}
+    fblock = _block(sld, "FUNCTIONS", "END_FUNCTIONS")
+    i = 0
+    while i < len(fblock):
+        m = re.match(r"FUNCTION_NUMBER\s*:\s*(\d+)", fblock[i], re.I)
+        if m:
+            fid, smooth, rows = int(m.group(1)), 0, []
+            j = i + 1
+            while j < len(fblock) and not fblock[j].upper().startswith("END_CONDITION"):
+                t = fblock[j].upper()
+                if t.startswith("TIME_SHAPE"):
+                    if "DISCRETE" not in t:
+                        raise ValueError(f"{name}: function {fid}: only DISCRETE shapes are on this path")
+                    smooth = 1 if "SMOOTH" in t else 0
+                if t.startswith("SHAPE_DEFINITION"):
+                    npts = int(_num(fblock[j + 1].split()[0]))
+                    for k in range(npts):
+                        v = [_num(x) for x in fblock[j + 2 + k].split()]
+                        rows.append(v[1:4] + [v[0]])          # (v_x, v_y, v_z, t)
+                    j += 1 + npts
+                j += 1
+            funcs[fid] = (np.array(rows), smooth)
+            i = j
+        i += 1
+    fids = sorted(funcs)
+    fidx = {f: k for k, f in enumerate(fids)}
+    maxpts = max([len(funcs[f][0]) for f in fids], default=1)
+    ftab = np.zeros((max(1, len(fids)), maxpts, 4))
+    fnpts = np.ones(max(1, len(fids)), np.int32)
+    fsmooth = np.zeros(max(1, len(fids)), np.int32)
+    for f in fids:
+        rows, sm = funcs[f]
+        ftab[fidx[f], :len(rows)] = rows
+        fnpts[fidx[f]] = len(rows)
+        fsmooth[fidx[f]] = sm
+
+    # ---- CODES, NODES rules -> fixity and load functions
+    fix = np.zeros(len(node_ids), dtype=np.uint8)
+    func_of_node = np.full(len(node_ids), -1, dtype=np.int32)
+    for l in _block(sld, "CODES, NODES", "END_CODES") or _block(sld, "CODES,NODES", "END_CODES"):
+        m = re.match(r"([\d\s&]+?)\s+([01]{3})\s+(.*)$", l)
+        if not m:
+            continue
+        codes = set(int(c) for c in m.group(1).split("&"))
+        digits = m.group(2)
+        mf = re.search(r"MODULE_FUNCTION\s*=\s*(\d+)", m.group(3), re.I)
+        vals = [_num(x) for x in re.split(r"[\s,]+", m.group(3).split(",")[0].strip())[:3]]
+        if any(abs(v) > 0 for v in vals) and not mf:
+            raise ValueError(f"{name}: constant nonzero prescribed values are not on this path")
+        mask = sum(1 << d for d in range(3) if digits[d] == "1")
+        for n, cs in enumerate(node_codes):
+            if cs == codes:
+                fix[n] = mask
+                func_of_node[n] = fidx[int(mf.group(1))] if (mf and mask) else -1
+
+    if t1 is None:
+        raise ValueError(f"{name}: TIME_INTERVAL missing")
+    prob = dict(coord=coord, lnods=lnods, props=np.stack([props[m] for m in mat_ids]),
+                rho=np.array([rho.get(m, 1.0) for m in mat_ids]), mat_of_elem=mat_of_elem,
+                law_of_mat=np.array([law[m] for m in mat_ids], np.int32), fixmask=fix,
+                func_of_node=func_of_node, func_table=ftab, func_npts=fnpts, func_smooth=fsmooth,
+                fext0=None, fext_func=-1)
+    prob["run"] = dict(t0=t0, t_end=t1, max_steps=nmax, scheme=scheme, phi=phi, safety=safety)
+    prob["node_sets"] = {k: np.array(v, np.int64) for k, v in sorted(node_sets.items())}
+    prob["node_codes"] = node_codes
+    prob["node_ids"] = np.array(node_ids)
+    prob["name"] = name
+    return prob
+
+
+def reactions(prob, fint):
+    """Reaction forces on prescribed dofs (f_int on those dofs; no external
+    load acts there), zero elsewhere (sld_elmmat.f90:228-244)."""
+    mask = np.stack([(prob["fixmask"] >> d) & 1 for d in range(3)], axis=1).astype(bool)
+    return np.where(mask, fint, 0.0)
+
+
+def write_node_sets(path, prob, history):
+    """history: list of (step, time, u [npoin,3], fint [npoin,3]).  One row
+    per (step, set): sums over the set's nodes of DISPX..Z / their count
+    (the set's mean displacement) and of the reactions FRXIX..Z."""
+    with open(path, "w") as f:
+        f.write("# node sets of " + prob["name"] + "\n")
+        f.write("# columns: step time set DISPX DISPY DISPZ FRXIX FRXIY FRXIZ\n")
+        for step, t, u, fint in history:
+            r = reactions(prob, fint)
+            for s, nodes in prob["node_sets"].items():
+                d = u[nodes].mean(axis=0)
+                q = r[nodes].sum(axis=0)
+                f.write(f"{step:d} {t:.15e} {s:d} " + " ".join(f"{x:.15e}" for x in (*d, *q)) + "\n")
+
+
+def run_case(dat_path, out_dir=".", every=None, mode=0):
+    """Run a deck on the GPU to its TIME_INTERVAL end with dt = safety x the
+    critical step (TIME_COUPLING FROM_CRITICAL), writing
+    <out_dir>/<name>-nodes.sld.set every `every` steps (default: 200 rows)."""
+    from .solid import Solidz
+    prob = read_case(dat_path)
+    r = prob["run"]
+    with Solidz(prob) as dev:
+        dev.set_mode(mode)
+        dt = r["safety"] * dev.critical_dt()
+        nsteps = int(np.ceil((r["t_end"] - r["t0"]) / dt))
+        if r["max_steps"]:
+            nsteps = min(nsteps, r["max_steps"])
+        every = every or max(1, nsteps // 200)
+        hist, done = [], 0
+        while done < nsteps:
+            n = min(every, nsteps - done)
+            dev.step(dt, nsteps=n, scheme=r["scheme"], phi=r["phi"])
+            done += n
+            hist.append((done, float(dev.get("time")[0]), dev.get("u"), dev.get("fint")))
+    path = os.path.join(out_dir, prob["name"] + "-nodes.sld.set")
+    write_node_sets(path, prob, hist)
+    return path, prob, dt, nsteps
+
+
+if __name__ == "__main__":
+    import argparse
+    ap = argparse.ArgumentParser(description="run an Alya solidz SMA deck on the GPU")
+    ap.add_argument("dat", help="path to <name>.dat")
+    ap.add_argument("--out", default=".")
+    ap.add_argument("--every", type=int, default=None)
+    a = ap.parse_args()
+    p, prob, dt, n = run_case(a.dat, a.out, a.every)
+    print(f"{prob['name']}: {n} steps of {dt:.4e} s -> {p}")

@@ -1,0 +1,54 @@
+"""Alya deck reader on the reference's own uniaxial test deck
+(tests/uniaxial_test/sm154_d1C.*, kept as data under tests/golden/):
+parsed geometry, boundary codes, CODES, NODES rules, functions, material
+line and integrator; then the whole deck run by the CPU oracle shows the
+superelastic cycle its load table drives."""
+import os
+
+import numpy as np
+
+from alya_sma_amd import deck
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CASE = os.path.join(HERE, "golden", "uniaxial_test", "sm154_d1C.dat")
+
+
+def test_deck_parses_the_uniaxial_case():
+    p = deck.read_case(CASE)
+    assert p["coord"].shape == (8, 3) and p["lnods"].tolist() == [[0, 1, 2, 3, 4, 5, 6, 7]]
+    # node codes from the six boundary faces (fix.dat: boundary i has code i)
+    assert p["node_codes"][0] == {1, 3, 6} and p["node_codes"][6] == {2, 4, 5}
+    # CODES, NODES: "1 & 3 & 4 111 ... MODULE_FUNCTION= 1" (node 2) etc.
+    assert p["fixmask"].tolist() == [7, 7, 5, 5, 3, 3, 1, 1]
+    assert p["func_of_node"].tolist() == [-1, 0, 0, -1, -1, 0, 0, -1]
+    tab = p["func_table"][0]
+    np.testing.assert_allclose(tab[:, 3], [0, 5e-3, 1e-2, 1.5e-2, 2e-2])
+    np.testing.assert_allclose(tab[:, 0], [0, 0.1, 0, 0.04, 0])
+    assert p["func_smooth"].tolist() == [1] and p["func_npts"].tolist() == [5]
+    np.testing.assert_allclose(p["props"][0, :17], [1000, 0.3, 0.1, 90, 150, 0, 0, 0, 700, 70, 30,
+                                                    0, 0, 0, 0, 10, 10])
+    assert p["rho"].tolist() == [6.4e-9] and p["law_of_mat"].tolist() == [160]
+    r = p["run"]
+    assert (r["scheme"], r["phi"], r["safety"], r["t_end"]) == (1, 1.033, 0.1, 0.02)
+    assert {k: v.tolist() for k, v in p["node_sets"].items()} == {1: [3], 2: [7]}
+
+
+def test_deck_run_by_the_oracle_cycles_through_transformation():
+    from alya_sma_amd import _lib
+    from oracle import oracle_py as O
+    p = deck.read_case(CASE)
+    o = O.StepOracle(p)
+    dt = p["run"]["safety"] * o.critical_dt()
+    n = int(np.ceil(p["run"]["t_end"] / dt))
+    xs, sx = [], []
+    for k in range(40):
+        o.step(dt, scheme=1, phi=p["run"]["phi"], n=n // 40)
+        xs.append(o.sdv[0, ..., 0].mean())
+        sx.append(o.S[..., 0].mean())
+    xs, sx = np.array(xs), np.array(sx)
+    # E = 1000 MPa, Rs_AS = 90 sqrt(2/3) = 73.5 MPa, Rf_AS = 122.5 MPa: the
+    # first load (10% strain) passes the start of forward transformation but
+    # not its finish; unloading and the 4% reload leave no martensite
+    assert xs[:10].max() > 0.02 and np.argmax(xs[:12] > 0) >= 6
+    assert 73.0 < sx[:12].max() < 122.5
+    assert xs[-1] == 0.0 and abs(sx[-1]) < 1.0
