@@ -75,6 +75,8 @@ def lib():
     L.sld_dev_coloring.argtypes = [p, p, p, i32, p]
     L.sld_dev_set_tile.argtypes = [p, i32, i32, i32]
     L.sld_dev_set_assembly.argtypes = [p, i32]
+    L.sld_dev_set_energy.argtypes = [p, i32]
+    L.sld_dev_energy.argtypes = [p, p]
     L.sld_dev_autotune.argtypes = [p, d, i32, d, i32, p, p]
     L.sld_dev_stream.argtypes = [p]
     L.sld_dev_stream.restype = p

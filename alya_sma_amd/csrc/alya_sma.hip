@@ -45,11 +45,11 @@ unsigned nblocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 struct GraphKey {
   double dt, phi;
-  int scheme, mode, store, epb, xcd, persistent, assembly, tn;
+  int scheme, mode, store, epb, xcd, persistent, assembly, tn, energy;
   bool operator<(const GraphKey& o) const {
-    return std::tie(dt, phi, scheme, mode, store, epb, xcd, persistent, assembly, tn) <
+    return std::tie(dt, phi, scheme, mode, store, epb, xcd, persistent, assembly, tn, energy) <
            std::tie(o.dt, o.phi, o.scheme, o.mode, o.store, o.epb, o.xcd, o.persistent,
-                    o.assembly, o.tn);
+                    o.assembly, o.tn, o.energy);
   }
 };
 
@@ -99,6 +99,12 @@ struct sld_dev {
   uint32_t* flags_out = nullptr;
   unsigned long long* counters = nullptr;
   double* clock = nullptr;           // [2] t^n in slot cur
+  // energy balance, allocated when first switched on
+  int energy = 0;
+  double* du[3] = {};
+  double* fint_prev[3] = {};
+  double* energy_part = nullptr;     // [3][corrector blocks]
+  double* energy_acc = nullptr;      // allie, allwk, allke, etota, steps
   int mode = SLD_MODE_FP64, store = 0, epb = 8, xcd = 1, persistent = 0, use_graph = 1;
   int assembly = SLD_ASSEMBLY_GATHER;
   double dtcrit = 0.0;
@@ -155,6 +161,14 @@ sld::NodalArgs nodal_args(sld_dev* d, int tn) {
   A.func_maxpts = d->func_maxpts;
   A.fext_func = d->fext_func;
   A.clock = d->clock;
+  if (d->energy) {
+    for (int k = 0; k < 3; ++k) {
+      A.du[k] = d->du[k];
+      A.fint_prev[k] = d->fint_prev[k];
+    }
+    A.mass = d->mass;
+    A.energy_part = d->energy_part;
+  }
   return A;
 }
 
@@ -338,6 +352,9 @@ int enqueue_step(sld_dev* d, double dt, int scheme, double phi, int tn, KernelTi
   TBEGIN(2);
   if (scheme == 0) hipLaunchKernelGGL((sld::sld_correct<0>), dim3(nb), dim3(256), 0, s, N, dt);
   else hipLaunchKernelGGL((sld::sld_correct<1>), dim3(nb), dim3(256), 0, s, N, dt);
+  if (d->energy)
+    hipLaunchKernelGGL(sld::sld_energy_reduce, dim3(1), dim3(256), 0, s, d->energy_part, (int)nb,
+                       d->energy_acc);
   TEND();
   HIPCHK(hipGetLastError());
   return 0;
@@ -736,6 +753,46 @@ int sld_dev_set_assembly(sld_dev* d, int32_t assembly) {
   return SMA_OK;
 }
 
+int sld_dev_set_energy(sld_dev* d, int32_t on) {
+  if (!d) return fail(SMA_ERR_ARG, "null handle");
+  if (on != 0 && on != 1) return fail(SMA_ERR_ARG, "on must be 0 or 1");
+  HIPCHK(hipStreamSynchronize(d->stream));
+  if (on && !d->energy_acc) {
+    const int64_t np = d->npoin;
+    int rc = 0;
+    for (int k = 0; k < 3 && !rc; ++k) {
+      rc = dalloc(&d->du[k], np);
+      if (!rc) track(d, d->du[k]);
+      if (!rc) rc = dalloc(&d->fint_prev[k], np);
+      if (!rc) track(d, d->fint_prev[k]);
+    }
+    if (!rc) rc = dalloc(&d->energy_part, 3 * (size_t)nblocks(np, 256));
+    if (!rc) track(d, d->energy_part);
+    if (!rc) rc = dalloc(&d->energy_acc, 8);
+    if (rc) return rc;   // what was allocated is tracked and freed with the handle
+    track(d, d->energy_acc);
+  }
+  if (on) {
+    // a fresh balance: no previous internal force, no accumulated work
+    const size_t bytes = (size_t)d->npoin * sizeof(double);
+    for (int k = 0; k < 3; ++k) HIPCHK(hipMemsetAsync(d->fint_prev[k], 0, bytes, d->stream));
+    HIPCHK(hipMemsetAsync(d->energy_acc, 0, 8 * sizeof(double), d->stream));
+    HIPCHK(hipStreamSynchronize(d->stream));
+  }
+  d->energy = on;
+  return SMA_OK;
+}
+
+int sld_dev_energy(sld_dev* d, double* out) {
+  if (!d || !out) return fail(SMA_ERR_ARG, "null argument");
+  if (!d->energy_acc) return fail(SMA_ERR_STATE, "the energy balance was never switched on");
+  HIPCHK(hipStreamSynchronize(d->stream));
+  double acc[5];
+  HIPCHK(hipMemcpy(acc, d->energy_acc, sizeof(acc), hipMemcpyDeviceToHost));
+  for (int k = 0; k < 5; ++k) out[k] = acc[k];
+  return SMA_OK;
+}
+
 int sld_dev_step(sld_dev* d, double dt, int32_t scheme, double phi, int64_t nsteps) {
   if (!d) return fail(SMA_ERR_ARG, "null handle");
   if (scheme != 0 && scheme != 1) return fail(SMA_ERR_ARG, "scheme must be 0 (CD) or 1 (TW)");
@@ -745,7 +802,8 @@ int sld_dev_step(sld_dev* d, double dt, int32_t scheme, double phi, int64_t nste
     return fail(SMA_ERR_STATE, "surrogate mode without a surrogate");
   int64_t left = nsteps;
   if (d->use_graph && left >= 2) {
-    GraphKey key{dt, phi, scheme, d->mode, d->store, d->epb, d->xcd, d->persistent, d->assembly, d->cur};
+    GraphKey key{dt, phi, scheme, d->mode, d->store, d->epb, d->xcd, d->persistent, d->assembly, d->cur,
+                 d->energy};
     auto it = d->graphs.find(key);
     if (it == d->graphs.end()) {
       hipGraph_t g = nullptr;

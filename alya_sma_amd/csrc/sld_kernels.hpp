@@ -116,6 +116,13 @@ struct NodalArgs {
   const void* efor;            // gather: corner forces [slot][3] (double or float),
                                // slots of a node in ascending element order
   unsigned* sur_count;         // surrogate compact-list counter, cleared per step
+  // energy balance (mod_sld_energy.f90), when du is non-null: the predictor
+  // stores du = u^{n+1} - u^n, the corrector writes per-block partial sums of
+  // (f_int^{n+1} + f_int^n).du, (f_ext^{n+1} + f_ext^n).du and m v.v
+  double* du[3];
+  double* fint_prev[3];
+  const double* mass;
+  double* energy_part;         // [3][gridDim of the corrector]
 };
 
 // Predictor of node i from the step-n state (u, v, a): u^{n+1} with the
@@ -154,6 +161,7 @@ __global__ void __launch_bounds__(256) sld_predict(NodalArgs A, double dt, doubl
   predict_node(A, i, SCHEME, dt, phi, A.clock[A.tn] + dt, un, vn);
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
+    if (A.du[0]) A.du[d][i] = un[d] - A.u[d][i];
     A.u[d][i] = un[d];
     A.v[d][i] = vn[d];
     if (A.zero_fint) {
@@ -184,11 +192,32 @@ __device__ __forceinline__ void gather_fint(const NodalArgs& A, int64_t i, doubl
   fi[2] = (double)s2;
 }
 
-template <int SCHEME>
-__global__ void __launch_bounds__(256) sld_correct(NodalArgs A, double dt) {
+// Per-node energy terms of one step: internal-force and external-force work
+// over the displacement increment (trapezoidal, mod_sld_energy.f90:113-131)
+// and m v.v (mod_sld_energy.f90:134-140); f_int^{n+1} replaces f_int^n.
+__device__ __forceinline__ void node_energy(const NodalArgs& A, int64_t i, const double fg[3],
+                                            double fx, double fx_n, double e[3]) {
 #pragma clang fp contract(off)
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= A.npoin) return;
+  double ie = 0.0, wk = 0.0, ke = 0.0;
+  const double m = A.mass[i];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double du = A.du[d][i];
+    ie = ie + (fg[d] + A.fint_prev[d][i]) * du;
+    if (A.fext_func >= 0) wk = wk + (fx * A.fext0[d][i] + fx_n * A.fext0[d][i]) * du;
+    const double v = A.v[d][i];
+    ke = ke + v * v * m;
+    A.fint_prev[d][i] = fg[d];
+  }
+  e[0] = ie;
+  e[1] = wk;
+  e[2] = ke;
+}
+
+template <int SCHEME>
+__device__ __forceinline__ void correct_node(const NodalArgs& A, int64_t i, double dt,
+                                             double e[3]) {
+#pragma clang fp contract(off)
   const double cutim = A.clock[A.tn] + dt;  // t^{n+1}
   if (i == 0) {
     A.clock[1 - A.tn] = cutim;
@@ -223,6 +252,76 @@ __global__ void __launch_bounds__(256) sld_correct(NodalArgs A, double dt) {
       if ((fix >> d) & 1) { A.v[d][i] = 0.0; acc = 0.0; }
       A.a[d][i] = acc;
     }
+  }
+  if (A.energy_part) {
+    double fx_n = 0.0;
+    if (A.fext_func >= 0)
+      fx_n = funbou(A.func_table + (size_t)A.fext_func * A.func_maxpts * 4,
+                    A.func_npts[A.fext_func], A.func_smooth[A.fext_func], 0, A.clock[A.tn]);
+    node_energy(A, i, fg, fx, fx_n, e);
+  }
+}
+
+// Fixed-order block sum of three values; thread 0 writes part[k*gridDim.x + b].
+__device__ __forceinline__ void block_sum3(double e[3], double* part) {
+  __shared__ double red[3][256];
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) red[k][t] = e[k];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) red[k][t] = red[k][t] + red[k][t + w];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) part[(size_t)k * gridDim.x + blockIdx.x] = red[k][0];
+  }
+}
+
+template <int SCHEME>
+__global__ void __launch_bounds__(256) sld_correct(NodalArgs A, double dt) {
+#pragma clang fp contract(off)
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double e[3] = {0.0, 0.0, 0.0};
+  if (i < A.npoin) correct_node<SCHEME>(A, i, dt, e);
+  if (A.energy_part) block_sum3(e, A.energy_part);
+}
+
+// Sums the corrector's partials in a fixed order and advances the totals
+// acc = {allie, allwk, allke, etota, steps} (mod_sld_energy.f90:141-162): the
+// first step after a reset takes the full increment, later ones half the
+// trapezoid.
+__global__ void __launch_bounds__(256) sld_energy_reduce(const double* part, int nparts,
+                                                         double* acc) {
+#pragma clang fp contract(off)
+  double e[3] = {0.0, 0.0, 0.0};
+  for (int b = threadIdx.x; b < nparts; b += 256) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) e[k] = e[k] + part[(size_t)k * nparts + b];
+  }
+  __shared__ double red[3][256];
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) red[k][t] = e[k];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) red[k][t] = red[k][t] + red[k][t + w];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    const double w = acc[4] == 0.0 ? 1.0 : 0.5;
+    acc[0] = acc[0] + w * fabs(red[0][0]);
+    acc[1] = acc[1] + w * fabs(red[1][0]);
+    acc[2] = 0.5 * red[2][0];
+    acc[3] = acc[0] + acc[2] - acc[1];
+    acc[4] = acc[4] + 1.0;
   }
 }
 

@@ -63,6 +63,21 @@ module mod_sld_dev
        integer(c_int) :: rc
      end function sld_dev_set_mode
 
+     function sld_dev_set_energy(dev, on) bind(C, name="sld_dev_set_energy") result(rc)
+       import :: c_int, c_int32_t, c_ptr
+       type(c_ptr), value :: dev
+       integer(c_int32_t), value :: on
+       integer(c_int) :: rc
+     end function sld_dev_set_energy
+
+     ! out(5) = allie, allwk, allke, etota, steps (mod_sld_energy.f90:141-162)
+     function sld_dev_energy(dev, out) bind(C, name="sld_dev_energy") result(rc)
+       import :: c_int, c_double, c_ptr
+       type(c_ptr), value :: dev
+       real(c_double), intent(out) :: out(5)
+       integer(c_int) :: rc
+     end function sld_dev_energy
+
      function sld_dev_step(dev, dt, scheme, phi, nsteps) bind(C, name="sld_dev_step") result(rc)
        import :: c_int, c_int32_t, c_int64_t, c_double, c_ptr
        type(c_ptr), value :: dev

@@ -138,6 +138,17 @@ class Solidz:
         a = {"scatter": 0, "gather": 1}.get(assembly, assembly)
         check(lib().sld_dev_set_assembly(self.h, int(a)), "sld_dev_set_assembly")
 
+    def set_energy(self, on=True):
+        """Start (True) a fresh energy balance or stop (False) accumulating."""
+        check(lib().sld_dev_set_energy(self.h, 1 if on else 0), "sld_dev_set_energy")
+
+    def energy(self):
+        """{"allie", "allwk", "allke", "etota", "steps"} of the running balance
+        (mod_sld_energy.f90:141-162)."""
+        out = np.zeros(5, np.float64)
+        check(lib().sld_dev_energy(self.h, out.ctypes.data), "sld_dev_energy")
+        return dict(zip(("allie", "allwk", "allke", "etota", "steps"), out.tolist()))
+
     def autotune(self, dt, nsteps=3, scheme=0, phi=1.033):
         """Pick the fastest assembly and element-kernel launch shape; state
         is unchanged."""

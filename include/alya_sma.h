@@ -193,6 +193,18 @@ int sld_dev_set_tile(sld_dev *dev, int32_t elems_per_block, int32_t xcd_remap, i
 #define SLD_ASSEMBLY_GATHER  1
 int sld_dev_set_assembly(sld_dev *dev, int32_t assembly);
 
+/* Energy balance of the explicit run (replaces sld_energy,
+ * mod_sld_energy.f90:71-162, called from sld_endste.f90): on = 1 starts a
+ * fresh balance (allocated on first use), on = 0 stops accumulating.  Each
+ * step adds the trapezoidal work of the internal and external forces over
+ * the displacement increment (full weight on the first step of a balance,
+ * half afterwards, absolute value of each step's sum) and sets the kinetic
+ * energy 0.5 sum m v.v from the corrected velocity.  Sums are fixed-order:
+ * repeat runs are bitwise equal.  out[5] = {allie, allwk, allke, etota =
+ * allie + allke - allwk, steps accumulated}. */
+int sld_dev_set_energy(sld_dev *dev, int32_t on);
+int sld_dev_energy(sld_dev *dev, double *out);
+
 /* Autotuner: times nsteps eager steps for every launch shape (assembly
  * scatter/gather x elements per workgroup 8/16/32 x XCD mapping on/off x
  * persistent 0/2/4 workgroups per CU) from the same saved state, restores

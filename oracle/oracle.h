@@ -115,6 +115,10 @@ typedef struct {
     uint32_t *flags;           /* [nelem][8] or NULL                         */
     double t;                  /* current time                               */
     int64_t nsteps;
+    /* energy balance (mod_sld_energy.f90:71-162); accumulated when
+     * fint_prev != NULL ([npoin][3], f_int of the previous step) */
+    double *fint_prev;
+    double allie, allwk, allke, etota;
 } orc_state;
 
 int orc_sld_mass(const orc_model *m, orc_state *s);

@@ -142,14 +142,16 @@ class _State(C.Structure):
     _fields_ = [("u", C.c_void_p), ("v", C.c_void_p), ("a", C.c_void_p),
                 ("fint", C.c_void_p), ("rhs", C.c_void_p), ("mass", C.c_void_p),
                 ("sdv", C.c_void_p), ("S", C.c_void_p), ("flags", C.c_void_p),
-                ("t", C.c_double), ("nsteps", C.c_int64)]
+                ("t", C.c_double), ("nsteps", C.c_int64),
+                ("fint_prev", C.c_void_p), ("allie", C.c_double), ("allwk", C.c_double),
+                ("allke", C.c_double), ("etota", C.c_double)]
 
 
 class StepOracle:
     """FP64 CPU oracle of the explicit step on a problem described by the
     same plain arrays the device library takes (see alya_sma_amd.problem)."""
 
-    def __init__(self, prob, elem_order=None, color_ptr=None, keep_stress=True):
+    def __init__(self, prob, elem_order=None, color_ptr=None, keep_stress=True, energy=False):
         self._keep = []
         k = self._keep.append
 
@@ -198,6 +200,8 @@ class StepOracle:
         s.fint, s.rhs, s.mass = _ptr(self.fint), _ptr(self.rhs), _ptr(self.mass)
         s.sdv, s.S, s.flags = _ptr(self.sdv), _ptr(self.S), _ptr(self.flags)
         s.t, s.nsteps = 0.0, 0
+        self.fint_prev = np.zeros((n, 3)) if energy else None
+        s.fint_prev = _ptr(self.fint_prev)
         self.s = s
         lib().orc_sld_mass(C.byref(self.m), C.byref(self.s))
 
@@ -211,6 +215,10 @@ class StepOracle:
     @property
     def t(self):
         return self.s.t
+
+    def energy(self):
+        """(allie, allwk, allke, etota) of mod_sld_energy.f90 (energy=True)."""
+        return self.s.allie, self.s.allwk, self.s.allke, self.s.etota
 
     def funbou(self, f, c, t):
         return lib().orc_funbou(C.byref(self.m), f, c, t)

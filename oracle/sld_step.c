@@ -204,8 +204,16 @@ int orc_sld_step(const orc_model *m, orc_state *s, double dt, int scheme, double
     int64_t np = m->npoin;
     double cutim = s->t + dt;             /* setgts.f90:98 */
     double t_end = cutim + dt, t_mid = 0.5 * (cutim + t_end);
-    double fx = 0.0;
-    if (m->fext0) fx = orc_funbou(m, m->fext_func, 0, cutim);
+    double fx = 0.0, fx_n = 0.0;
+    if (m->fext0) {
+        fx = orc_funbou(m, m->fext_func, 0, cutim);
+        fx_n = orc_funbou(m, m->fext_func, 0, s->t);
+    }
+    double *u_old = NULL;
+    if (s->fint_prev) {
+        u_old = (double *)malloc(sizeof(double) * 3 * np);
+        memcpy(u_old, s->u, sizeof(double) * 3 * np);
+    }
 
     /* predictor */
     for (int64_t p = 0; p < np; p++) {
@@ -248,6 +256,26 @@ int orc_sld_step(const orc_model *m, orc_state *s, double dt, int scheme, double
                 if (fixed) { s->v[3 * p + d] = 0.0; s->a[3 * p + d] = 0.0; }
             }
         }
+    }
+    /* energies (mod_sld_energy.f90:113-140): trapezoidal internal energy
+     * and external work over the displacement increment (full weight on
+     * the first step), kinetic energy of the lumped mass */
+    if (s->fint_prev) {
+        double ie = 0.0, wk = 0.0, ke = 0.0;
+        for (int64_t p = 0; p < np; p++)
+            for (int d = 0; d < 3; d++) {
+                double du = s->u[3 * p + d] - u_old[3 * p + d];
+                ie = ie + (s->fint[3 * p + d] + s->fint_prev[3 * p + d]) * du;
+                if (m->fext0) wk = wk + (fx * m->fext0[3 * p + d] + fx_n * m->fext0[3 * p + d]) * du;
+                ke = ke + s->v[3 * p + d] * s->v[3 * p + d] * s->mass[p];
+            }
+        double w = s->nsteps == 0 ? 1.0 : 0.5;
+        s->allie = s->allie + w * fabs(ie);
+        s->allwk = s->allwk + w * fabs(wk);
+        s->allke = 0.5 * ke;
+        s->etota = s->allie + s->allke - s->allwk;
+        memcpy(s->fint_prev, s->fint, sizeof(double) * 3 * np);
+        free(u_old);
     }
     /* sld_updunk ENDSTE: TIME_N <- ITER_K */
     memcpy(s->sdv + 24 * m->nelem, s->sdv, sizeof(double) * 24 * m->nelem);

@@ -1,0 +1,110 @@
+"""Device energy balance (sld_dev_set_energy / sld_dev_energy, the
+sld_energy of mod_sld_energy.f90:71-162) against the CPU oracle's: same
+totals to 1e-10 relative, bitwise repeatable, no effect on the step."""
+import numpy as np
+import pytest
+
+from alya_sma_amd import problems
+from alya_sma_amd.solid import Solidz
+from oracle import oracle_py as O
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-10
+KEYS = ("allie", "allwk", "allke", "etota")
+
+
+def _check(dev, orc, label, scale):
+    e = dev.energy()
+    ref = dict(zip(KEYS, orc.energy()))
+    errs = {k: abs(e[k] - ref[k]) / scale for k in KEYS}
+    print(label, {k: f"{ref[k]:.6e}/{x:.1e}" for k, x in errs.items()}, flush=True)
+    for k, x in errs.items():
+        assert x < TOL, (label, k, x)
+    return e
+
+
+@pytest.mark.parametrize("scheme", [0, 1])
+@pytest.mark.parametrize("assembly", ["gather", "scatter"])
+def test_elastic_bar_energy_matches_oracle(scheme, assembly):
+    """Axial end force ramped on a 12x2x2 Nitinol bar: 3000 graph-replayed
+    steps, energies compared every 500."""
+    prob = problems.cantilever_problem(nx=12, ny=2, nz=2, lx=6.0, ly=1.0, lz=1.0, traction=10.0,
+                                       direction=0, nsteps=6000, E_noise=0.0)
+    r = prob["run"]
+    print(f"elastic bar scheme {scheme} {assembly}", flush=True)
+    with Solidz(prob) as dev:
+        dev.set_assembly(assembly)
+        ptr, order = dev.coloring()
+        orc = O.StepOracle(prob, elem_order=order, color_ptr=ptr, energy=True)
+        dev.set_energy(True)
+        for k in range(6):
+            dev.step(r["dt"], nsteps=500, scheme=scheme, phi=r["phi"])
+            orc.step(r["dt"], scheme=scheme, phi=r["phi"], n=500)
+            # central differences integrate the reaction into the velocity of
+            # clamped dofs (sld_updunk), so allke can dwarf allie
+            scale = max(abs(x) for x in orc.energy())
+            e = _check(dev, orc, f"  step {500 * (k + 1)}", scale)
+        assert e["steps"] == 3000.0
+
+
+def test_sma_bar_energy_matches_oracle():
+    """10x2x2 SMA bar stretched through the forward plateau (config 0's
+    ramp, first 800 of 2000 steps, transformation active), device restarted
+    from the oracle's state every 25 steps as the undamped plateau amplifies
+    round-off; the energy sums carry over the restarts."""
+    prob = problems.bar_problem()
+    r = prob["run"]
+    with Solidz(prob) as dev:
+        dev.set_store(True)
+        ptr, order = dev.coloring()
+        orc = O.StepOracle(prob, elem_order=order, color_ptr=ptr, energy=True)
+        dev.set_energy(True)
+        for chunk in range(32):
+            if chunk:
+                dev.set("u", orc.u)
+                dev.set("v", orc.v)
+                dev.set("a", orc.a)
+                dev.set("sdv", orc.sdv[1])
+                dev.set("time", np.array([orc.t]))
+            dev.step(r["dt"], nsteps=25, scheme=0, phi=r["phi"])
+            orc.step(r["dt"], scheme=0, phi=r["phi"], n=25)
+        assert dev.counters()["kf_AS"] > 0
+        _check(dev, orc, "sma bar step 800", max(abs(x) for x in orc.energy()))
+
+
+def test_energy_is_repeatable_and_leaves_the_step_alone():
+    prob = problems.cantilever_problem(nx=40, ny=4, nz=4, lx=5.0, ly=0.5, lz=0.5, traction=500.0,
+                                       nsteps=400)
+    r = prob["run"]
+    runs = []
+    for energy in (True, True, False):
+        with Solidz(prob) as dev:
+            if energy:
+                dev.set_energy(True)
+            dev.step(r["dt"], nsteps=300, scheme=0, phi=r["phi"])
+            runs.append((dev.get("u"), dev.get("v"), dev.energy() if energy else None))
+    assert runs[0][2] == runs[1][2]
+    assert runs[0][2]["allie"] > 0.0
+    for k in range(2):
+        np.testing.assert_array_equal(runs[0][k], runs[2][k])
+        np.testing.assert_array_equal(runs[0][k], runs[1][k])
+
+
+def test_energy_restart_and_stop():
+    prob = problems.cantilever_problem(nx=12, ny=2, nz=2, lx=6.0, ly=1.0, lz=1.0, traction=10.0,
+                                       direction=0, nsteps=2000, E_noise=0.0)
+    r = prob["run"]
+    with Solidz(prob) as dev:
+        with pytest.raises(RuntimeError):
+            dev.energy()                      # never switched on
+        dev.set_energy(True)
+        dev.step(r["dt"], nsteps=400, scheme=1, phi=r["phi"])
+        e1 = dev.energy()
+        dev.set_energy(False)
+        dev.step(r["dt"], nsteps=100, scheme=1, phi=r["phi"])
+        assert dev.energy() == e1             # stopped: nothing accumulates
+        dev.set_energy(True)
+        e2 = dev.energy()
+        assert e2["allie"] == 0.0 and e2["steps"] == 0.0
+        dev.step(r["dt"], nsteps=3, scheme=1, phi=r["phi"])
+        assert dev.energy()["steps"] == 3.0
