@@ -384,7 +384,7 @@ struct GenArgs {
   uint32_t* flags;         // [n]
 };
 
-// Training samples drawn like config 2: log-strain components ~ N(0, sd^2),
+// Training samples drawn like the material-point workload: log-strain components ~ N(0, sd^2),
 // the previous state one increment (~ N(0, step_sd^2)) away, xS_0 in [0, 1]
 // (the transformation strain eL xS_0 inside the 0.05 ball), T ~ U[290, 340],
 // material parameters jittered +-jitter.  Labels from the exact FP64 map.

@@ -221,7 +221,7 @@ def batched_specimens(n=256, seed=3, median=8000, sigma=0.7, lo=2000, hi=60000, 
     return prob
 
 
-# Tissue of config 4: neo-Hookean (Ansys form) mu = 0.03 MPa, K = 1 MPa,
+# Tissue around the SMA tube: neo-Hookean (Ansys form) mu = 0.03 MPa, K = 1 MPa,
 # density 1000 kg/m^3 (props[0] = mu, props[1] = K as parco_sld for NEOHO ANSYS).
 TISSUE = np.array([0.03, 1.0] + [0.0] * 16, dtype=np.float64)
 RHO_TISSUE = 1.0e-9
@@ -231,7 +231,7 @@ LAW_SMA, LAW_NEOHOOKEAN = 160, 102
 def cylinder_problem(ntheta=360, nz=400, nr_tube=3, nr_tissue=39, r_in=4.0, r_tube=4.3,
                      r_out=10.0, length=40.0, crimp=0.4, nsteps=10000, safety=0.5,
                      props=NITINOL, rho=RHO_NITINOL):
-    """Config 4: SMA tube r in [r_in, r_tube] inside neo-Hookean tissue
+    """SMA tube r in [r_in, r_tube] inside neo-Hookean tissue
     r in [r_tube, r_out], length `length`, structured hex8 in (r, theta, z)
     (nr_tube + nr_tissue) x ntheta x nz elements (default 42 x 360 x 400 =
     6.05 M), closed in theta.  The inner surface r = r_in is crimped by the

@@ -227,8 +227,9 @@ def accuracy(prob, modes, K, cfgs, sur=None, tol=1e-4):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU "
-              "sharding of this path is described in DESIGN.md and not run here.", flush=True)
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): the step is "
+              "measured on one GPU's share of the mesh; a multi-GPU split of the "
+              "elements is not part of this library.", flush=True)
         sys.exit(2)
     from alya_sma_amd import problems
     from alya_sma_amd.device import device_count, synchronize

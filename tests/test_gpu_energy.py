@@ -48,7 +48,7 @@ def test_elastic_bar_energy_matches_oracle(scheme, assembly):
 
 
 def test_sma_bar_energy_matches_oracle():
-    """10x2x2 SMA bar stretched through the forward plateau (config 0's
+    """10x2x2 SMA bar stretched through the forward plateau (the 6%
     ramp, first 800 of 2000 steps, transformation active), device restarted
     from the oracle's state every 25 steps as the undamped plateau amplifies
     round-off; the energy sums carry over the restarts."""

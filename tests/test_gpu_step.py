@@ -54,8 +54,8 @@ def _sync_from_oracle(dev, orc):
 
 
 @pytest.mark.parametrize("scheme", [0, 1])
-def test_bar_config0_full_cycle(scheme):
-    """Config 0: 10x2x2 bar, ramp to 6% strain and back, 2000 steps.
+def test_bar_10x2x2_full_cycle(scheme):
+    """10x2x2 bar, ramp to 6% strain and back, 2000 steps.
 
     The undamped unloading through the reverse plateau amplifies a 1e-15
     perturbation a hundredfold every ~50 steps near its end (the CPU oracle
@@ -63,7 +63,7 @@ def test_bar_config0_full_cycle(scheme):
     every 25 steps and the step map is compared over each chunk."""
     prob = problems.bar_problem()
     r = prob["run"]
-    print(f"bar config0 scheme {scheme}: 2000 steps", flush=True)
+    print(f"bar 10x2x2 scheme {scheme}: 2000 steps", flush=True)
     dev, orc = _pair(prob)
     with dev:
         np.testing.assert_allclose(dev.get("mass"), orc.mass, rtol=1e-13)
@@ -80,7 +80,7 @@ def test_bar_config0_full_cycle(scheme):
         assert abs(t - orc.t) <= 1e-15 * orc.t
 
 
-def test_bar_config0_free_run_first_1500_steps():
+def test_bar_10x2x2_free_run_first_1500_steps():
     """Without resynchronisation the trajectories agree to 1e-10 through the
     loading, saturation and start of unloading."""
     prob = problems.bar_problem()
@@ -251,7 +251,7 @@ def test_fp32_mode_error_vs_fp64_oracle():
 
 
 def test_batched_uneven_specimens_match_oracle():
-    """Config 3 in small: six bars of uneven size, each with its own material
+    """Batched specimens in small: six bars of uneven size, each with its own material
     row and load function, stepped as one mesh (one element launch) against
     the oracle, restarted from the oracle's state every 25 steps."""
     prob = problems.batched_specimens(n=6, median=80, lo=30, hi=300, nsteps=400)
@@ -282,7 +282,7 @@ def test_batched_specimens_persistent_launch_bitwise():
 
 
 def test_tube_in_tissue_two_materials_match_oracle():
-    """Config 4 in small: SMA tube (model 160) in neo-Hookean tissue (model
+    """Tube in tissue in small: SMA tube (model 160) in neo-Hookean tissue (model
     102), inner surface crimped radially, against the oracle (restarted from
     its state every 25 steps), plus the critical step of the two materials."""
     prob = problems.cylinder_problem(ntheta=16, nz=6, nr_tube=2, nr_tissue=3, nsteps=400)

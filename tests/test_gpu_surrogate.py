@@ -129,7 +129,7 @@ def test_training_lowers_loss():
 
 
 def test_surrogate_in_step_gain():
-    """What the trained network gains in the step (uniaxial bar, config 0):
+    """What the trained network gains in the step (uniaxial 10x2x2 bar):
     from the FP64 trajectory's state at transforming steps, one step in FP32
     mode (exact return map) and one in surrogate mode.  Accepted points must
     lie within tol of the exact xS (the gate is exact: the single-branch

@@ -10,7 +10,7 @@ import pytest
 
 from oracle import oracle_py as O
 
-MU, K = 0.03, 1.0            # tissue of config 4, MPa
+MU, K = 0.03, 1.0            # tissue around the SMA tube, MPa
 
 
 def energy(C, mu=MU, k=K):
