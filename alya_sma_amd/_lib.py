@@ -35,6 +35,12 @@ class SldMaterialDesc(C.Structure):
                 ("mat_of_elem", C.c_void_p), ("E_of_elem", C.c_void_p), ("law", C.c_void_p)]
 
 
+class SldFaceDesc(C.Structure):
+    _fields_ = [("nface", C.c_int64), ("lnodb", C.c_void_p), ("code", C.c_void_p),
+                ("value", C.c_void_p), ("func", C.c_void_p), ("transient", C.c_int32),
+                ("follower", C.c_int32)]
+
+
 class SldBcDesc(C.Structure):
     _fields_ = [("fixmask", C.c_void_p), ("func_of_node", C.c_void_p),
                 ("nfunc", C.c_int32), ("func_maxpts", C.c_int32),
@@ -76,6 +82,7 @@ def lib():
     L.sld_dev_set_tile.argtypes = [p, i32, i32, i32]
     L.sld_dev_set_assembly.argtypes = [p, i32]
     L.sld_dev_set_energy.argtypes = [p, i32]
+    L.sld_dev_set_faces.argtypes = [p, p]
     L.sld_dev_energy.argtypes = [p, p]
     L.sld_dev_autotune.argtypes = [p, d, i32, d, i32, p, p]
     L.sld_dev_stream.argtypes = [p]

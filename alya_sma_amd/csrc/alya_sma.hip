@@ -103,8 +103,21 @@ struct sld_dev {
   int energy = 0;
   double* du[3] = {};
   double* fint_prev[3] = {};
+  double* fext_prev[3] = {};
   double* energy_part = nullptr;     // [3][corrector blocks]
   double* energy_acc = nullptr;      // allie, allwk, allke, etota, steps
+  // boundary faces with Neumann loads (sld_dev_set_faces)
+  struct Faces {
+    int64_t nface = 0;
+    int32_t transient = 0, follower = 0;
+    int32_t* lnodb = nullptr;
+    int32_t* code = nullptr;
+    double* value = nullptr;
+    int32_t* func = nullptr;
+    int32_t* slot = nullptr;           // [4 nface] node-sorted slot of each corner
+    int32_t* node_ptr = nullptr;       // [npoin+1]
+    double* fface = nullptr;           // [4 nface][3]
+  } faces;
   int mode = SLD_MODE_FP64, store = 0, epb = 8, xcd = 1, persistent = 0, use_graph = 1;
   int assembly = SLD_ASSEMBLY_GATHER;
   double dtcrit = 0.0;
@@ -166,8 +179,13 @@ sld::NodalArgs nodal_args(sld_dev* d, int tn) {
       A.du[k] = d->du[k];
       A.fint_prev[k] = d->fint_prev[k];
     }
+    for (int k = 0; k < 3; ++k) A.fext_prev[k] = d->fext_prev[k];
     A.mass = d->mass;
     A.energy_part = d->energy_part;
+  }
+  if (d->faces.nface > 0) {
+    A.node_face_ptr = d->faces.node_ptr;
+    A.fface = d->faces.fface;
   }
   return A;
 }
@@ -270,6 +288,31 @@ int enqueue_step(sld_dev* d, double dt, int scheme, double phi, int tn, KernelTi
   TBEGIN(0);
   if (scheme == 0) hipLaunchKernelGGL((sld::sld_predict<0>), dim3(nb), dim3(256), 0, s, N, dt, phi);
   else hipLaunchKernelGGL((sld::sld_predict<1>), dim3(nb), dim3(256), 0, s, N, dt, phi);
+  if (d->faces.nface > 0) {
+    const sld_dev::Faces& Fc = d->faces;
+    sld::FaceArgs F{};
+    F.nface = Fc.nface;
+    F.lnodb = Fc.lnodb;
+    F.code = Fc.code;
+    F.value = Fc.value;
+    F.func = Fc.func;
+    F.transient = Fc.transient;
+    F.follower = Fc.follower;
+    for (int k = 0; k < 3; ++k) {
+      F.coord[k] = d->coord[k];
+      F.u[k] = d->u[k];
+    }
+    F.fixmask = d->fixmask;
+    F.func_table = d->func_table;
+    F.func_npts = d->func_npts;
+    F.func_smooth = d->func_smooth;
+    F.func_maxpts = d->func_maxpts;
+    F.slot = Fc.slot;
+    F.fface = Fc.fface;
+    F.clock = d->clock;
+    F.tn = tn;
+    hipLaunchKernelGGL(sld::sld_face_load, dim3(nblocks(Fc.nface, 256)), dim3(256), 0, s, F, dt);
+  }
   TEND();
 
   {
@@ -765,6 +808,8 @@ int sld_dev_set_energy(sld_dev* d, int32_t on) {
       if (!rc) track(d, d->du[k]);
       if (!rc) rc = dalloc(&d->fint_prev[k], np);
       if (!rc) track(d, d->fint_prev[k]);
+      if (!rc) rc = dalloc(&d->fext_prev[k], np);
+      if (!rc) track(d, d->fext_prev[k]);
     }
     if (!rc) rc = dalloc(&d->energy_part, 3 * (size_t)nblocks(np, 256));
     if (!rc) track(d, d->energy_part);
@@ -775,11 +820,72 @@ int sld_dev_set_energy(sld_dev* d, int32_t on) {
   if (on) {
     // a fresh balance: no previous internal force, no accumulated work
     const size_t bytes = (size_t)d->npoin * sizeof(double);
-    for (int k = 0; k < 3; ++k) HIPCHK(hipMemsetAsync(d->fint_prev[k], 0, bytes, d->stream));
+    for (int k = 0; k < 3; ++k) {
+      HIPCHK(hipMemsetAsync(d->fint_prev[k], 0, bytes, d->stream));
+      HIPCHK(hipMemsetAsync(d->fext_prev[k], 0, bytes, d->stream));
+    }
     HIPCHK(hipMemsetAsync(d->energy_acc, 0, 8 * sizeof(double), d->stream));
     HIPCHK(hipStreamSynchronize(d->stream));
   }
   d->energy = on;
+  return SMA_OK;
+}
+
+int sld_dev_set_faces(sld_dev* d, const sld_face_desc* fd) {
+  if (!d || !fd) return fail(SMA_ERR_ARG, "null argument");
+  const int64_t nf = fd->nface, np = d->npoin;
+  if (nf < 0 || nf > (int64_t)INT32_MAX / 4) return fail(SMA_ERR_ARG, "nface out of range");
+  if (nf > 0 && (!fd->lnodb || !fd->code || !fd->value))
+    return fail(SMA_ERR_ARG, "lnodb, code and value are required");
+  if ((fd->transient != 0 && fd->transient != 1) || (fd->follower != 0 && fd->follower != 1))
+    return fail(SMA_ERR_ARG, "transient and follower must be 0 or 1");
+  for (int64_t f = 0; f < nf; ++f) {
+    if (fd->code[f] != 3 && fd->code[f] != 6)
+      return fail(SMA_ERR_ARG, "face code must be 3 (pressure) or 6 (traction)");
+    for (int a = 0; a < 4; ++a)
+      if (fd->lnodb[4 * f + a] < 0 || fd->lnodb[4 * f + a] >= np)
+        return fail(SMA_ERR_ARG, "face node out of range");
+    if (fd->func && (fd->func[f] < -1 || fd->func[f] >= d->nfunc))
+      return fail(SMA_ERR_ARG, "face function out of range");
+  }
+  // node-sorted corner slots: the corners of a node in ascending face order
+  std::vector<int32_t> ptr((size_t)np + 1, 0), slot((size_t)4 * nf);
+  for (int64_t k = 0; k < 4 * nf; ++k) ptr[(size_t)fd->lnodb[k] + 1]++;
+  for (int64_t p = 0; p < np; ++p) ptr[p + 1] += ptr[p];
+  {
+    std::vector<int32_t> fill(ptr.begin(), ptr.end() - 1);
+    for (int64_t k = 0; k < 4 * nf; ++k) slot[k] = fill[fd->lnodb[k]]++;
+  }
+  HIPCHK(hipStreamSynchronize(d->stream));
+  free_graphs(d);   // captured steps hold the previous face arguments
+  sld_dev::Faces& F = d->faces;
+  auto drop = [&](void* q) {
+    if (!q) return;
+    d->allocs.erase(std::remove(d->allocs.begin(), d->allocs.end(), q), d->allocs.end());
+    hipFree(q);
+  };
+  drop(F.lnodb); drop(F.code); drop(F.value); drop(F.func); drop(F.slot); drop(F.node_ptr);
+  drop(F.fface);
+  F = sld_dev::Faces{};
+  if (nf == 0) return SMA_OK;
+  std::vector<int32_t> fn((size_t)nf, -1);
+  if (fd->func)
+    for (int64_t f = 0; f < nf; ++f) fn[f] = fd->func[f];
+  int rc = upload(&F.lnodb, fd->lnodb, (size_t)4 * nf, d->stream);
+  if (!rc) { track(d, F.lnodb); rc = upload(&F.code, fd->code, (size_t)nf, d->stream); }
+  if (!rc) { track(d, F.code); rc = upload(&F.value, fd->value, (size_t)3 * nf, d->stream); }
+  if (!rc) { track(d, F.value); rc = upload(&F.func, fn.data(), (size_t)nf, d->stream); }
+  if (!rc) { track(d, F.func); rc = upload(&F.slot, slot.data(), slot.size(), d->stream); }
+  if (!rc) { track(d, F.slot); rc = upload(&F.node_ptr, ptr.data(), ptr.size(), d->stream); }
+  if (!rc) { track(d, F.node_ptr); rc = dalloc(&F.fface, (size_t)12 * nf); }
+  if (rc) {
+    F = sld_dev::Faces{};   // tracked buffers are released with the handle
+    return rc;
+  }
+  track(d, F.fface);
+  F.nface = nf;
+  F.transient = fd->transient;
+  F.follower = fd->follower;
   return SMA_OK;
 }
 

@@ -121,8 +121,35 @@ struct NodalArgs {
   // (f_int^{n+1} + f_int^n).du, (f_ext^{n+1} + f_ext^n).du and m v.v
   double* du[3];
   double* fint_prev[3];
+  double* fext_prev[3];
   const double* mass;
   double* energy_part;         // [3][gridDim of the corrector]
+  // boundary-face loads: [npoin+1] slot ranges per node (null = no faces)
+  // and the face-corner forces of the step [slot][3]
+  const int32_t* node_face_ptr;
+  const double* fface;
+};
+
+// Boundary faces with Neumann loads (sld_bouope.f90), one QUA04 face per
+// thread; corner forces go to node-sorted slots that the corrector sums.
+struct FaceArgs {
+  int64_t nface;
+  const int32_t* lnodb;        // [nface][4]
+  const int32_t* code;         // [nface] 3 pressure, 6 traction
+  const double* value;         // [nface][3]
+  const int32_t* func;         // [nface] or null
+  int32_t transient, follower;
+  const double* coord[3];
+  const double* u[3];          // u^{n+1} (follower loads)
+  const uint8_t* fixmask;
+  const double* func_table;
+  const int32_t* func_npts;
+  const int32_t* func_smooth;
+  int32_t func_maxpts;
+  const int32_t* slot;         // [4 nface] slot of each face corner
+  double* fface;               // [4 nface][3]
+  const double* clock;
+  int32_t tn;
 };
 
 // Predictor of node i from the step-n state (u, v, a): u^{n+1} with the
@@ -194,9 +221,10 @@ __device__ __forceinline__ void gather_fint(const NodalArgs& A, int64_t i, doubl
 
 // Per-node energy terms of one step: internal-force and external-force work
 // over the displacement increment (trapezoidal, mod_sld_energy.f90:113-131)
-// and m v.v (mod_sld_energy.f90:134-140); f_int^{n+1} replaces f_int^n.
+// and m v.v (mod_sld_energy.f90:134-140); f_int^{n+1} and f_ext^{n+1} replace
+// the previous step's.
 __device__ __forceinline__ void node_energy(const NodalArgs& A, int64_t i, const double fg[3],
-                                            double fx, double fx_n, double e[3]) {
+                                            const double fe[3], double e[3]) {
 #pragma clang fp contract(off)
   double ie = 0.0, wk = 0.0, ke = 0.0;
   const double m = A.mass[i];
@@ -204,7 +232,8 @@ __device__ __forceinline__ void node_energy(const NodalArgs& A, int64_t i, const
   for (int d = 0; d < 3; ++d) {
     const double du = A.du[d][i];
     ie = ie + (fg[d] + A.fint_prev[d][i]) * du;
-    if (A.fext_func >= 0) wk = wk + (fx * A.fext0[d][i] + fx_n * A.fext0[d][i]) * du;
+    wk = wk + (fe[d] + A.fext_prev[d][i]) * du;
+    A.fext_prev[d][i] = fe[d];
     const double v = A.v[d][i];
     ke = ke + v * v * m;
     A.fint_prev[d][i] = fg[d];
@@ -230,6 +259,19 @@ __device__ __forceinline__ void correct_node(const NodalArgs& A, int64_t i, doub
                 A.func_npts[A.fext_func], A.func_smooth[A.fext_func], 0, cutim);
   const double rm = A.rmass[i];
   const int fix = A.fixmask[i];
+  // f_ext: face loads (summed per node in face order) + fx fext0
+  double fe[3] = {0.0, 0.0, 0.0};
+  if (A.node_face_ptr) {
+    const int32_t k0 = A.node_face_ptr[i], k1 = A.node_face_ptr[i + 1];
+    for (int32_t k = k0; k < k1; ++k) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) fe[d] = fe[d] + A.fface[3 * (int64_t)k + d];
+    }
+  }
+  if (A.fext_func >= 0) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) fe[d] = fe[d] + fx * A.fext0[d][i];
+  }
   double fg[3];
   if (A.node_ptr) {
     if (A.fint_fp32) gather_fint<float>(A, i, fg);
@@ -243,7 +285,7 @@ __device__ __forceinline__ void correct_node(const NodalArgs& A, int64_t i, doub
   for (int d = 0; d < 3; ++d) {
     double fi = fg[d];
     double r = 0.0 - fi;
-    if (A.fext_func >= 0) r = r + fx * A.fext0[d][i];
+    r = r + fe[d];
     double acc = 0.0 + r * rm;
     if (SCHEME == 0) {
       A.a[d][i] = acc;
@@ -253,13 +295,7 @@ __device__ __forceinline__ void correct_node(const NodalArgs& A, int64_t i, doub
       A.a[d][i] = acc;
     }
   }
-  if (A.energy_part) {
-    double fx_n = 0.0;
-    if (A.fext_func >= 0)
-      fx_n = funbou(A.func_table + (size_t)A.fext_func * A.func_maxpts * 4,
-                    A.func_npts[A.fext_func], A.func_smooth[A.fext_func], 0, A.clock[A.tn]);
-    node_energy(A, i, fg, fx, fx_n, e);
-  }
+  if (A.energy_part) node_energy(A, i, fg, fe, e);
 }
 
 // Fixed-order block sum of three values; thread 0 writes part[k*gridDim.x + b].
@@ -322,6 +358,93 @@ __global__ void __launch_bounds__(256) sld_energy_reduce(const double* part, int
     acc[2] = 0.5 * red[2][0];
     acc[3] = acc[0] + acc[2] - acc[1];
     acc[4] = acc[4] + 1.0;
+  }
+}
+
+// Face loads of one step at t^{n+1} (sld_bouope.f90:88-277 with the values
+// of sld_updbcs.f90:153-181): geometry of the face's 4 nodes (reference, or
+// current with follower loads); value = the constant one, or under transient
+// conditions (column x of the face's function, 0, 0) and 0 without one
+// (sld_funbou.f90 itask 1); at the 2x2 Gauss points (s outer, t inner,
+// mod_quadrature_gauss.f90:2361-2370) the tangents of bouder
+// (mod_bouder.f90:74-89), n = t_s x t_t, |n|, the unit normal of chenor, the
+// traction (code 3: value_1 n/|n|, code 6: value) and the corner forces
+// |n| tract N_a on dofs that are not prescribed (QUA04 of mod_lagrange.f90).
+__global__ void __launch_bounds__(256) sld_face_load(FaceArgs F, double dt) {
+#pragma clang fp contract(off)
+  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= F.nface) return;
+  const double cutim = F.clock[F.tn] + dt;
+  int32_t nb[4];
+  double x[4][3];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    nb[a] = F.lnodb[4 * f + a];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      x[a][d] = F.coord[d][nb[a]];
+      if (F.follower) x[a][d] = x[a][d] + F.u[d][nb[a]];
+    }
+  }
+  double val[3];
+  if (F.transient) {
+    const int fn = F.func ? F.func[f] : -1;
+    val[0] = fn >= 0 ? funbou(F.func_table + (size_t)fn * F.func_maxpts * 4, F.func_npts[fn],
+                              F.func_smooth[fn], 0, cutim)
+                     : 0.0;
+    val[1] = 0.0;
+    val[2] = 0.0;
+  } else {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) val[d] = F.value[3 * f + d];
+  }
+  const int code = F.code[f];
+  const double g = 0.577350269189626;
+  double elfex[4][3];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) elfex[a][d] = 0.0;
+  for (int ig = 0; ig < 4; ++ig) {
+    const double s = ig < 2 ? -g : g, t = (ig & 1) ? g : -g, st = s * t;
+    const double N[4] = {(1.0 - t - s + st) * 0.25, (1.0 - t + s - st) * 0.25,
+                         (1.0 + t + s + st) * 0.25, (1.0 + t - s - st) * 0.25};
+    const double ds[4] = {(-1.0 + t) * 0.25, (1.0 - t) * 0.25, (1.0 + t) * 0.25,
+                          (-1.0 - t) * 0.25};
+    const double dtt[4] = {(-1.0 + s) * 0.25, (-1.0 - s) * 0.25, (1.0 + s) * 0.25,
+                           (1.0 - s) * 0.25};
+    double t1[3], t2[3], n[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      t1[d] = 0.0;
+      t2[d] = 0.0;
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        t1[d] = t1[d] + x[a][d] * ds[a];
+        t2[d] = t2[d] + x[a][d] * dtt[a];
+      }
+    }
+    n[0] = t1[1] * t2[2] - t1[2] * t2[1];
+    n[1] = t1[2] * t2[0] - t1[0] * t2[2];
+    n[2] = t1[0] * t2[1] - t1[1] * t2[0];
+    const double eucta = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    const double gbsur = 1.0 * eucta;
+    double tract[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) tract[d] = code == 3 ? (n[d] / eucta) * val[0] : val[d];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const int fix = F.fixmask[nb[a]];
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+        if (!((fix >> d) & 1)) elfex[a][d] = elfex[a][d] + gbsur * tract[d] * N[a];
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const int64_t k = F.slot[4 * f + a];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) F.fface[3 * k + d] = elfex[a][d];
   }
 }
 

@@ -8,17 +8,22 @@ one this path uses:
 
   .dat       TIME_INTERVAL (t0, t1), NUMBER_OF_STEPS, TIME_COUPLING
              FROM_CRITICAL
-  .geo.dat   ELEMENTS (HEX08), COORDINATES, BOUNDARIES (face nodes + element),
-             MATERIALS
-  .fix.dat   ON_BOUNDARIES (boundary -> code)
+  .dom.dat   FIELDS (FIELD = n: "node fx fy fz" nodal values)
+  .geo.dat   ELEMENTS (HEX08), COORDINATES, BOUNDARIES (QUA04 face nodes
+             [+ element]), MATERIALS
+  .fix.dat   ON_BOUNDARIES (boundary -> code), ON_NODES (node -> code)
   .set.dat   NODE (node -> node set)
   .sld.dat   PROPERTIES: MATERIAL, DENSITY, CONSTITUTIVE_MODEL NITINOL
              (sld_reaphy.f90: parco(1:17)) or NEOHO ANSYS mu K
-             (lawst 102); NUMERICAL_TREATMENT: TIME_INTEGRATION
-             (TCHAMWA_WIELGOSZ with PHITW, or central difference),
-             SAFETY_FACTOR; BOUNDARY_CONDITIONS CODES, NODES rules
-             ("c1 & c2 ... fixity values [, MODULE_FUNCTION= f]") and
-             FUNCTIONS (DISCRETE[, SMOOTH] shape tables, sld_funbou.f90)
+             (lawst 102), FORCE: FIELD = n (nodal forces ramped by
+             t/(t1 - t0), sld_matrix.f90:249-262); NUMERICAL_TREATMENT:
+             TIME_INTEGRATION (TCHAMWA_WIELGOSZ with PHITW, or central
+             difference), SAFETY_FACTOR; BOUNDARY_CONDITIONS[, TRANSIENT |
+             CONSTANT] with CODES, NODES rules ("c1 & c2 ... fixity values
+             [, MODULE_FUNCTION= f]"), CODES, BOUNDARIES rules ("code fixbo
+             v1 v2 v3 [, MODULE_FUNCTION= f]", fixbo 3 pressure or 6
+             traction: sld_bouope.f90), FOLLOWER_LOADS: ON, and FUNCTIONS
+             (DISCRETE[, SMOOTH] shape tables, sld_funbou.f90)
 
 A node's codes are the codes of the boundaries whose faces contain it
 (Alya's boundary-condition extrapolation to nodes); a CODES, NODES rule
@@ -117,6 +122,19 @@ def read_case(dat_path):
     for l in _block(dom, "BOUNDARIES", "END_BOUNDARIES"):
         v = l.split()
         bounds[int(v[0])] = [int(x) for x in v[1:5]]          # quad face nodes (element id follows)
+    fields = {}
+    fb = _block(dom, "FIELDS", "END_FIELDS")
+    fid = None
+    for l in fb:
+        m = re.match(r"FIELD\s*=\s*(\d+)", l, re.I)
+        if m:
+            fid = int(m.group(1))
+            fields[fid] = {}
+        elif l.upper().startswith("END_FIELD"):
+            fid = None
+        elif fid is not None:
+            v = l.split()
+            fields[fid][int(v[0])] = [_num(x) for x in v[1:]]
     mats = {}
     for l in _block(dom, "MATERIALS", "END_MATERIALS"):
         v = l.split()
@@ -139,6 +157,9 @@ def read_case(dat_path):
         if b in bcode:
             for n in nodes:
                 node_codes[nidx[n]].add(bcode[b])
+    for l in _block(dom, "ON_NODES", "END_ON_NODES"):
+        v = l.split()
+        node_codes[nidx[int(v[0])]].add(int(v[1]))
 
     # ---- node sets (.set.dat)
     node_sets = {}
@@ -149,8 +170,12 @@ def read_case(dat_path):
 
     # ---- materials (.sld.dat)
     props, rho, law = {}, {}, {}
+    force_field = None
     cur = 1
     for l in _block(sld, "PROPERTIES", "END_PROPERTIES"):
+        m = re.match(r"FORCE\s*:\s*FIELD\s*=\s*(\d+)", l, re.I)
+        if m:
+            force_field = int(m.group(1))
         m = re.match(r"MATERIAL\s*=\s*(\d+)", l, re.I)
         if m:
             cur = int(m.group(1))
@@ -229,6 +254,64 @@ def read_case(dat_path):
         fnpts[fidx[f]] = len(rows)
         fsmooth[fidx[f]] = sm
 
+    # ---- nodal forces (FORCE: FIELD): f_ext = field * cutim / (t1 - t0),
+    # a linear function through (t0, t0/(t1-t0)) and (t1, t1/(t1-t0)), whose
+    # continuation past t1 (plast cutim / tlast) is the same line
+    fext0, fext_func = None, -1
+    if force_field is not None:
+        if force_field not in fields:
+            raise ValueError(f"{name}: FORCE: FIELD={force_field} has no FIELDS entry")
+        if t1 is None:
+            raise ValueError(f"{name}: TIME_INTERVAL missing")
+        fext0 = np.zeros((len(node_ids), 3))
+        for n, vals in fields[force_field].items():
+            fext0[nidx[n], :len(vals[:3])] = vals[:3]
+        span = t1 - t0
+        ramp = np.zeros((1, max(2, ftab.shape[1]), 4))
+        ramp[0, 0] = [t0 / span, 0.0, 0.0, t0]
+        ramp[0, 1] = [t1 / span, 0.0, 0.0, t1]
+        if ftab.shape[1] < 2:
+            ftab = np.concatenate([ftab, np.zeros((ftab.shape[0], 2 - ftab.shape[1], 4))], axis=1)
+        if fids:
+            ftab = np.concatenate([ftab, ramp], axis=0)
+            fnpts = np.append(fnpts, np.int32(2))
+            fsmooth = np.append(fsmooth, np.int32(0))
+        else:
+            ftab, fnpts, fsmooth = ramp, np.array([2], np.int32), np.array([0], np.int32)
+        fext_func = ftab.shape[0] - 1
+
+    # ---- boundary-condition header: transient unless CONSTANT
+    transient = True
+    for l in sld:
+        u = l.upper().replace(" ", "")
+        if u.startswith("BOUNDARY_CONDITIONS"):
+            transient = "CONST" not in u
+            break
+    follower = any(re.match(r"FOLLOWER_LOADS\s*[:=]?\s*ON", l, re.I) for l in sld)
+
+    # ---- CODES, BOUNDARIES rules -> Neumann face loads
+    face_rules = {}
+    for l in _block(sld, "CODES, BOUNDARIES", "END_CODES") or _block(sld, "CODES,BOUNDARIES", "END_CODES"):
+        head = l.split(",")[0].split()
+        if len(head) < 2:
+            continue
+        code, fixbo = int(head[0]), int(head[1])
+        if fixbo not in (3, 6):
+            raise ValueError(f"{name}: boundary condition {fixbo} on code {code} is not on this path "
+                             "(3 pressure and 6 traction are)")
+        vals = [_num(x) for x in head[2:5]] + [0.0] * (5 - len(head))
+        mf = re.search(r"MODULE_FUNCTION\s*=\s*(\d+)", l, re.I)
+        face_rules[code] = (fixbo, vals[:3], fidx[int(mf.group(1))] if mf else -1)
+    faces = None
+    if face_rules:
+        sel = [b for b in sorted(bounds) if bcode.get(b) in face_rules]
+        lnodb = np.array([[nidx[n] for n in bounds[b]] for b in sel], dtype=np.int32).reshape(-1, 4)
+        rules = [face_rules[bcode[b]] for b in sel]
+        faces = dict(lnodb=lnodb, code=np.array([r[0] for r in rules], np.int32),
+                     value=np.array([r[1] for r in rules], np.float64).reshape(-1, 3),
+                     func=np.array([r[2] for r in rules], np.int32),
+                     transient=transient, follower=follower)
+
     # ---- CODES, NODES rules -> fixity and load functions
     fix = np.zeros(len(node_ids), dtype=np.uint8)
     func_of_node = np.full(len(node_ids), -1, dtype=np.int32)
@@ -254,7 +337,9 @@ def read_case(dat_path):
                 rho=np.array([rho.get(m, 1.0) for m in mat_ids]), mat_of_elem=mat_of_elem,
                 law_of_mat=np.array([law[m] for m in mat_ids], np.int32), fixmask=fix,
                 func_of_node=func_of_node, func_table=ftab, func_npts=fnpts, func_smooth=fsmooth,
-                fext0=None, fext_func=-1)
+                fext0=fext0, fext_func=fext_func)
+    if faces is not None:
+        prob["faces"] = faces
     prob["run"] = dict(t0=t0, t_end=t1, max_steps=nmax, scheme=scheme, phi=phi, safety=safety)
     prob["node_sets"] = {k: np.array(v, np.int64) for k, v in sorted(node_sets.items())}
     prob["node_codes"] = node_codes

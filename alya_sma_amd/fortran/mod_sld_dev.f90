@@ -32,6 +32,14 @@ module mod_sld_dev
      integer(c_int32_t) :: fext_func
   end type sld_bc_desc
 
+  ! Neumann loads on QUA04 boundary faces (sld_bouope.f90): code 3 pressure,
+  ! 6 traction; lnodb 0-based node indices
+  type, bind(C) :: sld_face_desc
+     integer(c_int64_t) :: nface
+     type(c_ptr)        :: lnodb, code, value, func
+     integer(c_int32_t) :: transient, follower
+  end type sld_face_desc
+
   interface
      function sma_last_error() bind(C, name="sma_last_error") result(p)
        import :: c_ptr
@@ -62,6 +70,13 @@ module mod_sld_dev
        integer(c_int32_t), value :: mode
        integer(c_int) :: rc
      end function sld_dev_set_mode
+
+     function sld_dev_set_faces(dev, faces) bind(C, name="sld_dev_set_faces") result(rc)
+       import :: c_int, c_ptr, sld_face_desc
+       type(c_ptr), value :: dev
+       type(sld_face_desc), intent(in) :: faces
+       integer(c_int) :: rc
+     end function sld_dev_set_faces
 
      function sld_dev_set_energy(dev, on) bind(C, name="sld_dev_set_energy") result(rc)
        import :: c_int, c_int32_t, c_ptr

@@ -62,6 +62,40 @@ def smooth_table(t_end, values):
     return tab
 
 
+# HEX08 faces as QUA04 node quadruples (local node order of the element:
+# 1-4 on zeta = -1, 5-8 on zeta = +1, counter-clockwise)
+HEX8_FACES = np.array([[0, 3, 2, 1], [4, 5, 6, 7], [0, 1, 5, 4],
+                       [1, 2, 6, 5], [2, 3, 7, 6], [0, 4, 7, 3]], dtype=np.int64)
+
+
+def boundary_faces(coord, lnods, select=None):
+    """QUA04 boundary faces of a HEX08 mesh: faces that belong to one
+    element only, each ordered so that t_s x t_t (the normal of the QUA04
+    map, node 1 at (s, t) = (-1, -1), counter-clockwise) points out of its
+    element.  select(centre [nf, 3]) -> bool mask keeps a subset.
+    Returns (lnodb [nf, 4] int32, elem [nf] int32)."""
+    lnods = np.asarray(lnods, dtype=np.int64)
+    ne = lnods.shape[0]
+    faces = lnods[:, HEX8_FACES].reshape(-1, 4)               # [6 ne, 4]
+    key = np.sort(faces, axis=1)
+    _, inv, cnt = np.unique(key, axis=0, return_inverse=True, return_counts=True)
+    keep = cnt[inv.ravel()] == 1
+    fb = faces[keep]
+    el = np.repeat(np.arange(ne), 6)[keep]
+    x = coord[fb]                                              # [nf, 4, 3]
+    centre = x.mean(axis=1)
+    if select is not None:
+        m = np.asarray(select(centre), dtype=bool)
+        fb, el, x, centre = fb[m], el[m], x[m], centre[m]
+    # t_s x t_t at the face centre vs the direction out of the element
+    ts = 0.25 * (-x[:, 0] + x[:, 1] + x[:, 2] - x[:, 3])
+    tt = 0.25 * (-x[:, 0] - x[:, 1] + x[:, 2] + x[:, 3])
+    out = centre - coord[lnods[el]].mean(axis=1)
+    flip = np.einsum("ij,ij->i", np.cross(ts, tt), out) < 0.0
+    fb[flip] = fb[flip][:, ::-1]
+    return fb.astype(np.int32), el.astype(np.int32)
+
+
 def bar_problem(nx=10, ny=2, nz=2, lx=10.0, ly=2.0, lz=2.0, strain=0.06,
                 nsteps=2000, safety=0.5, props=NITINOL, rho=RHO_NITINOL, use_lib=False):
     """Uniaxial SMA bar (displacement-controlled ramp to `strain`, then unload).

@@ -96,6 +96,8 @@ class Solidz:
               "sld_dev_create")
         self.h = h
         self._keep = []
+        if prob.get("faces") is not None:
+            self.set_faces(prob["faces"])
 
     def close(self):
         if getattr(self, "h", None):
@@ -137,6 +139,30 @@ class Solidz:
         per-node sum in the corrector); both give bitwise equal results."""
         a = {"scatter": 0, "gather": 1}.get(assembly, assembly)
         check(lib().sld_dev_set_assembly(self.h, int(a)), "sld_dev_set_assembly")
+
+    def set_faces(self, faces):
+        """Neumann loads on boundary faces: dict(lnodb [nf, 4], code [nf]
+        (3 pressure, 6 traction), value [nf, 3], func [nf] or None,
+        transient, follower); None or nf = 0 removes them."""
+        keep = []
+
+        def arr(x, dt):
+            if x is None:
+                return None
+            a = np.ascontiguousarray(x, dtype=dt)
+            keep.append(a)
+            return a
+
+        if faces is None or len(faces["code"]) == 0:
+            desc = _lib.SldFaceDesc(0, None, None, None, None, 0, 0)
+        else:
+            desc = _lib.SldFaceDesc(len(faces["code"]), _p(arr(faces["lnodb"], np.int32)),
+                                    _p(arr(faces["code"], np.int32)),
+                                    _p(arr(faces["value"], np.float64)),
+                                    _p(arr(faces.get("func"), np.int32)),
+                                    int(bool(faces.get("transient", False))),
+                                    int(bool(faces.get("follower", False))))
+        check(lib().sld_dev_set_faces(self.h, C.byref(desc)), "sld_dev_set_faces")
 
     def set_energy(self, on=True):
         """Start (True) a fresh energy balance or stop (False) accumulating."""

@@ -193,12 +193,40 @@ int sld_dev_set_tile(sld_dev *dev, int32_t elems_per_block, int32_t xcd_remap, i
 #define SLD_ASSEMBLY_GATHER  1
 int sld_dev_set_assembly(sld_dev *dev, int32_t assembly);
 
+/* Neumann loads on boundary faces (replaces sld_bouope,
+ * sld_bouope.f90:19-386, with the per-step boundary values of
+ * sld_updbcs.f90:153-181).  Faces are QUA04 (4 nodes in the order of the
+ * reference's shape2: (s,t) = (-1,-1), (1,-1), (1,1), (-1,1)); each is
+ * integrated at 2x2 Gauss points (+-0.577350269189626, weight 1).  At
+ * t^{n+1} a face carries
+ *   code 3 (pressure):  traction = val[0] n, n = t_s x t_t / |t_s x t_t|
+ *   code 6 (traction):  traction = val[0..2]
+ * with val = value[f][0..2] under constant boundary conditions
+ * (transient = 0), else (column x of function func[f] at t^{n+1}, 0, 0), or 0
+ * where func is NULL or -1 (sld_funbou.f90 itask 1: the reference's
+ * transient boundary values).  follower = 1 integrates over the current
+ * faces X + u^{n+1} (FOLLOWER_LOADS: ON, kfl_follo_sld), else over the
+ * reference faces.  Loads enter f_ext of dofs that are not prescribed;
+ * the corrector adds each node's face corners in ascending face order.
+ * nface = 0 removes the faces.  Functions are those of sld_bc_desc. */
+typedef struct {
+  int64_t nface;
+  const int32_t *lnodb;        /* [nface][4] node indices (0-based) */
+  const int32_t *code;         /* [nface] 3 or 6 (kfl_fixbo_sld) */
+  const double *value;         /* [nface][3] bvnat_sld */
+  const int32_t *func;         /* [nface] function index or -1 (kfl_funbo_sld), or NULL */
+  int32_t transient;           /* 1: BOUNDARY_CONDITIONS, TRANSIENT (kfl_conbc_sld = 0) */
+  int32_t follower;            /* 1: FOLLOWER_LOADS: ON */
+} sld_face_desc;
+int sld_dev_set_faces(sld_dev *dev, const sld_face_desc *faces);
+
 /* Energy balance of the explicit run (replaces sld_energy,
  * mod_sld_energy.f90:71-162, called from sld_endste.f90): on = 1 starts a
  * fresh balance (allocated on first use), on = 0 stops accumulating.  Each
  * step adds the trapezoidal work of the internal and external forces over
  * the displacement increment (full weight on the first step of a balance,
- * half afterwards, absolute value of each step's sum) and sets the kinetic
+ * half afterwards, absolute value of each step's sum; the previous step's
+ * forces are zero at the start, as fintt/fextt_sld(TIME_N)) and sets the kinetic
  * energy 0.5 sum m v.v from the corrected velocity.  Sums are fixed-order:
  * repeat runs are bitwise equal.  out[5] = {allie, allwk, allke, etota =
  * allie + allke - allwk, steps accumulated}. */

@@ -99,7 +99,19 @@ typedef struct {
     const int32_t *color_ptr;  /* [ncolor+1] ranges into elem_order or NULL  */
     int32_t ncolor;
     const int32_t *law_of_mat; /* [nmat] 160 SMA, 102 neo-Hookean; NULL = 160 */
+    /* Neumann loads on QUA04 boundary faces (sld_bouope.f90) */
+    int64_t nface;
+    const int32_t *lnodb;      /* [nface][4] 0-based                         */
+    const int32_t *face_code;  /* [nface] 3 pressure along the normal, 6 traction */
+    const double  *face_value; /* [nface][3] constant values (bvnat)         */
+    const int32_t *face_func;  /* [nface] function id or -1; NULL = none     */
+    int32_t face_transient;    /* 1: value = (function column x, 0, 0) or 0  */
+    int32_t face_follower;     /* 1: integrate over X + u^{n+1}              */
 } orc_model;
+
+/* Nodal forces of the boundary faces at time t over the geometry
+ * coord (+ u when follower): ff[npoin][3], overwritten. */
+void orc_face_loads(const orc_model *m, const double *u, double t, double *ff);
 
 /* Neo-Hookean (Ansys) PK2 at one point: props[0] = mu, props[1] = K,
  * F column-major, J = det F (sld_stress_model_102.f90:60-119). */
@@ -118,6 +130,7 @@ typedef struct {
     /* energy balance (mod_sld_energy.f90:71-162); accumulated when
      * fint_prev != NULL ([npoin][3], f_int of the previous step) */
     double *fint_prev;
+    double *fext_prev;         /* [npoin][3] f_ext of the previous step      */
     double allie, allwk, allke, etota;
 } orc_state;
 

@@ -50,6 +50,7 @@ def lib():
         _lib.orc_sld_step.argtypes = [p, p, d, C.c_int, d]
         _lib.orc_funbou.argtypes = [p, C.c_int32, C.c_int, d]
         _lib.orc_funbou.restype = d
+        _lib.orc_face_loads.argtypes = [p, p, d, p]
     return _lib
 
 
@@ -135,7 +136,10 @@ class _Model(C.Structure):
                 ("func_table", C.c_void_p), ("func_maxpts", C.c_int32),
                 ("fext0", C.c_void_p), ("fext_func", C.c_int32),
                 ("elem_order", C.c_void_p), ("color_ptr", C.c_void_p), ("ncolor", C.c_int32),
-                ("law_of_mat", C.c_void_p)]
+                ("law_of_mat", C.c_void_p),
+                ("nface", C.c_int64), ("lnodb", C.c_void_p), ("face_code", C.c_void_p),
+                ("face_value", C.c_void_p), ("face_func", C.c_void_p),
+                ("face_transient", C.c_int32), ("face_follower", C.c_int32)]
 
 
 class _State(C.Structure):
@@ -143,7 +147,8 @@ class _State(C.Structure):
                 ("fint", C.c_void_p), ("rhs", C.c_void_p), ("mass", C.c_void_p),
                 ("sdv", C.c_void_p), ("S", C.c_void_p), ("flags", C.c_void_p),
                 ("t", C.c_double), ("nsteps", C.c_int64),
-                ("fint_prev", C.c_void_p), ("allie", C.c_double), ("allwk", C.c_double),
+                ("fint_prev", C.c_void_p), ("fext_prev", C.c_void_p),
+                ("allie", C.c_double), ("allwk", C.c_double),
                 ("allke", C.c_double), ("etota", C.c_double)]
 
 
@@ -188,6 +193,15 @@ class StepOracle:
         m.color_ptr = _ptr(arr(color_ptr, np.int32))
         m.ncolor = 0 if color_ptr is None else len(color_ptr) - 1
         m.law_of_mat = _ptr(arr(prob.get("law_of_mat"), np.int32))
+        fc = prob.get("faces")
+        if fc is not None and len(fc["code"]):
+            m.nface = len(fc["code"])
+            m.lnodb = _ptr(arr(fc["lnodb"], np.int32))
+            m.face_code = _ptr(arr(fc["code"], np.int32))
+            m.face_value = _ptr(arr(fc["value"], np.float64))
+            m.face_func = _ptr(arr(fc.get("func"), np.int32))
+            m.face_transient = int(bool(fc.get("transient", False)))
+            m.face_follower = int(bool(fc.get("follower", False)))
         self.m = m
         n, e = self.npoin, self.nelem
         self.u = np.zeros((n, 3)); self.v = np.zeros((n, 3)); self.a = np.zeros((n, 3))
@@ -201,7 +215,9 @@ class StepOracle:
         s.sdv, s.S, s.flags = _ptr(self.sdv), _ptr(self.S), _ptr(self.flags)
         s.t, s.nsteps = 0.0, 0
         self.fint_prev = np.zeros((n, 3)) if energy else None
+        self.fext_prev = np.zeros((n, 3)) if energy else None
         s.fint_prev = _ptr(self.fint_prev)
+        s.fext_prev = _ptr(self.fext_prev)
         self.s = s
         lib().orc_sld_mass(C.byref(self.m), C.byref(self.s))
 
@@ -222,6 +238,14 @@ class StepOracle:
 
     def funbou(self, f, c, t):
         return lib().orc_funbou(C.byref(self.m), f, c, t)
+
+    def face_loads(self, u, t):
+        """Nodal forces [npoin, 3] of the boundary faces at time t with
+        displacement u (used by follower loads)."""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        ff = np.zeros((self.npoin, 3))
+        lib().orc_face_loads(C.byref(self.m), _ptr(u), t, _ptr(ff))
+        return ff
 
 
 # ------------------------------------------------------- reference harnesses

@@ -132,6 +132,74 @@ double orc_funbou(const orc_model *m, int32_t f, int c, double t)
     return x;
 }
 
+/* Boundary faces (sld_bouope.f90:88-277): per face, the geometry bocod of
+ * its 4 nodes (reference, or current with follower loads), the value of the
+ * step (sld_updbcs.f90:153-181: constant, or under transient conditions the
+ * function's column x in component 1 and zero elsewhere, sld_funbou.f90
+ * itask 1), then at each of the 2x2 Gauss points (mod_quadrature_gauss.f90:
+ * 2361-2370, s outer) the tangents t_s, t_t of bouder (mod_bouder.f90:74-89),
+ * n = t_s x t_t, eucta = |n|, the unit normal of chenor (chenor.f90, the
+ * normalisation; its orientation test is unreachable), the traction (code 3:
+ * value(1) n, code 6: value) and elfex += w eucta tract N_a on dofs that are
+ * not prescribed.  QUA04 shape functions of mod_lagrange.f90 shape2. */
+void orc_face_loads(const orc_model *m, const double *u, double t, double *ff)
+{
+    const double g = 0.577350269189626;
+    const double gp[4][2] = {{-g, -g}, {-g, g}, {g, -g}, {g, g}};
+    memset(ff, 0, sizeof(double) * 3 * m->npoin);
+    for (int64_t f = 0; f < m->nface; f++) {
+        const int32_t *nb = m->lnodb + 4 * f;
+        double x[4][3], val[3], elfex[4][3] = {{0}};
+        for (int a = 0; a < 4; a++)
+            for (int d = 0; d < 3; d++) {
+                x[a][d] = m->coord[3 * nb[a] + d];
+                if (m->face_follower) x[a][d] = x[a][d] + u[3 * nb[a] + d];
+            }
+        int fn = m->face_func ? m->face_func[f] : -1;
+        if (m->face_transient) {
+            val[0] = fn >= 0 ? orc_funbou(m, fn, 0, t) : 0.0;
+            val[1] = 0.0;
+            val[2] = 0.0;
+        } else {
+            for (int d = 0; d < 3; d++) val[d] = m->face_value[3 * f + d];
+        }
+        for (int ig = 0; ig < 4; ig++) {
+            double s = gp[ig][0], tt = gp[ig][1], st = s * tt;
+            double N[4] = {(1.0 - tt - s + st) * 0.25, (1.0 - tt + s - st) * 0.25,
+                           (1.0 + tt + s + st) * 0.25, (1.0 + tt - s - st) * 0.25};
+            double ds[4] = {(-1.0 + tt) * 0.25, (1.0 - tt) * 0.25, (1.0 + tt) * 0.25,
+                            (-1.0 - tt) * 0.25};
+            double dt[4] = {(-1.0 + s) * 0.25, (-1.0 - s) * 0.25, (1.0 + s) * 0.25,
+                            (1.0 - s) * 0.25};
+            double t1[3], t2[3], n[3];
+            for (int d = 0; d < 3; d++) {
+                t1[d] = 0.0;
+                t2[d] = 0.0;
+                for (int a = 0; a < 4; a++) {
+                    t1[d] = t1[d] + x[a][d] * ds[a];
+                    t2[d] = t2[d] + x[a][d] * dt[a];
+                }
+            }
+            n[0] = t1[1] * t2[2] - t1[2] * t2[1];
+            n[1] = t1[2] * t2[0] - t1[0] * t2[2];
+            n[2] = t1[0] * t2[1] - t1[1] * t2[0];
+            double eucta = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+            double gbsur = 1.0 * eucta, tract[3];
+            if (m->face_code[f] == 3) {
+                for (int d = 0; d < 3; d++) tract[d] = (n[d] / eucta) * val[0];
+            } else {
+                for (int d = 0; d < 3; d++) tract[d] = val[d];
+            }
+            for (int a = 0; a < 4; a++)
+                for (int d = 0; d < 3; d++)
+                    if (!((m->fixmask[nb[a]] >> d) & 1))
+                        elfex[a][d] = elfex[a][d] + gbsur * tract[d] * N[a];
+        }
+        for (int a = 0; a < 4; a++)
+            for (int d = 0; d < 3; d++) ff[3 * nb[a] + d] = ff[3 * nb[a] + d] + elfex[a][d];
+    }
+}
+
 /* Element loop: fint, rhs and the new history/stress for every element. */
 static void element_loop(const orc_model *m, orc_state *s)
 {
@@ -204,11 +272,8 @@ int orc_sld_step(const orc_model *m, orc_state *s, double dt, int scheme, double
     int64_t np = m->npoin;
     double cutim = s->t + dt;             /* setgts.f90:98 */
     double t_end = cutim + dt, t_mid = 0.5 * (cutim + t_end);
-    double fx = 0.0, fx_n = 0.0;
-    if (m->fext0) {
-        fx = orc_funbou(m, m->fext_func, 0, cutim);
-        fx_n = orc_funbou(m, m->fext_func, 0, s->t);
-    }
+    double fx = 0.0;
+    if (m->fext0) fx = orc_funbou(m, m->fext_func, 0, cutim);
     double *u_old = NULL;
     if (s->fint_prev) {
         u_old = (double *)malloc(sizeof(double) * 3 * np);
@@ -239,9 +304,14 @@ int orc_sld_step(const orc_model *m, orc_state *s, double dt, int scheme, double
         }
     }
     element_loop(m, s);
+    /* f_ext = face loads (sld_bouope, assembled after the element loop in
+     * sld_matrix.f90:221) + the nodal force pattern */
+    double *fe = (double *)calloc((size_t)3 * np, sizeof(double));
+    if (m->nface > 0) orc_face_loads(m, s->u, cutim, fe);
     if (m->fext0)
         for (int64_t p = 0; p < np; p++)
-            for (int d = 0; d < 3; d++) s->rhs[3 * p + d] += fx * m->fext0[3 * p + d];
+            for (int d = 0; d < 3; d++) fe[3 * p + d] = fe[3 * p + d] + fx * m->fext0[3 * p + d];
+    for (int64_t p = 0; p < 3 * np; p++) s->rhs[p] = s->rhs[p] + fe[p];
     /* solve + corrector */
     for (int64_t p = 0; p < np; p++) {
         double fact1 = 1.0 / s->mass[p];
@@ -266,7 +336,7 @@ int orc_sld_step(const orc_model *m, orc_state *s, double dt, int scheme, double
             for (int d = 0; d < 3; d++) {
                 double du = s->u[3 * p + d] - u_old[3 * p + d];
                 ie = ie + (s->fint[3 * p + d] + s->fint_prev[3 * p + d]) * du;
-                if (m->fext0) wk = wk + (fx * m->fext0[3 * p + d] + fx_n * m->fext0[3 * p + d]) * du;
+                wk = wk + (fe[3 * p + d] + s->fext_prev[3 * p + d]) * du;
                 ke = ke + s->v[3 * p + d] * s->v[3 * p + d] * s->mass[p];
             }
         double w = s->nsteps == 0 ? 1.0 : 0.5;
@@ -275,8 +345,10 @@ int orc_sld_step(const orc_model *m, orc_state *s, double dt, int scheme, double
         s->allke = 0.5 * ke;
         s->etota = s->allie + s->allke - s->allwk;
         memcpy(s->fint_prev, s->fint, sizeof(double) * 3 * np);
+        memcpy(s->fext_prev, fe, sizeof(double) * 3 * np);
         free(u_old);
     }
+    free(fe);
     /* sld_updunk ENDSTE: TIME_N <- ITER_K */
     memcpy(s->sdv + 24 * m->nelem, s->sdv, sizeof(double) * 24 * m->nelem);
     s->t = cutim;

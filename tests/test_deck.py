@@ -52,3 +52,37 @@ def test_deck_run_by_the_oracle_cycles_through_transformation():
     assert xs[:10].max() > 0.02 and np.argmax(xs[:12] > 0) >= 6
     assert 73.0 < sx[:12].max() < 122.5
     assert xs[-1] == 0.0 and abs(sx[-1]) < 1.0
+
+
+def test_deck_neumann_keywords(tmp_path):
+    """A deck of our own with CODES, BOUNDARIES (fixbo 3 + MODULE_FUNCTION),
+    FOLLOWER_LOADS: ON under TRANSIENT conditions and FORCE: FIELD: the
+    faces, codes and values read are those written, and the nodal-force
+    ramp equals cutim / (t1 - t0) (sld_matrix.f90:249-262)."""
+    from _decks import LZ, PRESSURE, T1, TIP_FORCE, write_beam_deck
+    from oracle import oracle_py as O
+    path, ref = write_beam_deck(str(tmp_path))
+    p = deck.read_case(path)
+    f = p["faces"]
+    top = ref["code"] == 2
+    np.testing.assert_array_equal(f["lnodb"], ref["faces"][top])
+    assert f["code"].tolist() == [3] * int(top.sum())
+    assert f["func"].tolist() == [0] * int(top.sum())
+    assert f["transient"] and f["follower"]
+    # the clamped end from the face codes, nothing else fixed
+    clamped = np.abs(ref["coord"][:, 0]) < 1e-9
+    assert (p["fixmask"][clamped] == 7).all() and (p["fixmask"][~clamped] == 0).all()
+    fe = np.zeros_like(ref["coord"])
+    fe[ref["tip"]] = TIP_FORCE
+    np.testing.assert_array_equal(p["fext0"], fe)
+    o = O.StepOracle(p)
+    for t in (0.0, 0.3 * T1, T1, 1.7 * T1):
+        assert abs(o.funbou(p["fext_func"], 0, t) - t / T1) <= 1e-15 * max(t / T1, 1.0)
+    assert abs(o.funbou(0, 0, T1 / 2) - PRESSURE) < 1e-12
+    # the top faces' normal is +z: a negative pressure pushes down, on the
+    # 4 x 1 mm top minus the four corner shares (0.25 mm^2 / 4 each) that sit
+    # on the clamped edge, whose dofs take no load
+    ff = o.face_loads(np.zeros_like(ref["coord"]), T1 / 2)
+    np.testing.assert_allclose(ff.sum(axis=0), [0.0, 0.0, PRESSURE * (4.0 - 0.25)], atol=1e-12)
+    assert (ff[clamped] == 0.0).all()
+    assert (ff[np.abs(ref["coord"][:, 2] - LZ) > 1e-9] == 0.0).all()

@@ -43,3 +43,30 @@ def test_deck_on_gpu_matches_oracle(tmp_path):
     last = rows[-2:]
     assert np.all(np.isfinite(last)) and np.all(np.abs(last[:, 3]) < 1e-12)
     print(f"deck: {n} TW steps of {dt:.3e} s, {k} output rows per set", flush=True)
+
+
+def test_neumann_deck_on_gpu_matches_oracle(tmp_path):
+    """A deck of our own with a follower pressure on boundary faces
+    (CODES, BOUNDARIES fixbo 3, TRANSIENT, FOLLOWER_LOADS: ON) and nodal
+    forces (FORCE: FIELD) run by the deck runner on the GPU against the
+    oracle's node-set history over the whole run."""
+    from _decks import write_beam_deck
+    case, _ = write_beam_deck(str(tmp_path))
+    print("deck beam (faces + nodal force) on the GPU", flush=True)
+    path, prob, dt, n = deck.run_case(case, out_dir=str(tmp_path), every=200)
+    rows = np.loadtxt(path).reshape(-1, 9)
+    o = O.StepOracle(prob)
+    done, k, moved = 0, 0, 0.0
+    while done < n:
+        m = min(200, n - done)
+        o.step(dt, scheme=1, phi=prob["run"]["phi"], n=m)
+        done += m
+        row = rows[k]
+        nodes = prob["node_sets"][1]
+        assert row[0] == done
+        ref = o.u[nodes].mean(axis=0)
+        moved = max(moved, np.abs(ref).max())
+        np.testing.assert_allclose(row[3:6], ref, rtol=0, atol=1e-10 * max(moved, 1e-30))
+        k += 1
+    assert moved > 1e-3
+    print(f"beam deck: {n} TW steps, tip deflection up to {moved:.3e} mm", flush=True)
