@@ -43,13 +43,16 @@ unsigned nblocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 }  // namespace
 
+// Steps per captured graph of a long run (even: see sld_dev_step).
+constexpr int GRAPH_RUN = 16;
+
 struct GraphKey {
   double dt, phi;
-  int scheme, mode, store, epb, xcd, persistent, assembly, tn, energy;
+  int scheme, mode, store, epb, xcd, persistent, assembly, tn, energy, len;
   bool operator<(const GraphKey& o) const {
-    return std::tie(dt, phi, scheme, mode, store, epb, xcd, persistent, assembly, tn, energy) <
+    return std::tie(dt, phi, scheme, mode, store, epb, xcd, persistent, assembly, tn, energy, len) <
            std::tie(o.dt, o.phi, o.scheme, o.mode, o.store, o.epb, o.xcd, o.persistent,
-                    o.assembly, o.tn, o.energy);
+                    o.assembly, o.tn, o.energy, o.len);
   }
 };
 
@@ -279,15 +282,20 @@ struct KernelTimer {
 #define TBEGIN(k) do { if (T) T->mark(k); } while (0)
 #define TEND() do { if (T) T->done(); } while (0)
 
-// One explicit step; tn = history slot holding TIME_N.
-int enqueue_step(sld_dev* d, double dt, int scheme, double phi, int tn, KernelTimer* T = nullptr) {
+// One explicit step; tn = history slot holding TIME_N.  predict = false:
+// the previous step's corrector already ran this step's predictor (fused);
+// fuse = true: this step's corrector runs the next step's predictor.
+int enqueue_step(sld_dev* d, double dt, int scheme, double phi, int tn, KernelTimer* T = nullptr,
+                 bool predict = true, bool fuse = false) {
   hipStream_t s = d->stream;
   sld::NodalArgs N = nodal_args(d, tn);
   const bool gather = d->assembly == SLD_ASSEMBLY_GATHER;
   const unsigned nb = nblocks(d->npoin, 256);
   TBEGIN(0);
-  if (scheme == 0) hipLaunchKernelGGL((sld::sld_predict<0>), dim3(nb), dim3(256), 0, s, N, dt, phi);
-  else hipLaunchKernelGGL((sld::sld_predict<1>), dim3(nb), dim3(256), 0, s, N, dt, phi);
+  if (predict) {
+    if (scheme == 0) hipLaunchKernelGGL((sld::sld_predict<0>), dim3(nb), dim3(256), 0, s, N, dt, phi);
+    else hipLaunchKernelGGL((sld::sld_predict<1>), dim3(nb), dim3(256), 0, s, N, dt, phi);
+  }
   if (d->faces.nface > 0) {
     const sld_dev::Faces& Fc = d->faces;
     sld::FaceArgs F{};
@@ -393,8 +401,13 @@ int enqueue_step(sld_dev* d, double dt, int scheme, double phi, int tn, KernelTi
     TEND();
   }
   TBEGIN(2);
-  if (scheme == 0) hipLaunchKernelGGL((sld::sld_correct<0>), dim3(nb), dim3(256), 0, s, N, dt);
-  else hipLaunchKernelGGL((sld::sld_correct<1>), dim3(nb), dim3(256), 0, s, N, dt);
+  if (fuse) {
+    if (scheme == 0) hipLaunchKernelGGL((sld::sld_correct<0, true>), dim3(nb), dim3(256), 0, s, N, dt, phi);
+    else hipLaunchKernelGGL((sld::sld_correct<1, true>), dim3(nb), dim3(256), 0, s, N, dt, phi);
+  } else {
+    if (scheme == 0) hipLaunchKernelGGL((sld::sld_correct<0, false>), dim3(nb), dim3(256), 0, s, N, dt, phi);
+    else hipLaunchKernelGGL((sld::sld_correct<1, false>), dim3(nb), dim3(256), 0, s, N, dt, phi);
+  }
   if (d->energy)
     hipLaunchKernelGGL(sld::sld_energy_reduce, dim3(1), dim3(256), 0, s, d->energy_part, (int)nb,
                        d->energy_acc);
@@ -907,23 +920,29 @@ int sld_dev_step(sld_dev* d, double dt, int32_t scheme, double phi, int64_t nste
   if (d->mode == SLD_MODE_SURROGATE && !surrogate_attached(d))
     return fail(SMA_ERR_STATE, "surrogate mode without a surrogate");
   int64_t left = nsteps;
-  if (d->use_graph && left >= 2) {
+  // runs of GRAPH_RUN steps, then pairs, replay captured graphs (an even
+  // length returns the history and clock slots to where they started);
+  // inside a run each corrector also predicts the next step
+  for (int len : {GRAPH_RUN, 2}) {
+    if (!d->use_graph || left < len) continue;
     GraphKey key{dt, phi, scheme, d->mode, d->store, d->epb, d->xcd, d->persistent, d->assembly, d->cur,
-                 d->energy};
+                 d->energy, len};
     auto it = d->graphs.find(key);
     if (it == d->graphs.end()) {
       hipGraph_t g = nullptr;
       hipGraphExec_t ge = nullptr;
       HIPCHK(hipStreamBeginCapture(d->stream, hipStreamCaptureModeThreadLocal));
-      int rc = enqueue_step(d, dt, scheme, phi, d->cur);
-      if (!rc) rc = enqueue_step(d, dt, scheme, phi, 1 - d->cur);
+      int rc = 0;
+      for (int k = 0; k < len && !rc; ++k)
+        rc = enqueue_step(d, dt, scheme, phi, (k & 1) ? 1 - d->cur : d->cur, nullptr, k == 0,
+                          k + 1 < len);
       hipError_t ec = hipStreamEndCapture(d->stream, &g);
       if (rc) return rc;
       if (ec != hipSuccess) return fail(SMA_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
       HIPCHK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
       it = d->graphs.emplace(key, std::make_pair(g, ge)).first;
     }
-    for (; left >= 2; left -= 2) HIPCHK(hipGraphLaunch(it->second.second, d->stream));
+    for (; left >= len; left -= len) HIPCHK(hipGraphLaunch(it->second.second, d->stream));
   }
   for (; left > 0; --left) {
     int rc = enqueue_step(d, dt, scheme, phi, d->cur);

@@ -154,15 +154,17 @@ struct FaceArgs {
 
 // Predictor of node i from the step-n state (u, v, a): u^{n+1} with the
 // Dirichlet values at t^{n+1} = cutim, and v^{n+1/2} (CD) or v^{n+1} (TW).
-// scheme: 0 = central difference, 1 = Tchamwa-Wielgosz.
-__device__ __forceinline__ void predict_node(const NodalArgs& A, int64_t i, int scheme, double dt,
-                                             double phi, double cutim, double un[3], double vn[3]) {
+// scheme: 0 = central difference, 1 = Tchamwa-Wielgosz.  uvw = the state,
+// already loaded.
+__device__ __forceinline__ void predict_vals(const NodalArgs& A, int64_t i, int scheme, double dt,
+                                             double phi, double cutim, int fix, const double u0[3],
+                                             const double v0[3], const double a0[3], double un[3],
+                                             double vn[3]) {
 #pragma clang fp contract(off)
   const double t_end = cutim + dt, t_mid = 0.5 * (cutim + t_end);
-  const int fix = A.fixmask[i];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    const double u = A.u[d][i], v = A.v[d][i], a = A.a[d][i];
+    const double u = u0[d], v = v0[d], a = a0[d];
     if (scheme == 0) {
       vn[d] = v + (t_mid - cutim) * a;
       un[d] = u + dt * vn[d];
@@ -180,15 +182,25 @@ __device__ __forceinline__ void predict_node(const NodalArgs& A, int64_t i, int 
   }
 }
 
-template <int SCHEME>  // 0 = central difference, 1 = Tchamwa-Wielgosz
-__global__ void __launch_bounds__(256) sld_predict(NodalArgs A, double dt, double phi) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= A.npoin) return;
-  double un[3], vn[3];
-  predict_node(A, i, SCHEME, dt, phi, A.clock[A.tn] + dt, un, vn);
+__device__ __forceinline__ void predict_node(const NodalArgs& A, int64_t i, int scheme, double dt,
+                                             double phi, double cutim, double un[3], double vn[3]) {
+  double u[3], v[3], a[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    if (A.du[0]) A.du[d][i] = un[d] - A.u[d][i];
+    u[d] = A.u[d][i];
+    v[d] = A.v[d][i];
+    a[d] = A.a[d][i];
+  }
+  predict_vals(A, i, scheme, dt, phi, cutim, A.fixmask[i], u, v, a, un, vn);
+}
+
+// Stores the predicted state (and du for the energy balance); the coloured
+// scatter also clears f_int for the element launches that add into it.
+__device__ __forceinline__ void store_prediction(const NodalArgs& A, int64_t i, const double u0[3],
+                                                 const double un[3], const double vn[3]) {
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    if (A.du[0]) A.du[d][i] = un[d] - u0[d];
     A.u[d][i] = un[d];
     A.v[d][i] = vn[d];
     if (A.zero_fint) {
@@ -198,22 +210,54 @@ __global__ void __launch_bounds__(256) sld_predict(NodalArgs A, double dt, doubl
   }
 }
 
+template <int SCHEME>  // 0 = central difference, 1 = Tchamwa-Wielgosz
+__global__ void __launch_bounds__(256) sld_predict(NodalArgs A, double dt, double phi) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.npoin) return;
+  double u[3], v[3], a[3], un[3], vn[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    u[d] = A.u[d][i];
+    v[d] = A.v[d][i];
+    a[d] = A.a[d][i];
+  }
+  predict_vals(A, i, SCHEME, dt, phi, A.clock[A.tn] + dt, A.fixmask[i], u, v, a, un, vn);
+  store_prediction(A, i, u, un, vn);
+}
+
 // Gather assembly: f_int at node i = 0 + f_(e1) + f_(e2) + ... over the
 // element corners at the node in ascending (colour-sorted) element order,
 // which is the order in which the coloured scatter adds them.
+// A node shared by 8 elements (every interior node of a hex mesh) loads its
+// 24 values at once, in one round trip; others walk their slots.
 template <typename R>
-__device__ __forceinline__ void gather_fint(const NodalArgs& A, int64_t i, double fi[3]) {
+__device__ __forceinline__ void gather_fint(const NodalArgs& A, int64_t i, double fi[3],
+                                            bool store) {
   const int32_t k0 = A.node_ptr[i], k1 = A.node_ptr[i + 1];
-  const R* ef = (const R*)A.efor;
+  const R* ef = (const R*)A.efor + 3 * (int64_t)k0;
   R s0 = R(0), s1 = R(0), s2 = R(0);
-  for (int32_t k = k0; k < k1; ++k) {
-    s0 += ef[3 * (int64_t)k];
-    s1 += ef[3 * (int64_t)k + 1];
-    s2 += ef[3 * (int64_t)k + 2];
+  if (k1 - k0 == 8) {
+    R c[24];
+#pragma unroll
+    for (int j = 0; j < 24; ++j) c[j] = ef[j];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      s0 += c[3 * j];
+      s1 += c[3 * j + 1];
+      s2 += c[3 * j + 2];
+    }
+  } else {
+    for (int32_t k = 0; k < k1 - k0; ++k) {
+      s0 += ef[3 * k];
+      s1 += ef[3 * k + 1];
+      s2 += ef[3 * k + 2];
+    }
   }
-  ((R*)A.fint[0])[i] = s0;
-  ((R*)A.fint[1])[i] = s1;
-  ((R*)A.fint[2])[i] = s2;
+  if (store) {
+    ((R*)A.fint[0])[i] = s0;
+    ((R*)A.fint[1])[i] = s1;
+    ((R*)A.fint[2])[i] = s2;
+  }
   fi[0] = (double)s0;
   fi[1] = (double)s1;
   fi[2] = (double)s2;
@@ -224,7 +268,8 @@ __device__ __forceinline__ void gather_fint(const NodalArgs& A, int64_t i, doubl
 // and m v.v (mod_sld_energy.f90:134-140); f_int^{n+1} and f_ext^{n+1} replace
 // the previous step's.
 __device__ __forceinline__ void node_energy(const NodalArgs& A, int64_t i, const double fg[3],
-                                            const double fe[3], double e[3]) {
+                                            const double fe[3], const double vel[3],
+                                            double e[3]) {
 #pragma clang fp contract(off)
   double ie = 0.0, wk = 0.0, ke = 0.0;
   const double m = A.mass[i];
@@ -234,7 +279,7 @@ __device__ __forceinline__ void node_energy(const NodalArgs& A, int64_t i, const
     ie = ie + (fg[d] + A.fint_prev[d][i]) * du;
     wk = wk + (fe[d] + A.fext_prev[d][i]) * du;
     A.fext_prev[d][i] = fe[d];
-    const double v = A.v[d][i];
+    const double v = vel[d];
     ke = ke + v * v * m;
     A.fint_prev[d][i] = fg[d];
   }
@@ -243,8 +288,12 @@ __device__ __forceinline__ void node_energy(const NodalArgs& A, int64_t i, const
   e[2] = ke;
 }
 
-template <int SCHEME>
-__device__ __forceinline__ void correct_node(const NodalArgs& A, int64_t i, double dt,
+// Corrector of node i.  FUSE (inside a captured run of steps): the
+// corrected a and v stay in registers and the next step's predictor runs
+// on them at t^{n+2} = t^{n+1} + dt, with the same arithmetic as its own
+// kernel; a, v^{n+1} and f_int reach memory only at the last step of the run.
+template <int SCHEME, bool FUSE>
+__device__ __forceinline__ void correct_node(const NodalArgs& A, int64_t i, double dt, double phi,
                                              double e[3]) {
 #pragma clang fp contract(off)
   const double cutim = A.clock[A.tn] + dt;  // t^{n+1}
@@ -274,13 +323,14 @@ __device__ __forceinline__ void correct_node(const NodalArgs& A, int64_t i, doub
   }
   double fg[3];
   if (A.node_ptr) {
-    if (A.fint_fp32) gather_fint<float>(A, i, fg);
-    else gather_fint<double>(A, i, fg);
+    if (A.fint_fp32) gather_fint<float>(A, i, fg, !FUSE);
+    else gather_fint<double>(A, i, fg, !FUSE);
   } else {
 #pragma unroll
     for (int d = 0; d < 3; ++d)
       fg[d] = A.fint_fp32 ? (double)((const float*)A.fint[d])[i] : ((const double*)A.fint[d])[i];
   }
+  double an[3], vn[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     double fi = fg[d];
@@ -288,14 +338,26 @@ __device__ __forceinline__ void correct_node(const NodalArgs& A, int64_t i, doub
     r = r + fe[d];
     double acc = 0.0 + r * rm;
     if (SCHEME == 0) {
-      A.a[d][i] = acc;
-      A.v[d][i] = A.v[d][i] + (t_end - t_mid) * acc;
+      an[d] = acc;
+      vn[d] = A.v[d][i] + (t_end - t_mid) * acc;
     } else {
-      if ((fix >> d) & 1) { A.v[d][i] = 0.0; acc = 0.0; }
-      A.a[d][i] = acc;
+      vn[d] = A.v[d][i];
+      if ((fix >> d) & 1) { vn[d] = 0.0; acc = 0.0; }
+      an[d] = acc;
+    }
+    if (!FUSE) {
+      A.a[d][i] = an[d];
+      if (SCHEME == 0 || ((fix >> d) & 1)) A.v[d][i] = vn[d];
     }
   }
-  if (A.energy_part) node_energy(A, i, fg, fe, e);
+  if (A.energy_part) node_energy(A, i, fg, fe, vn, e);
+  if (FUSE) {
+    double u[3], un[3], vp[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) u[d] = A.u[d][i];
+    predict_vals(A, i, SCHEME, dt, phi, cutim + dt, fix, u, vn, an, un, vp);
+    store_prediction(A, i, u, un, vp);
+  }
 }
 
 // Fixed-order block sum of three values; thread 0 writes part[k*gridDim.x + b].
@@ -318,12 +380,12 @@ __device__ __forceinline__ void block_sum3(double e[3], double* part) {
   }
 }
 
-template <int SCHEME>
-__global__ void __launch_bounds__(256) sld_correct(NodalArgs A, double dt) {
+template <int SCHEME, bool FUSE>
+__global__ void __launch_bounds__(256) sld_correct(NodalArgs A, double dt, double phi) {
 #pragma clang fp contract(off)
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double e[3] = {0.0, 0.0, 0.0};
-  if (i < A.npoin) correct_node<SCHEME>(A, i, dt, e);
+  if (i < A.npoin) correct_node<SCHEME, FUSE>(A, i, dt, phi, e);
   if (A.energy_part) block_sum3(e, A.energy_part);
 }
 
