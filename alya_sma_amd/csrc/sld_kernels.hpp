@@ -59,6 +59,34 @@ __host__ __device__ __forceinline__ R hex_deriv(int a, int g, int k) {
   if (k == 1) return s * f[0] * f[2];
   return s * f[0] * f[1];
 }
+// Walsh-Hadamard form of the same derivatives.  With node signs s_am and GP
+// signs s_gm (bit m of the index: xi = 4, eta = 2, zeta = 1), c the GP
+// coordinate and (m1, m2) the two directions other than k,
+//   dN_a/dxi_k(g) = 1/8 s_ak (1 + c s_a1 s_g1)(1 + c s_a2 s_g2),
+// so sum_a x_a dN_a/dxi_k(g) = 1/8 [V(k) + c s_g1 V(k|m1) + c s_g2 V(k|m2)
+// + c^2 s_g1 s_g2 V(7)] with V(h) = sum_a (prod_{m in h} s_am) x_a, and
+// sum_g Q(g) dN_a/dxi_k(g) = 1/8 s_ak [U0 + c s_a1 U1 + c s_a2 U2
+// + c^2 s_a1 s_a2 U12] with U the sums over g of Q(g), s_g1 Q(g), s_g2 Q(g),
+// s_g1 s_g2 Q(g).  V and U are per element; only the signed weights differ
+// between lanes.
+constexpr double HEX_GPOS = 0.577350269189626;
+// node with sign bits (bx, by, bz) in the reference node order
+__host__ __device__ constexpr int hex_node(int bx, int by, int bz) {
+  return 4 * bz + (by ? 3 - bx : bx);
+}
+// in-place Walsh-Hadamard butterfly over bit m of an 8-slot array: slot
+// b | m becomes the signed difference (+ side minus - side)
+template <typename R>
+__device__ __forceinline__ void wht_stage(R (&v)[8], int m) {
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    if (b & m) continue;
+    const R lo = v[b], hi = v[b | m];
+    v[b] = lo + hi;
+    v[b | m] = hi - lo;
+  }
+}
+
 template <typename R>
 __host__ __device__ __forceinline__ R hex_shape(int a, int g) {
   const double gpos = 0.577350269189626;
@@ -601,9 +629,10 @@ __device__ __forceinline__ int opaque(int x) {
 template <typename R, int EPB, bool STORE, int SUR = 0, bool GATHER = false>
 __global__ void __launch_bounds__(EPB * 8, ElemOccupancy<R>::waves) sld_element(ElemArgs A) {
   __shared__ double s_abs[EPB][8][6];     // gathered absolute x and u
-  __shared__ R s_x[EPB][8][3];            // x_a - x_0
-  __shared__ R s_u[EPB][8][3];            // u_a - u_0
-  __shared__ R s_q[EPB][8][9];            // Q = detJ tau j^-T per Gauss point
+  __shared__ R s_rel[EPB][6][8];          // x_a - x_0 (0..2), u_a - u_0 (3..5)
+  __shared__ R s_v[EPB][8][8];            // V(h) of the 6 components (pad 2)
+  __shared__ R s_q[EPB][9][8];            // Q(3i+k) at each Gauss point
+  __shared__ R s_u4[EPB][9][4];           // U0, U1, U2, U12 of each Q(3i+k)
 
   const int tid = threadIdx.x;
   const int le = tid >> 3, g = tid & 7;
@@ -639,10 +668,20 @@ __global__ void __launch_bounds__(EPB * 8, ElemOccupancy<R>::waves) sld_element(
   __syncthreads();
   if (valid) {
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      s_x[le][g][d] = (R)(s_abs[le][g][d] - s_abs[le][0][d]);
-      s_u[le][g][d] = (R)(s_abs[le][g][3 + d] - s_abs[le][0][3 + d]);
-    }
+    for (int c = 0; c < 6; ++c) s_rel[le][c][g] = (R)(s_abs[le][g][c] - s_abs[le][0][c]);
+  }
+  __syncthreads();
+  // thread c < 6: V(h) of component c, nodes taken in sign-bit order
+  if (valid && g < 6) {
+    const int c = opaque(g);
+    R v[8];
+#pragma unroll
+    for (int b = 0; b < 8; ++b) v[b] = s_rel[le][c][hex_node((b >> 2) & 1, (b >> 1) & 1, b & 1)];
+    wht_stage(v, 4);
+    wht_stage(v, 2);
+    wht_stage(v, 1);
+#pragma unroll
+    for (int h = 0; h < 8; ++h) s_v[le][h][c] = v[h];
   }
   __syncthreads();
 
@@ -650,16 +689,29 @@ __global__ void __launch_bounds__(EPB * 8, ElemOccupancy<R>::waves) sld_element(
   uint32_t fl = 0;
   if (valid) {
     const int gk = opaque(g);
-    R J[9] = {0}, Gu[9] = {0};  // col-major: J(i,k) = J[i + 3k]
-    // node 0 is the origin of the relative coordinates: its terms vanish
+    R J[9], Gu[9];  // col-major: J(i,k) = J[i + 3k]
+    {
+      // signed weights of the lane's GP: c s_gm / 8 and c^2 s_gm s_gn / 8
+      const R c1 = (R)(0.125 * HEX_GPOS), c2 = (R)(0.125 * HEX_GPOS * HEX_GPOS);
+      const R k0 = (R)0.125;
+      const R ax = (gk & 4) ? c1 : -c1, ay = (gk & 2) ? c1 : -c1, az = (gk & 1) ? c1 : -c1;
+      const R byz = ((gk >> 1) ^ gk) & 1 ? -c2 : c2;
+      const R bxz = ((gk >> 2) ^ gk) & 1 ? -c2 : c2;
+      const R bxy = ((gk >> 2) ^ (gk >> 1)) & 1 ? -c2 : c2;
+      const R* V7 = s_v[le][7];
 #pragma unroll
-    for (int a = 1; a < 8; ++a) {
-      const R D[3] = {hex_deriv<R>(a, gk, 0), hex_deriv<R>(a, gk, 1), hex_deriv<R>(a, gk, 2)};
+      for (int k = 0; k < 3; ++k) {
+        const int hk = 4 >> k, h1 = k == 0 ? 2 : 4, h2 = k == 2 ? 2 : 1;
+        const R w1 = k == 0 ? ay : ax, w2 = k == 2 ? ay : az, w3 = k == 0 ? byz : k == 1 ? bxz : bxy;
+        const R* Vk = s_v[le][hk];
+        const R* V1 = s_v[le][hk | h1];
+        const R* V2 = s_v[le][hk | h2];
 #pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        R x = s_x[le][a][i], uu = s_u[le][a][i];
-        J[i] += x * D[0]; J[i + 3] += x * D[1]; J[i + 6] += x * D[2];
-        Gu[i] += uu * D[0]; Gu[i + 3] += uu * D[1]; Gu[i + 6] += uu * D[2];
+        for (int i = 0; i < 6; ++i) {
+          const R val = k0 * Vk[i] + w1 * V1[i] + w2 * V2[i] + w3 * V7[i];
+          if (i < 3) J[i + 3 * k] = val;
+          else Gu[i - 3 + 3 * k] = val;
+        }
       }
     }
     R Ji[9];
@@ -784,7 +836,7 @@ __global__ void __launch_bounds__(EPB * 8, ElemOccupancy<R>::waves) sld_element(
     for (int i = 0; i < 3; ++i)
 #pragma unroll
       for (int k = 0; k < 3; ++k)
-        s_q[le][g][3 * i + k] =
+        s_q[le][3 * i + k][g] =
             detJ * (T[i] * jci[k] + T[i + 3] * jci[k + 3] + T[i + 6] * jci[k + 6]);
 
     if constexpr (STORE) {
@@ -801,20 +853,52 @@ __global__ void __launch_bounds__(EPB * 8, ElemOccupancy<R>::waves) sld_element(
   if constexpr (SUR == 1) continue;   // uniform: the prepass has no scatter
   count_flags(A.counters, fl, valid);
   __syncthreads();
+  // thread g: U of Q(j) for j = g (and j = 8 on thread 0): pair sums over the
+  // k-sign, then the two-direction Hadamard over (m1, m2)
+  if (valid) {
+    for (int j = opaque(g); j < 9; j += 8) {
+      const int k = j % 3;
+      const int hk = 4 >> k, h1 = k == 0 ? 2 : 4, h2 = k == 2 ? 2 : 1;
+      R p[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        // q = 2 b1 + b2: the GPs with m1-sign b1 and m2-sign b2
+        const int b = ((q >> 1) ? h1 : 0) | ((q & 1) ? h2 : 0);
+        p[q] = s_q[le][j][b] + s_q[le][j][b | hk];
+      }
+      const R s0 = p[0] + p[1], d0 = p[1] - p[0], s1 = p[2] + p[3], d1 = p[3] - p[2];
+      s_u4[le][j][0] = s0 + s1;   // sum
+      s_u4[le][j][1] = s1 - s0;   // m1-signed
+      s_u4[le][j][2] = d0 + d1;   // m2-signed
+      s_u4[le][j][3] = d1 - d0;   // m1 m2-signed
+    }
+  }
+  __syncthreads();
 
-  // -- phase 3: thread g now owns node a = g: f_ai = sum_g' sum_k Q_ik(g') D_ak(g')
+  // -- phase 3: thread g now owns node a = g:
+  // f_ai = sum_k s_ak/8 [U0 + c s_a1 U1 + c s_a2 U2 + c^2 s_a1 s_a2 U12](3i+k)
   if (valid) {
     const int a = opaque(g);
-    using Acc = typename ElemTraits<R>::Acc;
-    Acc f0 = 0, f1 = 0, f2 = 0;
+    const int sx = (0x66 >> a) & 1, sy = (0xCC >> a) & 1, sz = (0xF0 >> a) & 1;
+    const R c0 = (R)0.125, c1 = (R)(0.125 * HEX_GPOS), c2 = (R)(0.125 * HEX_GPOS * HEX_GPOS);
+    R f[3];
 #pragma unroll
-    for (int gg = 0; gg < 8; ++gg) {
-      R D0 = hex_deriv<R>(a, gg, 0), D1 = hex_deriv<R>(a, gg, 1), D2 = hex_deriv<R>(a, gg, 2);
-      const R* q = s_q[le][gg];
-      f0 += q[0] * D0 + q[1] * D1 + q[2] * D2;
-      f1 += q[3] * D0 + q[4] * D1 + q[5] * D2;
-      f2 += q[6] * D0 + q[7] * D1 + q[8] * D2;
+    for (int k = 0; k < 3; ++k) {
+      const int sk = k == 0 ? sx : k == 1 ? sy : sz;
+      const int s1 = k == 0 ? sy : sx, s2 = k == 2 ? sy : sz;
+      // signs s_ak, s_ak s_a1, s_ak s_a2, s_ak s_a1 s_a2 as bit parities
+      const R l0 = sk ? c0 : -c0;
+      const R l1 = (sk ^ s1 ^ 1) ? c1 : -c1;
+      const R l2 = (sk ^ s2 ^ 1) ? c1 : -c1;
+      const R l3 = (sk ^ s1 ^ s2) ? c2 : -c2;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        const R* u = s_u4[le][3 * i + k];
+        const R term = l0 * u[0] + l1 * u[1] + l2 * u[2] + l3 * u[3];
+        f[i] = k == 0 ? term : f[i] + term;
+      }
     }
+    const R f0 = f[0], f1 = f[1], f2 = f[2];
     if constexpr (GATHER) {
       R* ef = (R*)A.efor + 3 * (int64_t)A.corner_slot[(int64_t)e * 8 + a];
       ef[0] = f0;

@@ -174,6 +174,9 @@ class StepOracle:
         m.lnods = _ptr(arr(prob["lnods"], np.int32))
         m.coord = _ptr(arr(prob["coord"], np.float64))
         props = arr(prob["props"], np.float64).reshape(-1, 18)
+        # largest Young's modulus of the model (a stress scale for tests)
+        E_el = prob.get("E_of_elem")
+        self.E_max = float(np.max(E_el) if E_el is not None else np.max(props[:, 0]))
         m.nmat = props.shape[0]
         m.props = _ptr(props)
         m.rho = _ptr(arr(prob["rho"], np.float64))

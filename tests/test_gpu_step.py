@@ -33,8 +33,13 @@ def _compare(dev, orc, label, quiet=False):
     # a = (f_ext - f_int)/m is a small difference of large nodal forces: its
     # natural scale is max|f_int| / min(m), not max|a|
     a_scale = max(np.max(np.abs(orc.fint)) / np.min(orc.mass), np.max(np.abs(orc.a)))
+    # F formed from coordinates (here ~10 mm over 1 mm elements) carries
+    # ~1e-15 absolute rounding, so a stress at strain eps holds ~1e-15/eps of
+    # its digits in either code: stresses are compared relative to
+    # max(|S|, E 1e-4)
+    S_scale = max(np.max(np.abs(orc.S)), 1e-4 * orc.E_max)
     errs = dict(u=_rel(u, orc.u), v=_rel(v, orc.v), a=np.max(np.abs(a - orc.a)) / a_scale,
-                fint=_rel(f, orc.fint), S=_rel(S, orc.S))
+                fint=_rel(f, orc.fint), S=np.max(np.abs(S - orc.S)) / S_scale)
     if not quiet:
         print(label, {k: f"{x:.1e}" for k, x in errs.items()}, flush=True)
     for k, x in errs.items():
@@ -60,7 +65,10 @@ def test_bar_10x2x2_full_cycle(scheme):
     The undamped unloading through the reverse plateau amplifies a 1e-15
     perturbation a hundredfold every ~50 steps near its end (the CPU oracle
     does the same to itself), so the device restarts from the oracle's state
-    every 25 steps and the step map is compared over each chunk."""
+    every 25 steps and the step map is compared over each chunk.  The first
+    chunk ends at ~1e-5 strain, where F formed from the 10 mm coordinates keeps
+    only ~1e-10 of its digits in either code: comparisons start at step 50
+    (strain ~7e-5)."""
     prob = problems.bar_problem()
     r = prob["run"]
     print(f"bar 10x2x2 scheme {scheme}: 2000 steps", flush=True)
@@ -72,7 +80,8 @@ def test_bar_10x2x2_full_cycle(scheme):
                 _sync_from_oracle(dev, orc)
             dev.step(r["dt"], nsteps=25, scheme=scheme, phi=1.033)
             orc.step(r["dt"], scheme=scheme, phi=1.033, n=25)
-            _compare(dev, orc, f"bar scheme {scheme} step {25 * (chunk + 1)}", quiet=chunk % 20 != 19)
+            if chunk:
+                _compare(dev, orc, f"bar scheme {scheme} step {25 * (chunk + 1)}", quiet=chunk % 20 != 19)
         xs = orc.sdv[0, ..., 0]
         c = dev.counters()
         assert c["kf_AS"] > 0 and c["kf_SA"] > 0
