@@ -87,6 +87,25 @@ __device__ __forceinline__ void wht_stage(R (&v)[8], int m) {
   }
 }
 
+// U0, U1, U2, U12 of the 8 GP values q[b] of Q(3i+k): pair sums over the
+// k-sign, then the two-direction Hadamard over (m1, m2).
+template <typename R>
+__device__ __forceinline__ void u4_of_q(const R* q, int k, R* u) {
+  const int hk = 4 >> k, h1 = k == 0 ? 2 : 4, h2 = k == 2 ? 2 : 1;
+  R p[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    // t = 2 b1 + b2: the GPs with m1-sign b1 and m2-sign b2
+    const int b = ((t >> 1) ? h1 : 0) | ((t & 1) ? h2 : 0);
+    p[t] = q[b] + q[b | hk];
+  }
+  const R s0 = p[0] + p[1], d0 = p[1] - p[0], s1 = p[2] + p[3], d1 = p[3] - p[2];
+  u[0] = s0 + s1;   // sum
+  u[1] = s1 - s0;   // m1-signed
+  u[2] = d0 + d1;   // m2-signed
+  u[3] = d1 - d0;   // m1 m2-signed
+}
+
 template <typename R>
 __host__ __device__ __forceinline__ R hex_shape(int a, int g) {
   const double gpos = 0.577350269189626;
@@ -589,6 +608,7 @@ constexpr int COUNTER_SLOTS = 256;
 
 // Wave-level flag counters: one 64-bit atomic per wave and per set bit type.
 __device__ __forceinline__ void count_flags(unsigned long long* ctr, uint32_t fl, bool valid) {
+  if (__ballot(valid && fl != 0u) == 0ull) return;   // wave-uniform: nothing to count
   ctr += 8 * (blockIdx.x % COUNTER_SLOTS);
   const uint32_t bits[7] = {sma::FLAG_KF_AS, sma::FLAG_KF_SA, sma::FLAG_NEWTON_FAIL,
                             sma::FLAG_EIGEN_FAIL, sma::FLAG_DETF_NONPOS,
@@ -632,7 +652,7 @@ __global__ void __launch_bounds__(EPB * 8, ElemOccupancy<R>::waves) sld_element(
   __shared__ R s_rel[EPB][6][8];          // x_a - x_0 (0..2), u_a - u_0 (3..5)
   __shared__ R s_v[EPB][8][8];            // V(h) of the 6 components (pad 2)
   __shared__ R s_q[EPB][9][8];            // Q(3i+k) at each Gauss point
-  __shared__ R s_u4[EPB][9][4];           // U0, U1, U2, U12 of each Q(3i+k)
+  __shared__ R s_u4[EPB][8][4];           // U0, U1, U2, U12 of Q(0..7)
 
   const int tid = threadIdx.x;
   const int le = tid >> 3, g = tid & 7;
@@ -853,25 +873,10 @@ __global__ void __launch_bounds__(EPB * 8, ElemOccupancy<R>::waves) sld_element(
   if constexpr (SUR == 1) continue;   // uniform: the prepass has no scatter
   count_flags(A.counters, fl, valid);
   __syncthreads();
-  // thread g: U of Q(j) for j = g (and j = 8 on thread 0): pair sums over the
-  // k-sign, then the two-direction Hadamard over (m1, m2)
+  // thread g: U of Q(j = g) (Q(8) is formed in phase 3 by every thread)
   if (valid) {
-    for (int j = opaque(g); j < 9; j += 8) {
-      const int k = j % 3;
-      const int hk = 4 >> k, h1 = k == 0 ? 2 : 4, h2 = k == 2 ? 2 : 1;
-      R p[4];
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        // q = 2 b1 + b2: the GPs with m1-sign b1 and m2-sign b2
-        const int b = ((q >> 1) ? h1 : 0) | ((q & 1) ? h2 : 0);
-        p[q] = s_q[le][j][b] + s_q[le][j][b | hk];
-      }
-      const R s0 = p[0] + p[1], d0 = p[1] - p[0], s1 = p[2] + p[3], d1 = p[3] - p[2];
-      s_u4[le][j][0] = s0 + s1;   // sum
-      s_u4[le][j][1] = s1 - s0;   // m1-signed
-      s_u4[le][j][2] = d0 + d1;   // m2-signed
-      s_u4[le][j][3] = d1 - d0;   // m1 m2-signed
-    }
+    const int j = opaque(g);
+    u4_of_q(s_q[le][j], j % 3, s_u4[le][j]);
   }
   __syncthreads();
 
@@ -881,6 +886,8 @@ __global__ void __launch_bounds__(EPB * 8, ElemOccupancy<R>::waves) sld_element(
     const int a = opaque(g);
     const int sx = (0x66 >> a) & 1, sy = (0xCC >> a) & 1, sz = (0xF0 >> a) & 1;
     const R c0 = (R)0.125, c1 = (R)(0.125 * HEX_GPOS), c2 = (R)(0.125 * HEX_GPOS * HEX_GPOS);
+    R u8[4];
+    u4_of_q(s_q[le][8], 2, u8);
     R f[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -893,7 +900,7 @@ __global__ void __launch_bounds__(EPB * 8, ElemOccupancy<R>::waves) sld_element(
       const R l3 = (sk ^ s1 ^ s2) ? c2 : -c2;
 #pragma unroll
       for (int i = 0; i < 3; ++i) {
-        const R* u = s_u4[le][3 * i + k];
+        const R* u = (3 * i + k == 8) ? u8 : s_u4[le][3 * i + k];
         const R term = l0 * u[0] + l1 * u[1] + l2 * u[2] + l3 * u[3];
         f[i] = k == 0 ? term : f[i] + term;
       }
