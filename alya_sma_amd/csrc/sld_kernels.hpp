@@ -658,12 +658,13 @@ __global__ void __launch_bounds__(EPB * 8, ElemOccupancy<R>::waves) sld_element(
   const int le = tid >> 3, g = tid & 7;
   int bid = blockIdx.x;
   if (A.xcd_remap) {
-    // workgroups are dealt round-robin to the 8 XCDs: give each XCD a
-    // contiguous run of tiles so neighbouring elements share its L2
-    const int nb = gridDim.x, per = (nb + 7) / 8;
+    // workgroups are dealt round-robin to the 8 XCDs: workgroup b runs on
+    // XCD b % 8 as that XCD's (b / 8)-th.  Give each XCD a contiguous run of
+    // tiles so neighbouring elements share its L2: XCD x owns q + (x < r)
+    // workgroups (nb = 8 q + r), its run starts at x q + min(x, r).
+    const int nb = gridDim.x, q = nb >> 3, r = nb & 7;
     const int xcd = bid & 7, k = bid >> 3;
-    // only remap when the grid divides evenly (keeps the map a bijection)
-    if (nb % 8 == 0) bid = xcd * per + k;
+    bid = xcd * q + min(xcd, r) + k;
   }
   // persistent launches give each workgroup a contiguous run of tiles
   const int ntiles = (A.e_end - A.e_begin + EPB - 1) / EPB;
