@@ -99,6 +99,7 @@ struct sld_dev {
   double* sdv[2][3] = {};
   int cur = 1;                       // index of the TIME_N history and clock slot
   double* S_out = nullptr;
+  double* P_out = nullptr;
   uint32_t* flags_out = nullptr;
   unsigned long long* counters = nullptr;
   double* clock = nullptr;           // [2] t^n in slot cur
@@ -342,6 +343,7 @@ int enqueue_step(sld_dev* d, double dt, int scheme, double phi, int tn, KernelTi
     E.kg = d->kg;
     E.law_of_elem = d->law_of_elem;
     E.S_out = d->S_out;
+    E.P_out = d->P_out;
     E.flags_out = d->flags_out;
     E.counters = d->counters;
     E.xcd_remap = d->xcd;
@@ -776,9 +778,12 @@ int sld_dev_set_store(sld_dev* d, int32_t store) {
     const int64_t ngp = d->nelem * 8;
     if (dalloc(&d->S_out, 9 * ngp)) return SMA_ERR_HIP;
     track(d, d->S_out);
+    if (dalloc(&d->P_out, 9 * ngp)) return SMA_ERR_HIP;
+    track(d, d->P_out);
     if (dalloc(&d->flags_out, ngp)) return SMA_ERR_HIP;
     track(d, d->flags_out);
     HIPCHK(hipMemsetAsync(d->S_out, 0, 9 * ngp * sizeof(double), d->stream));
+    HIPCHK(hipMemsetAsync(d->P_out, 0, 9 * ngp * sizeof(double), d->stream));
     HIPCHK(hipMemsetAsync(d->flags_out, 0, ngp * sizeof(uint32_t), d->stream));
   }
   d->store = store ? 1 : 0;
@@ -1162,12 +1167,25 @@ int sld_dev_get(sld_dev* d, int32_t field, void* host, int64_t count) {
       }
       return SMA_OK;
     }
-    case SLD_FIELD_STRESS: {
+    case SLD_FIELD_REACTIONS: {
+      if (need(3 * np)) return SMA_ERR_ARG;
+      std::vector<double> f(3 * np);
+      int rc = sld_dev_get(d, SLD_FIELD_FINT, f.data(), 3 * np);
+      if (rc) return rc;
+      std::vector<uint8_t> fix(np);
+      HIPCHK(hipMemcpy(fix.data(), d->fixmask, np, hipMemcpyDeviceToHost));
+      double* out = (double*)host;
+      for (int64_t p = 0; p < np; ++p)
+        for (int k = 0; k < 3; ++k) out[3 * p + k] = ((fix[p] >> k) & 1) ? 0.0 - f[3 * p + k] : 0.0;
+      return SMA_OK;
+    }
+    case SLD_FIELD_STRESS: case SLD_FIELD_PK1: {
       if (!d->S_out) return fail(SMA_ERR_STATE, "stress not stored (sld_dev_set_store)");
       if (need(9 * ngp)) return SMA_ERR_ARG;
       double* out = (double*)host;
       tmp.resize(9 * ngp);
-      HIPCHK(hipMemcpy(tmp.data(), d->S_out, 9 * ngp * sizeof(double), hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(tmp.data(), field == SLD_FIELD_PK1 ? d->P_out : d->S_out,
+                       9 * ngp * sizeof(double), hipMemcpyDeviceToHost));
       for (int k = 0; k < 9; ++k)
         for (int64_t i = 0; i < ne; ++i)
           for (int g = 0; g < 8; ++g)

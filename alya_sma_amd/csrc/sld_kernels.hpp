@@ -578,6 +578,7 @@ struct ElemArgs {
   void* efor;                   // gather: corner forces [slot][3] (double or float)
   const int32_t* corner_slot;   // gather: [8 * e + a] -> slot of the corner at its node
   double* S_out;                // optional PK2 [9][ngp]
+  double* P_out;                // optional first Piola F S [9][ngp] (with S_out)
   uint32_t* flags_out;          // optional [ngp]
   unsigned long long* counters; // [COUNTER_SLOTS][8] per-flag Gauss-point counts
   int32_t xcd_remap;            // 1 = spread consecutive tiles per XCD
@@ -867,6 +868,13 @@ __global__ void __launch_bounds__(EPB * 8, ElemOccupancy<R>::waves) sld_element(
       const int64_t ngp = A.nelem * 8;
 #pragma unroll
       for (int q = 0; q < 9; ++q) A.S_out[q * ngp + gp] = (double)S[q];
+      // P_jk = sum_i F_ji S_ik (sld_builtin_materials.f90:355-371), column-major
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+          A.P_out[(j + 3 * k) * ngp + gp] =
+              (double)(Fm[j] * S[3 * k] + Fm[j + 3] * S[1 + 3 * k] + Fm[j + 6] * S[2 + 3 * k]);
       A.flags_out[gp] = fl;
     }
     }  // SUR != 1

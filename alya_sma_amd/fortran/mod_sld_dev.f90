@@ -12,7 +12,8 @@ module mod_sld_dev
   integer(c_int32_t), parameter :: SLD_MODE_FP64 = 0, SLD_MODE_FP32 = 1, SLD_MODE_SURROGATE = 2
   integer(c_int32_t), parameter :: SLD_FIELD_U = 0, SLD_FIELD_V = 1, SLD_FIELD_A = 2, &
        SLD_FIELD_FINT = 3, SLD_FIELD_MASS = 4, SLD_FIELD_SDV = 5, SLD_FIELD_STRESS = 6, &
-       SLD_FIELD_FLAGS = 7, SLD_FIELD_TIME = 8, SLD_FIELD_COUNTERS = 9
+       SLD_FIELD_FLAGS = 7, SLD_FIELD_TIME = 8, SLD_FIELD_COUNTERS = 9, SLD_FIELD_PK1 = 10, &
+       SLD_FIELD_REACTIONS = 11
   integer(c_int32_t), parameter :: SLD_SCHEME_CD = 0, SLD_SCHEME_TW = 1
 
   type, bind(C) :: sld_mesh_desc

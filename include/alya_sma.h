@@ -162,6 +162,14 @@ double sld_dev_critical_dt(const sld_dev *dev);
 #define SLD_FIELD_FLAGS    7   /* [nelem][8] uint32 (needs sld_dev_set_store) */
 #define SLD_FIELD_TIME     8   /* [1] current time */
 #define SLD_FIELD_COUNTERS 9   /* [8] uint64 Gauss-point counts per flag bit 0..6 */
+#define SLD_FIELD_PK1      10  /* [nelem][8][9] first Piola P = F S, column-major
+                                  (gppio_sld, sld_builtin_materials.f90:355-371;
+                                  needs sld_dev_set_store) */
+#define SLD_FIELD_REACTIONS 11 /* [npoin][3] frxid_sld: the element RHS f_ext - f_int
+                                  on prescribed dofs, 0 elsewhere
+                                  (sld_elmmat.f90:228-237; no element body forces
+                                  here, so -f_int); Alya's node-set FRXIX is its
+                                  negative (sld_nodset.f90:100) */
 int sld_dev_get(sld_dev *dev, int32_t field, void *host, int64_t count);
 
 /* Overwrite state (restart, sld_restar.f90:63-65): fields U, V, A, SDV

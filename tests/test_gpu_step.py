@@ -356,3 +356,39 @@ def test_fused_graph_runs_equal_eager_steps(scheme, assembly, mode):
           f"max |u| {np.abs(out[1]['u']).max():.3e}", flush=True)
     for k in out[0]:
         np.testing.assert_array_equal(out[0][k], out[1][k], err_msg=k)
+
+
+def test_pk1_and_reactions_fields():
+    """First Piola P = F S per Gauss point (gppio_sld) against the oracle's
+    F (restated kinematics) times its S, and P F^T symmetric (= tau); the
+    reactions field is -f_int on prescribed dofs and zero elsewhere
+    (frxid_sld), balancing the prescribed end of the bar."""
+    prob = problems.bar_problem()
+    r = prob["run"]
+    dev, orc = _pair(prob)
+    with dev:
+        dev.step(r["dt"], nsteps=600, scheme=0, phi=1.033)
+        orc.step(r["dt"], scheme=0, phi=1.033, n=600)
+        P = dev.get("pk1")
+        S = dev.get("stress").reshape(-1, 8, 3, 3).transpose(0, 1, 3, 2)   # col-major -> [i][j]
+        Pm = P.reshape(-1, 8, 3, 3).transpose(0, 1, 3, 2)
+        u = dev.get("u")
+        X = prob["coord"]
+        err = 0.0
+        for e, nodes in enumerate(prob["lnods"]):
+            _, _, F, _ = O.hex8_kinematics(X[nodes], u[nodes])
+            ref = F @ S[e]
+            err = max(err, np.abs(Pm[e] - ref).max())
+            tau = Pm[e] @ F.transpose(0, 2, 1)
+            assert np.abs(tau - tau.transpose(0, 2, 1)).max() <= 1e-9 * np.abs(tau).max()
+        assert err <= 1e-10 * np.abs(Pm).max(), err
+        R = dev.get("reactions")
+        fint = dev.get("fint")
+        fixed = np.stack([(prob["fixmask"] >> k) & 1 for k in range(3)], axis=1).astype(bool)
+        np.testing.assert_array_equal(R[~fixed], 0.0)
+        np.testing.assert_array_equal(R[fixed], -fint[fixed])
+        # x-reactions of the two x-faces (the clamped one and the prescribed one)
+        # balance up to the inertia of the bar
+        x = X[:, 0]
+        lo, hi = R[np.abs(x) < 1e-9, 0].sum(), R[np.abs(x - x.max()) < 1e-9, 0].sum()
+        assert lo * hi < 0.0
