@@ -33,7 +33,9 @@ the given value without a function.
 
 write_node_sets() writes per-step node-set histories (DISPX..Z and the
 reactions FRXIX..Z on prescribed dofs) as text columns, in the spirit of
-Alya's node-set output (sld_outset.f90 / sld_nodset).
+Alya's node-set output (sld_outset.f90 / sld_nodset).  Witness points
+(.ker.dat WITNESS_POINTS coordinates, .sld.dat WITNESS_POINTS variables) are
+written by write_witness() (alya_sma_amd.witness).
 """
 import os
 import re
@@ -97,6 +99,7 @@ def read_case(dat_path):
     main = _lines(base + ".dat")
     dom = _lines(base + ".dom.dat")
     sld = _lines(base + ".sld.dat")
+    ker = _lines(base + ".ker.dat") if os.path.exists(base + ".ker.dat") else []
 
     # ---- time (.dat)
     t0, t1, nmax = 0.0, None, None
@@ -340,6 +343,26 @@ def read_case(dat_path):
                 fext0=fext0, fext_func=fext_func)
     if faces is not None:
         prob["faces"] = faces
+    # ---- witness points (.ker.dat coordinates, .sld.dat variables)
+    wpts = []
+    for i, l in enumerate(ker):
+        if re.match(r"WITNESS_POINTS\b", l, re.I) and not re.search(r"GEOME|MESH", l, re.I):
+            sc = [1.0, 1.0, 1.0]
+            for k, key in enumerate(("XSCAL", "YSCAL", "ZSCAL")):
+                mm = re.search(key + r"\w*\s*=\s*([^\s,]+)", l, re.I)
+                if mm:
+                    sc[k] = _num(mm.group(1))
+            for row in ker[i + 1:]:
+                if row.upper().startswith("END_WITNESS"):
+                    break
+                v = [_num(x) for x in re.split(r"[\s,]+", row.strip())[:3]]
+                wpts.append([sc[k] * v[k] for k in range(3)])
+            break
+    wvars = [l.split()[0].upper() for l in _block(sld, "WITNESS_POINTS", "END_WITNESS_POINTS")]
+    if wpts and wvars:
+        from .witness import VARIABLES
+        prob["witness"] = dict(points=np.array(wpts, np.float64),
+                               variables=[v for v in wvars if v in VARIABLES])
     prob["run"] = dict(t0=t0, t_end=t1, max_steps=nmax, scheme=scheme, phi=phi, safety=safety)
     prob["node_sets"] = {k: np.array(v, np.int64) for k, v in sorted(node_sets.items())}
     prob["node_codes"] = node_codes
@@ -370,28 +393,72 @@ def write_node_sets(path, prob, history):
                 f.write(f"{step:d} {t:.15e} {s:d} " + " ".join(f"{x:.15e}" for x in (*d, *q)) + "\n")
 
 
+def write_witness(path, prob, rows):
+    """rows: list of (step, time, values [nw, nvar]).  One line per (step,
+    point): step, time, point number (1-based), then the variables in the
+    order of the deck's WITNESS_POINTS list."""
+    w = prob["witness"]
+    with open(path, "w") as f:
+        f.write("# witness points of " + prob["name"] + "\n")
+        f.write("# columns: step time point " + " ".join(w["variables"]) + "\n")
+        for step, t, vals in rows:
+            for k, v in enumerate(vals):
+                f.write(f"{step:d} {t:.15e} {k + 1:d} " + " ".join(f"{x:.15e}" for x in v) + "\n")
+
+
+def witness_fields(prob, dev):
+    """Nodal fields the witness variables need, read from the device."""
+    from . import witness as W
+    names = prob["witness"]["variables"]
+    keys = {W.VARIABLES[n][0] for n in names}
+    u = dev.get("u")
+    f = {"u": u}
+    if "v" in keys:
+        f["v"] = dev.get("v")
+    if "a" in keys:
+        f["a"] = dev.get("a")
+    if "frx" in keys:
+        f["frx"] = -dev.get("reactions")
+    if "sig" in keys:
+        f["sig"] = W.nodal_cauchy(prob["coord"], prob["lnods"], u, dev.get("pk1"))
+    return f
+
+
 def run_case(dat_path, out_dir=".", every=None, mode=0):
     """Run a deck on the GPU to its TIME_INTERVAL end with dt = safety x the
     critical step (TIME_COUPLING FROM_CRITICAL), writing
-    <out_dir>/<name>-nodes.sld.set every `every` steps (default: 200 rows)."""
+    <out_dir>/<name>-nodes.sld.set every `every` steps (default: 200 rows),
+    and <out_dir>/<name>-witness.sld.wit when the deck has witness points."""
     from .solid import Solidz
+    from . import witness as W
     prob = read_case(dat_path)
     r = prob["run"]
+    wit = prob.get("witness")
+    if wit is not None:
+        welem, wxi = W.locate(prob["coord"], prob["lnods"], wit["points"])
     with Solidz(prob) as dev:
         dev.set_mode(mode)
+        if wit is not None and any(v.startswith("SIG") for v in wit["variables"]):
+            dev.set_store(True)
         dt = r["safety"] * dev.critical_dt()
         nsteps = int(np.ceil((r["t_end"] - r["t0"]) / dt))
         if r["max_steps"]:
             nsteps = min(nsteps, r["max_steps"])
         every = every or max(1, nsteps // 200)
-        hist, done = [], 0
+        hist, wrows, done = [], [], 0
         while done < nsteps:
             n = min(every, nsteps - done)
             dev.step(dt, nsteps=n, scheme=r["scheme"], phi=r["phi"])
             done += n
-            hist.append((done, float(dev.get("time")[0]), dev.get("u"), dev.get("fint")))
+            t = float(dev.get("time")[0])
+            hist.append((done, t, dev.get("u"), dev.get("fint")))
+            if wit is not None:
+                wrows.append((done, t, W.values(prob["lnods"], welem, wxi, witness_fields(prob, dev),
+                                                wit["variables"])))
     path = os.path.join(out_dir, prob["name"] + "-nodes.sld.set")
     write_node_sets(path, prob, hist)
+    if wit is not None:
+        write_witness(os.path.join(out_dir, prob["name"] + "-witness.sld.wit"), prob, wrows)
     return path, prob, dt, nsteps
 
 

@@ -11,6 +11,7 @@ import numpy as np
 import pytest
 
 from alya_sma_amd import deck
+from alya_sma_amd import witness as W
 from oracle import oracle_py as O
 
 pytestmark = pytest.mark.gpu
@@ -55,6 +56,10 @@ def test_neumann_deck_on_gpu_matches_oracle(tmp_path):
     print("deck beam (faces + nodal force) on the GPU", flush=True)
     path, prob, dt, n = deck.run_case(case, out_dir=str(tmp_path), every=200)
     rows = np.loadtxt(path).reshape(-1, 9)
+    wit = np.loadtxt(path.replace("-nodes.sld.set", "-witness.sld.wit"))
+    nw, nv = len(prob["witness"]["points"]), len(prob["witness"]["variables"])
+    wit = wit.reshape(-1, nw, 3 + nv)
+    welem, wxi = W.locate(prob["coord"], prob["lnods"], prob["witness"]["points"])
     o = O.StepOracle(prob)
     done, k, moved = 0, 0, 0.0
     while done < n:
@@ -67,6 +72,19 @@ def test_neumann_deck_on_gpu_matches_oracle(tmp_path):
         ref = o.u[nodes].mean(axis=0)
         moved = max(moved, np.abs(ref).max())
         np.testing.assert_allclose(row[3:6], ref, rtol=0, atol=1e-10 * max(moved, 1e-30))
+        # witness points: the oracle's fields through the same interpolation
+        _, _, F = W.gp_kinematics(prob["coord"], prob["lnods"], o.u)
+        Sm = o.S.reshape(-1, 8, 3, 3).transpose(0, 1, 3, 2)
+        pk1 = np.einsum("egij,egjk->egik", F, Sm).transpose(0, 1, 3, 2).reshape(-1, 8, 9)
+        fix = np.stack([(prob["fixmask"] >> c) & 1 for c in range(3)], axis=1).astype(bool)
+        fields = {"u": o.u, "frx": np.where(fix, o.fint, 0.0),
+                  "sig": W.nodal_cauchy(prob["coord"], prob["lnods"], o.u, pk1)}
+        refw = W.values(prob["lnods"], welem, wxi, fields, prob["witness"]["variables"])
+        got = wit[k, :, 3:]
+        assert (wit[k, :, 0] == done).all()
+        assert np.isnan(got[3]).all() and np.isnan(refw[3]).all()       # outside the beam
+        scale = np.maximum(np.nanmax(np.abs(refw), axis=0), 1e-30)   # per variable
+        assert (np.abs(got[:3] - refw[:3]) <= 1e-9 * scale).all(), (done, got[:3], refw[:3])
         k += 1
     assert moved > 1e-3
     print(f"beam deck: {n} TW steps, tip deflection up to {moved:.3e} mm", flush=True)

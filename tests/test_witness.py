@@ -1,0 +1,64 @@
+"""Witness points (sld_outwit.f90) pinned by closed forms: points found in
+a distorted mesh at the natural coordinates they were made from; trilinear
+interpolation reproduces an affine field exactly; the nodal Cauchy
+projection of a uniform stress is that stress at every node; the deck's
+WITNESS_POINTS (with SCALE) and variable list are read."""
+import numpy as np
+
+from alya_sma_amd import deck, problems as P, witness as W
+
+
+def _distorted(seed=0):
+    coord, lnods = P.box_mesh_np(4, 3, 2, 4.0, 3.0, 2.0)
+    rng = np.random.default_rng(seed)
+    interior = np.all((coord > 1e-9) & (coord < np.array([4.0, 3.0, 2.0]) - 1e-9), axis=1)
+    coord = coord + 0.15 * rng.uniform(-1, 1, coord.shape) * interior[:, None]
+    return coord, lnods
+
+
+def test_locate_recovers_element_and_natural_coordinates():
+    coord, lnods = _distorted()
+    rng = np.random.default_rng(1)
+    elems = rng.integers(0, len(lnods), 20)
+    xi = rng.uniform(-0.95, 0.95, (20, 3))
+    N, _ = W.shape(xi)
+    pts = np.einsum("wa,wai->wi", N, coord[lnods[elems]])
+    e, x = W.locate(coord, lnods, np.vstack([pts, [[10.0, 10.0, 10.0]]]))
+    np.testing.assert_array_equal(e[:-1], elems)
+    np.testing.assert_allclose(x[:-1], xi, atol=1e-12)
+    assert e[-1] == -1
+
+
+def test_interpolation_is_exact_for_affine_fields():
+    coord, lnods = _distorted(2)
+    G = np.array([[0.01, -0.02, 0.005], [0.003, 0.02, -0.01], [-0.004, 0.001, 0.03]])
+    c = np.array([0.1, -0.2, 0.3])
+    u = coord @ G.T + c
+    pts = np.array([[0.5, 0.5, 0.5], [3.3, 2.1, 1.7], [2.0, 1.5, 1.0]])
+    e, xi = W.locate(coord, lnods, pts)
+    vals = W.values(lnods, e, xi, {"u": u}, ["DISPX", "DISPY", "DISPZ"])
+    np.testing.assert_allclose(vals, pts @ G.T + c, atol=1e-14)
+
+
+def test_nodal_cauchy_of_a_uniform_stress():
+    coord, lnods = _distorted(3)
+    G = np.array([[0.02, 0.01, 0.0], [-0.01, 0.01, 0.005], [0.0, 0.002, -0.015]])
+    u = coord @ G.T
+    F = np.eye(3) + G
+    S = np.array([[10.0, 2.0, -1.0], [2.0, 5.0, 0.5], [-1.0, 0.5, -3.0]])
+    Pk = F @ S
+    pk1 = np.broadcast_to(Pk.T.reshape(9), (len(lnods), 8, 9))      # column-major per GP
+    sig = F @ S @ F.T / np.linalg.det(F)
+    caust = W.nodal_cauchy(coord, lnods, u, pk1)
+    ref = [sig[0, 0], sig[1, 1], sig[2, 2], sig[1, 2], sig[0, 2], sig[0, 1]]
+    np.testing.assert_allclose(caust, np.broadcast_to(ref, caust.shape), rtol=0, atol=1e-12)
+
+
+def test_deck_witness_section(tmp_path):
+    from _decks import WITNESS, WITNESS_VARS, write_beam_deck
+    path, _ = write_beam_deck(str(tmp_path))
+    p = deck.read_case(path)
+    np.testing.assert_allclose(p["witness"]["points"], WITNESS, rtol=1e-15, atol=1e-15)
+    assert p["witness"]["variables"] == WITNESS_VARS
+    e, xi = W.locate(p["coord"], p["lnods"], p["witness"]["points"])
+    assert (e[:3] >= 0).all() and e[3] == -1
