@@ -79,6 +79,8 @@ struct sld_dev {
   int32_t* node_ptr = nullptr;
   int32_t* corner_slot = nullptr;
   bool repeated_nodes = false;       // an element lists a node twice: gather only
+  int32_t slot_width = 0;            // corner slots per node when padded (gather), else 0
+  size_t efor_bytes = 0;
   double* mass = nullptr;
   double* rmass = nullptr;
   uint8_t* fixmask = nullptr;
@@ -167,6 +169,7 @@ sld::NodalArgs nodal_args(sld_dev* d, int tn) {
   A.tn = tn;
   A.zero_fint = gather ? 0 : 1;
   A.node_ptr = gather ? d->node_ptr : nullptr;
+  A.slot_width = gather ? d->slot_width : 0;
   A.sur_count = d->sur ? d->sur->count : nullptr;
   A.fint_fp32 = f32 ? 1 : 0;
   A.rmass = d->rmass;
@@ -619,9 +622,16 @@ int sld_dev_create(const sld_mesh_desc* mesh, const sld_material_desc* mat,
   track(d, d->color_list_dev);
   {
     // slots: the corners at each node in colour order (color_list), so the
-    // gather adds them in the order of the coloured scatter
+    // gather adds them in the order of the coloured scatter.  With at most
+    // 16 corners per node the slots are padded to the largest count W
+    // (node p owns slots W p .. W p + W - 1)
+    int32_t W = 0;
+    for (int64_t p = 0; p < np; ++p) W = std::max(W, nadj[p + 1]);
+    d->slot_width = (W <= 16 && (int64_t)W * np <= INT32_MAX) ? W : 0;
     for (int64_t p = 0; p < np; ++p) nadj[p + 1] += nadj[p];
     std::vector<int32_t> slot((size_t)ne * 8), fill(nadj.begin(), nadj.end() - 1);
+    if (d->slot_width)
+      for (int64_t p = 0; p < np; ++p) fill[p] = d->slot_width * (int32_t)p;
     for (int64_t i = 0; i < ne; ++i) {
       const int64_t e = d->color_list[i];
       for (int a = 0; a < 8; ++a) slot[8 * e + a] = fill[ln[8 * e + a]]++;
@@ -680,7 +690,13 @@ int sld_dev_create(const sld_mesh_desc* mesh, const sld_material_desc* mat,
     TRY(upload(&d->E_of_elem, Ee.data(), Ee.size(), s)); track(d, d->E_of_elem);
   }
   const int64_t ngp = ne * 8;
-  TRY(dalloc((double**)&d->efor, (size_t)ngp * 3)); track(d, d->efor);
+  {
+    // corner forces (FP64-sized, reused for FP32); padding slots stay zero
+    const size_t nslot = d->slot_width ? (size_t)d->slot_width * np : (size_t)ngp;
+    TRY(dalloc((double**)&d->efor, nslot * 3)); track(d, d->efor);
+    d->efor_bytes = nslot * 3 * sizeof(double);
+    HIPCHK(hipMemsetAsync(d->efor, 0, d->efor_bytes, s));
+  }
   TRY(dalloc(&d->kg, (size_t)ne * 2)); track(d, d->kg);
   if (any_neo) {
     std::vector<uint8_t> le((size_t)ne);
@@ -765,8 +781,13 @@ int sld_dev_set_mode(sld_dev* d, int32_t mode) {
   if (mode == SLD_MODE_SURROGATE && !surrogate_attached(d))
     return fail(SMA_ERR_STATE, "surrogate mode needs sld_dev_set_surrogate first");
   if (mode != d->mode) {
-    // carry the assembled force over to the precision the new mode reads
     HIPCHK(hipStreamSynchronize(d->stream));
+    // the corner-force slots change element size (FP64 <-> FP32): clear them
+    // so that the padding slots read as zero in the new layout
+    if ((mode == SLD_MODE_FP64) != (d->mode == SLD_MODE_FP64)) {
+      HIPCHK(hipMemsetAsync(d->efor, 0, d->efor_bytes, d->stream));
+      HIPCHK(hipStreamSynchronize(d->stream));
+    }
     d->mode = mode;
   }
   return SMA_OK;

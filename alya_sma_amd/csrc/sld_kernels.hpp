@@ -160,6 +160,7 @@ struct NodalArgs {
   int32_t tn;
   int32_t zero_fint;           // coloured scatter: the predictor clears f_int
   const int32_t* node_ptr;     // gather: [npoin+1] slot ranges per node, else null
+  int32_t slot_width;          // gather with W slots per node (node i: W i ..), or 0
   const void* efor;            // gather: corner forces [slot][3] (double or float),
                                // slots of a node in ascending element order
   unsigned* sur_count;         // surrogate compact-list counter, cleared per step
@@ -275,12 +276,17 @@ __global__ void __launch_bounds__(256) sld_predict(NodalArgs A, double dt, doubl
 // Gather assembly: f_int at node i = 0 + f_(e1) + f_(e2) + ... over the
 // element corners at the node in ascending (colour-sorted) element order,
 // which is the order in which the coloured scatter adds them.
-// A node shared by 8 elements (every interior node of a hex mesh) loads its
-// 24 values at once, in one round trip; others walk their slots.
+// Slots padded to W per node (W = the mesh's largest corner count, unused
+// slots zero): node i's values sit at a fixed place, so the loads need no
+// slot range and the wave's loads are contiguous; W = 8 (hex meshes) loads
+// all 24 values at once.  Trailing zeros leave the ordered sum unchanged.
+// Without padding, a node with 8 corners still loads them in one round trip.
 template <typename R>
 __device__ __forceinline__ void gather_fint(const NodalArgs& A, int64_t i, double fi[3],
                                             bool store) {
-  const int32_t k0 = A.node_ptr[i], k1 = A.node_ptr[i + 1];
+  const int W = A.slot_width;
+  const int32_t k0 = W ? W * (int32_t)i : A.node_ptr[i];
+  const int32_t k1 = W ? k0 + W : A.node_ptr[i + 1];
   const R* ef = (const R*)A.efor + 3 * (int64_t)k0;
   R s0 = R(0), s1 = R(0), s2 = R(0);
   if (k1 - k0 == 8) {
