@@ -368,16 +368,26 @@ __host__ __device__ __forceinline__ Hencky32 hencky_fp32(const float (&H)[9]) {
 #pragma unroll
   for (int q = 0; q < 6; ++q) t[q] = e[q] = z[q];
   if (h.ok) {
-    for (int k = 1; k <= nterms; ++k) {
+    // written out term by term: constant weights 1/(2k+1), no loop-carried
+    // register copies; each lane takes its own number of terms
+    auto term = [&](float w) {
       float tn[6];
       sym_mul_commuting(t, z2, tn);
-      const float w = fast_rcp((float)(2 * k + 1));
 #pragma unroll
       for (int q = 0; q < 6; ++q) {
         t[q] = tn[q];
         e[q] += w * tn[q];
       }
-    }
+    };
+    if (nterms >= 1) { term(1.0f / 3.0f);
+    if (nterms >= 2) { term(1.0f / 5.0f);
+    if (nterms >= 3) { term(1.0f / 7.0f);
+    if (nterms >= 4) { term(1.0f / 9.0f);
+    if (nterms >= 5) { term(1.0f / 11.0f);
+    if (nterms >= 6) { term(1.0f / 13.0f);
+    if (nterms >= 7) {
+      for (int k = 7; k <= nterms; ++k) term(1.0f / (float)(2 * k + 1));
+    } } } } } } }
   }
   const float th = e[0] + e[3] + e[5];
   const float t3 = th * (1.0f / 3.0f);
