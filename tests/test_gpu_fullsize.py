@@ -1,0 +1,84 @@
+"""Parity at the benchmark sizes: the cantilever 400x20x20 (1.28 M Gauss
+points), the 256-specimen batch (21.7 M) and the SMA tube in tissue (48.4 M).
+The device runs into the loading, its state is copied into the FP64 CPU
+oracle (run in the device's colour order, OpenMP over each colour), and both
+take the next steps: every nodal field, the history and the branch flags
+agree to 1e-10 in FP64, and the FP32 mode's stress is within 1e-4 (relative
+L2) of the FP64 step from the same state."""
+import numpy as np
+import pytest
+
+from alya_sma_amd import problems
+from alya_sma_amd.solid import Solidz
+from oracle import oracle_py as O
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-10
+
+
+def _rel(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+def _load_state(orc, dev):
+    orc.u[:] = dev.get("u")
+    orc.v[:] = dev.get("v")
+    orc.a[:] = dev.get("a")
+    sdv = dev.get("sdv")
+    orc.sdv[0] = sdv
+    orc.sdv[1] = sdv
+    orc.s.t = float(dev.get("time")[0])
+
+
+CASES = {
+    "cantilever": (lambda: problems.cantilever_problem(), 1500),
+    "specimens": (lambda: problems.batched_specimens(), 300),
+    "cylinder": (lambda: problems.cylinder_problem(), 3000),
+}
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_full_size_step_matches_oracle(case):
+    build, warm = CASES[case]
+    prob = build()
+    r = prob["run"]
+    n_gp = 8 * prob["lnods"].shape[0]
+    print(f"{case}: {n_gp} Gauss points, {warm} device steps, then 2 steps on both", flush=True)
+    with Solidz(prob) as dev:
+        dev.set_store(True)
+        dev.step(r["dt"], nsteps=warm, scheme=0, phi=r["phi"])
+        ptr, order = dev.coloring()
+        orc = O.StepOracle(prob, elem_order=order, color_ptr=ptr)
+        _load_state(orc, dev)
+        dev.step(r["dt"], nsteps=2, scheme=0, phi=r["phi"])
+        orc.step(r["dt"], scheme=0, phi=r["phi"], n=2)
+        u, f = dev.get("u"), dev.get("fint")
+        a_scale = max(np.max(np.abs(orc.fint)) / np.min(orc.mass), np.max(np.abs(orc.a)))
+        S_scale = max(np.max(np.abs(orc.S)), 1e-4 * orc.E_max)
+        errs = dict(u=_rel(u, orc.u), v=_rel(dev.get("v"), orc.v),
+                    a=np.max(np.abs(dev.get("a") - orc.a)) / a_scale, fint=_rel(f, orc.fint),
+                    S=np.max(np.abs(dev.get("stress") - orc.S)) / S_scale)
+        sdv = dev.get("sdv")
+        fl = dev.get("flags")
+        moving = np.abs(orc.u).max()
+        transforming = int(np.count_nonzero(orc.flags & 3))
+        print(f"  errors {({k: f'{x:.1e}' for k, x in errs.items()})}, max |u| {moving:.3e}, "
+              f"transforming points {transforming}", flush=True)
+        assert moving > 0.0
+        for k, x in errs.items():
+            assert x < TOL, (case, k, x)
+        np.testing.assert_array_equal(fl & 3, orc.flags & 3)
+        assert _rel(sdv[..., 1:], orc.sdv[0, ..., 1:]) < TOL
+        assert np.max(np.abs(sdv[..., 0] - orc.sdv[0, ..., 0])) < 1e-9
+
+        # FP32 mode from the same state: one step against the FP64 step
+        S64 = orc.S.copy()
+        _load_state(orc, dev)       # the device state after the two steps
+        orc.step(r["dt"], scheme=0, phi=r["phi"], n=1)
+        dev.set_mode(1)
+        dev.step(r["dt"], nsteps=1, scheme=0, phi=r["phi"])
+        S32 = dev.get("stress")
+        l2 = np.linalg.norm(S32 - orc.S) / np.linalg.norm(orc.S)
+        print(f"  FP32 one-step stress error {l2:.2e} (relative L2)", flush=True)
+        assert l2 < 1e-4
+        assert np.abs(S64).max() > 0.0
