@@ -48,7 +48,11 @@ __host__ __device__ constexpr int woff_c(int l) {
 constexpr int W_TOTAL = NH * NIN + 3 * NH * NH + NOUT * NH;      // 52736
 constexpr int B_TOTAL = 4 * NH + NOUT;                           // 524
 
-__device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
+// z sigmoid(z) with the hardware reciprocal (1 ulp): the result is rounded to
+// BF16 (8 significant bits) before it is used
+__device__ __forceinline__ float silu(float z) {
+  return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z));
+}
 __device__ __forceinline__ float silu_grad(float z) {
   const float s = 1.0f / (1.0f + __expf(-z));
   return s * (1.0f + z * (1.0f - s));
