@@ -69,3 +69,39 @@ def test_create_rejects_bad_input_before_touching_the_gpu():
     p["lnods"][0, 0] = 10_000
     with pytest.raises(_lib.HipError, match="out of range"):
         Solidz(p)
+
+
+@pytest.mark.parametrize("case, message", [
+    ("unused_node", "zero mass"),
+    ("bad_material", "material index out of range"),
+    ("bad_law", "law must be"),
+    ("bad_function", "func_of_node out of range"),
+    ("bad_fext_function", "fext_func out of range"),
+    ("bad_npts", "func_npts out of range"),
+    ("no_material", "no material"),
+])
+def test_create_error_paths(case, message):
+    """sld_dev_create validates every index and table on the host and reports
+    a message (sma_last_error) without touching the GPU."""
+    from alya_sma_amd.solid import Solidz
+    p = problems.bar_problem(nx=2, ny=1, nz=1, lx=2.0, ly=1.0, lz=1.0)
+    if case == "unused_node":
+        p["coord"] = np.vstack([p["coord"], [[9.0, 9.0, 9.0]]])
+        p["fixmask"] = np.append(p["fixmask"], np.uint8(0))
+        p["func_of_node"] = np.append(p["func_of_node"], np.int32(-1))
+    elif case == "bad_material":
+        p["mat_of_elem"] = np.array([0, 1], np.int32)
+    elif case == "bad_law":
+        p["law_of_mat"] = np.array([154], np.int32)
+    elif case == "bad_function":
+        p["func_of_node"] = p["func_of_node"].copy()
+        p["func_of_node"][0] = 3
+    elif case == "bad_fext_function":
+        p["fext0"] = np.zeros_like(p["coord"])
+        p["fext_func"] = 5
+    elif case == "bad_npts":
+        p["func_npts"] = np.array([7], np.int32)
+    elif case == "no_material":
+        p["props"] = np.zeros((0, 18))
+    with pytest.raises(_lib.HipError, match=message):
+        Solidz(p)
