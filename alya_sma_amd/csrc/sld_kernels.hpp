@@ -17,10 +17,14 @@
 //                 each corner force into its node-sorted slot; the
 //                 corrector sums each node's contiguous slots in colour
 //                 order, i.e. the same additions in the same order)
-//   sld_correct   [gather f_int], a = (f_ext - f_int) / m, velocity
-//                 corrector / TW Dirichlet zeroing (solric.f90:294-330;
-//                 mod_sld_solution_methods.f90:181-210 CD, :494-527 TW);
-//                 writes t^{n+1} = t^n + dt (setgts.f90:98)
+//   sld_face_load Neumann loads of boundary faces (sld_bouope.f90), when set
+//   sld_correct   [gather f_int], f_ext (faces + nodal pattern),
+//                 a = (f_ext - f_int) / m, velocity corrector / TW Dirichlet
+//                 zeroing (solric.f90:294-330; mod_sld_solution_methods.f90:
+//                 181-210 CD, :494-527 TW); writes t^{n+1} = t^n + dt
+//                 (setgts.f90:98); inside a captured run of steps it also
+//                 runs the next step's predictor; optional energy sums
+//                 (mod_sld_energy.f90) reduced by sld_energy_reduce
 // The history (xS, F_AS, F_SA) and the clock live in two slots that swap
 // roles every step (sld_updunk.f90:149-186, svegm TIME_N <- ITER_K): slot
 // tn holds TIME_N / t^n, the step writes slot 1 - tn.
@@ -599,13 +603,6 @@ struct ElemArgs {
   unsigned* sur_count;
   double T_feat;                // temperature fed to the surrogate (inert in the law)
 };
-
-template <typename R>
-struct ElemTraits;
-template <>
-struct ElemTraits<double> { using Acc = double; };
-template <>
-struct ElemTraits<float> { using Acc = float; };
 
 // Flag counters live in COUNTER_SLOTS slots of 8 (one 64-byte line each);
 // workgroup b adds into slot b % COUNTER_SLOTS, so the per-wave atomics of a
