@@ -794,20 +794,22 @@ __global__ void __launch_bounds__(EPB * 8, ElemOccupancy<R>::waves) sld_element(
       float Hf[9];
 #pragma unroll
       for (int q = 0; q < 9; ++q) Hf[q] = H[q];
-      // plain FP32: Hencky strain tensor by series; the spectral form serves
-      // stretches beyond the series' range and the surrogate modes, whose
-      // features are principal strains
-      sma::Hencky32 hk{};
-      if constexpr (SUR == 0) hk = sma::hencky_fp32(Hf);
-      if (SUR == 0 && hk.ok) {
-        rm = sma::return_map(m, hk.theta, hk.e_n, xS0, FAS0, FSA0);
-        sma::kirchhoff_dev(m, hk, rm.xS, tau);
-      } else {
-      sma::Spectral<float> sp = sma::spectral_fp32(Hf);
+      // FP32: Hencky strain tensor by series (theta and |e'| are all the
+      // return map and the stress need); the spectral form serves stretches
+      // beyond the series' range and, in the surrogate prepass, the listed
+      // points' principal strains (the network's features)
+      const sma::Hencky32 hk = sma::hencky_fp32(Hf);
       if constexpr (SUR == 1) {
+        double th = hk.theta, en = hk.e_n;
+        sma::Spectral<float> sp{};
+        if (!hk.ok) {
+          sp = sma::spectral_fp32(Hf);
+          th = sp.theta;
+          en = sp.e_n;
+        }
         bool kf_AS, kf_SA;
         double FAS, FSA;
-        const bool flagged = sma::flags_only(m, sp.theta, sp.e_n, xS0, FAS0, FSA0, kf_AS, kf_SA, FAS, FSA);
+        const bool flagged = sma::flags_only(m, th, en, xS0, FAS0, FSA0, kf_AS, kf_SA, FAS, FSA);
         // single-branch points go to the compact list: one atomic per wave
         const bool want = flagged && (kf_AS != kf_SA);
         const unsigned long long wm = __ballot(want);
@@ -820,30 +822,47 @@ __global__ void __launch_bounds__(EPB * 8, ElemOccupancy<R>::waves) sld_element(
         if (flagged) {
           st = 2;
           if (want) {
+            if (hk.ok) sp = sma::spectral_fp32(Hf);
             const unsigned slot = base + (unsigned)__popcll(wm & ((1ull << lane) - 1ull));
             const double E = A.E_of_elem ? A.E_of_elem[e] : pr[0];
-            mlp::make_features(pr, E, A.T_feat, sp.e[0], sp.e[1], sp.e[2], sp.theta, xS0, FAS0,
+            mlp::make_features(pr, E, A.T_feat, sp.e[0], sp.e[1], sp.e[2], th, xS0, FAS0,
                                FSA0, FAS, A.sur_feat + (int64_t)slot * mlp::NIN);
             A.sur_gp[slot] = (int32_t)gp;
-            A.sur_aux[2 * (int64_t)slot] = sp.theta;
-            A.sur_aux[2 * (int64_t)slot + 1] = sp.e_n;
+            A.sur_aux[2 * (int64_t)slot] = th;
+            A.sur_aux[2 * (int64_t)slot + 1] = en;
           }
         }
         A.sur_status[gp] = st;
-      } else if constexpr (SUR == 2) {
-        const uint8_t st = A.sur_status[gp];
-        if (st == 1) {
-          rm = sma::return_map_given(m, sp.theta, sp.e_n, xS0, FAS0, FSA0, A.sur_x[gp]);
-          if (rm.flags & 3u) rm.flags |= sma::FLAG_SURR_USED;
+      } else if (hk.ok) {
+        if constexpr (SUR == 2) {
+          const uint8_t st = A.sur_status[gp];
+          if (st == 1) {
+            rm = sma::return_map_given(m, hk.theta, hk.e_n, xS0, FAS0, FSA0, A.sur_x[gp]);
+            if (rm.flags & 3u) rm.flags |= sma::FLAG_SURR_USED;
+          } else {
+            rm = sma::return_map(m, hk.theta, hk.e_n, xS0, FAS0, FSA0);
+            if (st == 2 && (rm.flags & 3u)) rm.flags |= sma::FLAG_SURR_FALLBACK;
+          }
+        } else {
+          rm = sma::return_map(m, hk.theta, hk.e_n, xS0, FAS0, FSA0);
+        }
+        sma::kirchhoff_dev(m, hk, rm.xS, tau);
+      } else {
+        sma::Spectral<float> sp = sma::spectral_fp32(Hf);
+        if constexpr (SUR == 2) {
+          const uint8_t st = A.sur_status[gp];
+          if (st == 1) {
+            rm = sma::return_map_given(m, sp.theta, sp.e_n, xS0, FAS0, FSA0, A.sur_x[gp]);
+            if (rm.flags & 3u) rm.flags |= sma::FLAG_SURR_USED;
+          } else {
+            rm = sma::return_map(m, sp.theta, sp.e_n, xS0, FAS0, FSA0);
+            if (st == 2 && (rm.flags & 3u)) rm.flags |= sma::FLAG_SURR_FALLBACK;
+          }
         } else {
           rm = sma::return_map(m, sp.theta, sp.e_n, xS0, FAS0, FSA0);
-          if (st == 2 && (rm.flags & 3u)) rm.flags |= sma::FLAG_SURR_FALLBACK;
         }
-      } else {
-        rm = sma::return_map(m, sp.theta, sp.e_n, xS0, FAS0, FSA0);
-      }
-      if constexpr (SUR != 1) sma::kirchhoff<float>(m, sp, rm.xS, tau);
-      if (!sp.eig_ok) rm.flags |= sma::FLAG_EIGEN_FAIL;
+        sma::kirchhoff<float>(m, sp, rm.xS, tau);
+        if (!sp.eig_ok) rm.flags |= sma::FLAG_EIGEN_FAIL;
       }
 #pragma unroll
       for (int q = 0; q < 9; ++q) Fm[q] = H[q] + ((q % 4 == 0) ? R(1) : R(0));
