@@ -43,8 +43,9 @@ unsigned nblocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 }  // namespace
 
-// Steps per captured graph of a long run (even: see sld_dev_step).
-constexpr int GRAPH_RUN = 16;
+// Steps per captured graph of a long run, longest first (even lengths: see
+// sld_dev_step); each run pays one stand-alone predictor.
+constexpr int GRAPH_RUNS[] = {64, 16, 2};
 
 struct GraphKey {
   double dt, phi;
@@ -946,10 +947,10 @@ int sld_dev_step(sld_dev* d, double dt, int32_t scheme, double phi, int64_t nste
   if (d->mode == SLD_MODE_SURROGATE && !surrogate_attached(d))
     return fail(SMA_ERR_STATE, "surrogate mode without a surrogate");
   int64_t left = nsteps;
-  // runs of GRAPH_RUN steps, then pairs, replay captured graphs (an even
-  // length returns the history and clock slots to where they started);
-  // inside a run each corrector also predicts the next step
-  for (int len : {GRAPH_RUN, 2}) {
+  // runs of 64, 16 and 2 steps replay captured graphs (an even length
+  // returns the history and clock slots to where they started); inside a run
+  // each corrector also predicts the next step
+  for (int len : GRAPH_RUNS) {
     if (!d->use_graph || left < len) continue;
     GraphKey key{dt, phi, scheme, d->mode, d->store, d->epb, d->xcd, d->persistent, d->assembly, d->cur,
                  d->energy, len};

@@ -334,20 +334,20 @@ def test_tube_in_tissue_fp32_one_step_error():
 @pytest.mark.parametrize("assembly", ["gather", "scatter"])
 @pytest.mark.parametrize("mode", [0, 1])
 def test_fused_graph_runs_equal_eager_steps(scheme, assembly, mode):
-    """A run of 2 x 16 + 2 + 1 steps replays captured graphs whose
+    """A run of 64 + 2 x 16 + 2 + 1 steps replays captured graphs whose
     correctors also predict the next step; one step per call runs every
     kernel on its own.  State, energies and flag counters are bitwise equal."""
     prob = problems.cantilever_problem(nx=40, ny=4, nz=4, lx=5.0, ly=0.5, lz=0.5, traction=800.0,
                                        nsteps=400)
     r = prob["run"]
     out = []
-    for per_call in (35, 1):
+    for per_call in (99, 1):
         with Solidz(prob) as dev:
             dev.set_assembly(assembly)
             dev.set_mode(mode)
             dev.set_store(True)
             dev.set_energy(True)
-            for _ in range(35 // per_call):
+            for _ in range(99 // per_call):
                 dev.step(r["dt"], nsteps=per_call, scheme=scheme, phi=r["phi"])
             out.append({k: dev.get(k) for k in ("u", "v", "a", "fint", "sdv", "stress", "flags", "time")}
                        | {"energy": np.array(list(dev.energy().values())),
