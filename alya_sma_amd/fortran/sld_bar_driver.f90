@@ -20,6 +20,8 @@ program sld_bar_driver
   real(c_double), parameter :: lx = 10.0_8, ly = 2.0_8, lz = 2.0_8, strain = 0.06_8
   integer :: npoin, nelem, ip, istep, nsteps, nrun, uout, scheme
   real(c_double), allocatable, target :: coord(:,:), props(:), rho(:), table(:,:), u(:,:), f(:,:), sdv(:,:,:)
+  real(c_double), allocatable, target :: frx(:,:)
+  real(c_double) :: ener(5)
   integer(c_int32_t), allocatable, target :: lnods(:,:), func_of_node(:), npts(:), smooth(:)
   integer(c_int8_t), allocatable, target :: fixmask(:)
   type(sld_mesh_desc) :: mesh
@@ -78,6 +80,8 @@ program sld_bar_driver
   bc = sld_bc_desc(c_loc(fixmask), c_loc(func_of_node), 1_c_int32_t, 3_c_int32_t, c_loc(npts), &
                    c_loc(smooth), c_loc(table), c_null_ptr, -1_c_int32_t)
   call sld_dev_check(sld_dev_create(mesh, mat, bc, dev), 'sld_dev_create')
+  ! energy balance of the run (sld_energy, mod_sld_energy.f90:71-162)
+  call sld_dev_check(sld_dev_set_energy(dev, 1_c_int32_t), 'sld_dev_set_energy')
 
   allocate(u(3, npoin), f(3, npoin), sdv(3, 8, nelem))
   do istep = 1, nrun, 100
@@ -96,11 +100,18 @@ program sld_bar_driver
      write(*, '(i6,3es16.8)') min(istep + 99, nrun), t(1), tipu, tipf
   end do
   call sld_dev_check(sld_dev_get(dev, SLD_FIELD_SDV, c_loc(sdv), int(24*nelem, c_int64_t)), 'get sdv')
+  allocate(frx(3, npoin))
+  call sld_dev_check(sld_dev_get(dev, SLD_FIELD_REACTIONS, c_loc(frx), int(3*npoin, c_int64_t)), &
+                     'get reactions')
+  call sld_dev_check(sld_dev_energy(dev, ener), 'sld_dev_energy')
+  write(*, '(a,4es16.8)') 'energies allie allwk allke etota', ener(1:4)
   call get_command_argument(4, arg)
   open(newunit=uout, file=trim(arg), access='stream', form='unformatted', status='replace')
   write(uout) u
   write(uout) sdv
   write(uout) f
+  write(uout) frx
+  write(uout) ener
   close(uout)
   rc = sld_dev_destroy(dev)
 end program sld_bar_driver

@@ -65,15 +65,25 @@ def test_fortran_bar_driver_matches_oracle(scheme):
     npoin, nelem = prob["coord"].shape[0], prob["lnods"].shape[0]
     u = raw[:3 * npoin].reshape(npoin, 3)
     sdv = raw[3 * npoin:3 * npoin + 24 * nelem].reshape(nelem, 8, 3)
-    f = raw[3 * npoin + 24 * nelem:].reshape(npoin, 3)
+    k = 3 * npoin + 24 * nelem
+    f = raw[k:k + 3 * npoin].reshape(npoin, 3)
+    frx = raw[k + 3 * npoin:k + 6 * npoin].reshape(npoin, 3)
+    ener = raw[k + 6 * npoin:]
     # the oracle in the device's colour order
     from alya_sma_amd.solid import Solidz
     with Solidz(prob) as dev:
         ptr, order = dev.coloring()
-    orc = O.StepOracle(prob, elem_order=order, color_ptr=ptr)
+    orc = O.StepOracle(prob, elem_order=order, color_ptr=ptr, energy=True)
     orc.step(dt, scheme=scheme, n=1500)
     eu = np.max(np.abs(u - orc.u)) / np.max(np.abs(orc.u))
     ef = np.max(np.abs(f - orc.fint)) / np.max(np.abs(orc.fint))
     print(f"fortran bar driver scheme {scheme}: u err {eu:.1e}, fint err {ef:.1e}", flush=True)
     assert eu < 1e-10 and ef < 1e-10
     assert np.max(np.abs(sdv[..., 0] - orc.sdv[0, ..., 0])) < 1e-9
+    # reactions (frxid_sld) and the energy balance through the Fortran binding
+    fixed = np.stack([(prob["fixmask"] >> c) & 1 for c in range(3)], axis=1).astype(bool)
+    np.testing.assert_array_equal(frx[~fixed], 0.0)
+    assert np.max(np.abs(frx[fixed] + orc.fint[fixed])) <= 1e-10 * np.max(np.abs(orc.fint))
+    ref = np.array(orc.energy())
+    assert ener.shape == (5,) and ener[4] == 1500.0
+    assert np.max(np.abs(ener[:4] - ref)) <= 1e-10 * np.max(np.abs(ref))
