@@ -93,6 +93,7 @@ struct sld_dev {
   double* fext0[3] = {};
   int32_t fext_func = -1;
   double* props = nullptr;
+  double* thresholds = nullptr;      // [nmat][4] Rf_AS, Rs_AS, Rf_SA, Rs_SA
   int32_t nmat = 0;
   double* rho = nullptr;
   int32_t* mat_of_elem = nullptr;
@@ -342,6 +343,7 @@ int enqueue_step(sld_dev* d, double dt, int scheme, double phi, int tn, KernelTi
     E.efor = d->efor;
     E.corner_slot = d->corner_slot;
     E.props = d->props;
+    E.thresholds = d->thresholds;
     E.mat_of_elem = d->mat_of_elem;
     E.E_of_elem = d->E_of_elem;
     E.kg = d->kg;
@@ -662,7 +664,17 @@ int sld_dev_create(const sld_mesh_desc* mesh, const sld_material_desc* mat,
     }
   }
   d->fext_func = fext_func;
-  TRY(upload(&d->fixmask, bc->fixmask, (size_t)np, s)); track(d, d->fixmask);
+  {
+    // prescribed-dof bits, plus FIX_FORCE where the nodal force pattern is nonzero
+    std::vector<uint8_t> fm((size_t)np);
+    for (int64_t p = 0; p < np; ++p) {
+      fm[p] = (uint8_t)(bc->fixmask[p] & 7);
+      if (fext_func >= 0 &&
+          (bc->fext0[3 * p] != 0.0 || bc->fext0[3 * p + 1] != 0.0 || bc->fext0[3 * p + 2] != 0.0))
+        fm[p] |= (uint8_t)sld::FIX_FORCE;
+    }
+    TRY(upload(&d->fixmask, fm.data(), (size_t)np, s)); track(d, d->fixmask);
+  }
   if (bc->func_of_node) { TRY(upload(&d->func_of_node, bc->func_of_node, (size_t)np, s)); track(d, d->func_of_node); }
   d->nfunc = bc->nfunc;
   d->func_maxpts = bc->func_maxpts;
@@ -677,6 +689,11 @@ int sld_dev_create(const sld_mesh_desc* mesh, const sld_material_desc* mat,
   }
   d->nmat = mat->nmat;
   TRY(upload(&d->props, mat->props, (size_t)mat->nmat * 18, s)); track(d, d->props);
+  {
+    std::vector<double> th((size_t)mat->nmat * 4);
+    for (int k = 0; k < mat->nmat; ++k) sma::mat_thresholds(mat->props + 18 * k, &th[4 * k]);
+    TRY(upload(&d->thresholds, th.data(), th.size(), s)); track(d, d->thresholds);
+  }
   TRY(upload(&d->rho, mat->rho, (size_t)mat->nmat, s)); track(d, d->rho);
   std::vector<int32_t> me;
   std::vector<double> Ee;

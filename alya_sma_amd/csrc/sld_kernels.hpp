@@ -144,6 +144,10 @@ __device__ __forceinline__ double funbou(const double* __restrict__ tab, int npt
   return x;
 }
 
+// fixmask bits: 1, 2, 4 = prescribed x, y, z; FIX_FORCE = the node carries
+// a nonzero entry of the nodal force pattern fext0 (set by the library).
+constexpr int FIX_FORCE = 8;
+
 struct NodalArgs {
   int64_t npoin;
   double* u[3];
@@ -349,22 +353,31 @@ __device__ __forceinline__ void node_energy(const NodalArgs& A, int64_t i, const
 // corrected a and v stay in registers and the next step's predictor runs
 // on them at t^{n+2} = t^{n+1} + dt, with the same arithmetic as its own
 // kernel; a, v^{n+1} and f_int reach memory only at the last step of the run.
+// The node's loads are all issued first; the load function of the nodal
+// force pattern is evaluated only at nodes that carry one (FIX_FORCE).
 template <int SCHEME, bool FUSE>
 __device__ __forceinline__ void correct_node(const NodalArgs& A, int64_t i, double dt, double phi,
                                              double e[3]) {
 #pragma clang fp contract(off)
-  const double cutim = A.clock[A.tn] + dt;  // t^{n+1}
-  if (i == 0) {
-    A.clock[1 - A.tn] = cutim;
-    if (A.sur_count) A.sur_count[0] = 0u;   // the next step's compact list
-  }
-  const double t_end = cutim + dt, t_mid = 0.5 * (cutim + t_end);
-  double fx = 0.0;
-  if (A.fext_func >= 0)
-    fx = funbou(A.func_table + (size_t)A.fext_func * A.func_maxpts * 4,
-                A.func_npts[A.fext_func], A.func_smooth[A.fext_func], 0, cutim);
   const double rm = A.rmass[i];
   const int fix = A.fixmask[i];
+  double fg[3];
+  if (A.node_ptr) {
+    if (A.fint_fp32) gather_fint<float>(A, i, fg, !FUSE);
+    else gather_fint<double>(A, i, fg, !FUSE);
+  } else {
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      fg[d] = A.fint_fp32 ? (double)((const float*)A.fint[d])[i] : ((const double*)A.fint[d])[i];
+  }
+  double v0[3], u0[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    v0[d] = A.v[d][i];
+    if (FUSE) u0[d] = A.u[d][i];
+  }
+  const double cutim = A.clock[A.tn] + dt;  // t^{n+1}
+  const double t_end = cutim + dt, t_mid = 0.5 * (cutim + t_end);
   // f_ext: face loads (summed per node in face order) + fx fext0
   double fe[3] = {0.0, 0.0, 0.0};
   if (A.node_face_ptr) {
@@ -374,18 +387,11 @@ __device__ __forceinline__ void correct_node(const NodalArgs& A, int64_t i, doub
       for (int d = 0; d < 3; ++d) fe[d] = fe[d] + A.fface[3 * (int64_t)k + d];
     }
   }
-  if (A.fext_func >= 0) {
+  if (A.fext_func >= 0 && (fix & FIX_FORCE)) {
+    const double fx = funbou(A.func_table + (size_t)A.fext_func * A.func_maxpts * 4,
+                             A.func_npts[A.fext_func], A.func_smooth[A.fext_func], 0, cutim);
 #pragma unroll
     for (int d = 0; d < 3; ++d) fe[d] = fe[d] + fx * A.fext0[d][i];
-  }
-  double fg[3];
-  if (A.node_ptr) {
-    if (A.fint_fp32) gather_fint<float>(A, i, fg, !FUSE);
-    else gather_fint<double>(A, i, fg, !FUSE);
-  } else {
-#pragma unroll
-    for (int d = 0; d < 3; ++d)
-      fg[d] = A.fint_fp32 ? (double)((const float*)A.fint[d])[i] : ((const double*)A.fint[d])[i];
   }
   double an[3], vn[3];
 #pragma unroll
@@ -396,9 +402,9 @@ __device__ __forceinline__ void correct_node(const NodalArgs& A, int64_t i, doub
     double acc = 0.0 + r * rm;
     if (SCHEME == 0) {
       an[d] = acc;
-      vn[d] = A.v[d][i] + (t_end - t_mid) * acc;
+      vn[d] = v0[d] + (t_end - t_mid) * acc;
     } else {
-      vn[d] = A.v[d][i];
+      vn[d] = v0[d];
       if ((fix >> d) & 1) { vn[d] = 0.0; acc = 0.0; }
       an[d] = acc;
     }
@@ -409,11 +415,13 @@ __device__ __forceinline__ void correct_node(const NodalArgs& A, int64_t i, doub
   }
   if (A.energy_part) node_energy(A, i, fg, fe, vn, e);
   if (FUSE) {
-    double u[3], un[3], vp[3];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) u[d] = A.u[d][i];
-    predict_vals(A, i, SCHEME, dt, phi, cutim + dt, fix, u, vn, an, un, vp);
-    store_prediction(A, i, u, un, vp);
+    double un[3], vp[3];
+    predict_vals(A, i, SCHEME, dt, phi, cutim + dt, fix, u0, vn, an, un, vp);
+    store_prediction(A, i, u0, un, vp);
+  }
+  if (i == 0) {
+    A.clock[1 - A.tn] = cutim;
+    if (A.sur_count) A.sur_count[0] = 0u;   // the next step's compact list
   }
 }
 
@@ -578,6 +586,7 @@ struct ElemArgs {
   const double* coord[3];       // SoA reference coordinates
   const double* u[3];           // SoA displacement (ITER_K)
   const double* props;          // [nmat][18]
+  const double* thresholds;     // [nmat][4] mat_thresholds of each material
   const int32_t* mat_of_elem;   // [nelem] or null
   const double* E_of_elem;      // [nelem] or null
   const double* kg;             // [nelem][2] bulk and shear modulus per element
@@ -761,7 +770,8 @@ __global__ void __launch_bounds__(EPB * 8, ElemOccupancy<R>::waves) sld_element(
 
     const int mat = A.mat_of_elem ? A.mat_of_elem[e] : 0;
     const double* pr = A.props + 18 * mat;
-    const sma::Mat m = sma::make_mat_kg(pr, A.kg[2 * (int64_t)e], A.kg[2 * (int64_t)e + 1]);
+    const sma::Mat m = sma::make_mat_pre(pr, A.thresholds + 4 * mat, A.kg[2 * (int64_t)e],
+                                         A.kg[2 * (int64_t)e + 1]);
     const int64_t gp = (int64_t)e * 8 + g;
     const double xS0 = A.sdv_n[0][gp], FAS0 = A.sdv_n[1][gp], FSA0 = A.sdv_n[2][gp];
 
@@ -874,7 +884,9 @@ __global__ void __launch_bounds__(EPB * 8, ElemOccupancy<R>::waves) sld_element(
     fl = rm.flags;
 
     // Q = detJ * tau * j^-T
-    if (!(detj / detJ > R(0))) fl |= sma::FLAG_DETF_NONPOS;
+    // det F = detj / detJ > 0 (false for NaN)
+    const bool detf_pos = detj > R(0) ? detJ > R(0) : (detj < R(0) && detJ < R(0));
+    if (!detf_pos) fl |= sma::FLAG_DETF_NONPOS;
     const R T[9] = {tau[0], tau[1], tau[2], tau[1], tau[3], tau[4], tau[2], tau[4], tau[5]};
 #pragma unroll
     for (int i = 0; i < 3; ++i)

@@ -67,6 +67,36 @@ __host__ __device__ __forceinline__ Mat make_mat_kg(const double* __restrict__ p
   return m;
 }
 
+// Constants of make_mat_kg that depend on props(18) alone, per material:
+// c[0..3] = Rf_AS, Rs_AS, Rf_SA, Rs_SA (the same operations).
+__host__ __device__ __forceinline__ void mat_thresholds(const double* __restrict__ p, double* c) {
+  const Mat m = make_mat_kg(p, 0.0, 0.0);
+  c[0] = m.Rf_AS;
+  c[1] = m.Rs_AS;
+  c[2] = m.Rf_SA;
+  c[3] = m.Rs_SA;
+}
+
+// make_mat_kg with the thresholds c = mat_thresholds(p) given.
+__host__ __device__ __forceinline__ Mat make_mat_pre(const double* __restrict__ p,
+                                                    const double* __restrict__ c, double K,
+                                                    double G) {
+#pragma clang fp contract(off)
+  Mat m;
+  m.eL = p[2];
+  m.alfa = p[14];
+  m.C_AS = p[5];
+  m.C_SA = p[11];
+  m.K = K;
+  m.G = G;
+  m.Rf_AS = c[0];
+  m.Rs_AS = c[1];
+  m.Rf_SA = c[2];
+  m.Rs_SA = c[3];
+  m.G_1 = (2.0 * m.G + 9.0 * (m.alfa * m.alfa) * m.K) * m.eL;
+  return m;
+}
+
 __host__ __device__ __forceinline__ Mat make_mat(const double* __restrict__ p, double E) {
   double K, G;
   elastic_moduli(p, E, K, G);
