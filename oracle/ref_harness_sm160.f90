@@ -15,7 +15,9 @@
 !
 ! Before every point the harness prints "#PT <i>" on stdout so that the
 ! warnings the reference prints (Newton non-convergence, eigen-solver
-! non-convergence, inconsistent flags) can be attributed to a point.
+! non-convergence, inconsistent flags) can be attributed to a point.  With a
+! third argument "quiet" it prints no markers and reports the wall time of
+! the point loop as "#TIME <seconds>" (speed calibration of the CPU oracle).
 !
 ! The law is called exactly as sld_builtin_materials.f90:327-335 calls it,
 ! with pgaus=1 and flagt=0 (explicit scheme, sld_elmcla.f90:93-97).
@@ -25,13 +27,17 @@ program ref_harness_sm160
   use sld_stress_model_160, only : eval_sld_stress_model_160
   implicit none
   character(len=4096) :: fin, fout
+  character(len=16) :: mode
+  logical :: quiet
   integer :: uin, uout
-  integer(8) :: n, i
+  integer(8) :: n, i, c0, c1, rate
   real(rp), allocatable :: props(:,:), F(:,:), sdv_old(:,:), sdv_new(:,:), S(:,:)
   real(rp) :: gpgdi(3,3,1), gptmp(1), gpsdv(3,1,2), gpstr(3,3,1), gpdds(3,3,3,3,1)
 
   call get_command_argument(1, fin)
   call get_command_argument(2, fout)
+  call get_command_argument(3, mode)
+  quiet = trim(mode) == 'quiet'
   open(newunit=uin, file=trim(fin), access='stream', form='unformatted', status='old')
   read(uin) n
   allocate(props(18,n), F(9,n), sdv_old(3,n), sdv_new(3,n), S(9,n))
@@ -41,8 +47,9 @@ program ref_harness_sm160
   close(uin)
 
   gptmp = 0.0_rp
+  call system_clock(c0, rate)
   do i = 1, n
-     write(*,'(a,i0)') '#PT ', i - 1
+     if (.not. quiet) write(*,'(a,i0)') '#PT ', i - 1
      gpgdi(:,:,1) = reshape(F(:,i), (/3,3/))
      gpsdv(:,1,1) = 0.0_rp
      gpsdv(:,1,2) = sdv_old(:,i)
@@ -51,6 +58,8 @@ program ref_harness_sm160
      sdv_new(:,i) = gpsdv(:,1,1)
      S(:,i) = reshape(gpstr(:,:,1), (/9/))
   end do
+  call system_clock(c1)
+  if (quiet) write(*,'(a,es12.5)') '#TIME ', real(c1 - c0, rp) / real(rate, rp)
 
   open(newunit=uout, file=trim(fout), access='stream', form='unformatted', status='replace')
   write(uout) sdv_new
