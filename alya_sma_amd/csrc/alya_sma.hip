@@ -593,13 +593,20 @@ int sld_dev_create(const sld_mesh_desc* mesh, const sld_material_desc* mat,
   for (int64_t e = 0; e < ne; ++e) d->color_ptr[color[e] + 1]++;
   for (int c = 0; c < ncolor; ++c) d->color_ptr[c + 1] += d->color_ptr[c];
   // elements are stored in the caller's order (neighbours stay close for
-  // the gather launch); the coloured scatter walks them through color_list
+  // the gather launch), grouped by constitutive law when the mesh mixes
+  // laws (SMA first, the caller's order within each law) so that the eight
+  // elements of a wave take one branch; the coloured scatter walks them
+  // through color_list (stored positions)
   d->order.resize(ne);
   for (int64_t e = 0; e < ne; ++e) d->order[e] = (int32_t)e;
+  if (any_neo)
+    std::stable_partition(d->order.begin(), d->order.end(), [&](int32_t e) {
+      return mat->law[mat->mat_of_elem ? mat->mat_of_elem[e] : 0] == SLD_LAW_SMA160;
+    });
   d->color_list.resize(ne);
   {
     std::vector<int32_t> fill(d->color_ptr.begin(), d->color_ptr.end() - 1);
-    for (int64_t e = 0; e < ne; ++e) d->color_list[fill[color[e]]++] = (int32_t)e;
+    for (int64_t i = 0; i < ne; ++i) d->color_list[fill[color[d->order[i]]]++] = (int32_t)i;
   }
   auto bail = [&](int rc) {
     sld_dev_destroy(d);

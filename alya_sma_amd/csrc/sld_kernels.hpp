@@ -773,14 +773,22 @@ __global__ void __launch_bounds__(EPB * 8, ElemOccupancy<R>::waves) sld_element(
     const sma::Mat m = sma::make_mat_pre(pr, A.thresholds + 4 * mat, A.kg[2 * (int64_t)e],
                                          A.kg[2 * (int64_t)e + 1]);
     const int64_t gp = (int64_t)e * 8 + g;
-    const double xS0 = A.sdv_n[0][gp], FAS0 = A.sdv_n[1][gp], FSA0 = A.sdv_n[2][gp];
+    // a neo-Hookean element has no history: its values stay as they are in
+    // both slots (equal from creation and every set), so they are neither
+    // read nor written
+    const bool neo = A.law_of_elem && A.law_of_elem[e];
+    double xS0 = 0.0, FAS0 = 0.0, FSA0 = 0.0;
+    if (!neo) {
+      xS0 = A.sdv_n[0][gp];
+      FAS0 = A.sdv_n[1][gp];
+      FSA0 = A.sdv_n[2][gp];
+    }
 
     R tau[6];
     R Fm[9];
     sma::ReturnMap rm;
-    const bool neo = A.law_of_elem && A.law_of_elem[e];
     if (neo) {
-      // neo-Hookean element: no history (TIME_N carries over), no flags
+      // neo-Hookean element: no history, no flags
       R jm1;
       sma::neo_hookean_tau<R>(pr[0], pr[1], H, tau, jm1);
       rm.xS = xS0;
@@ -878,9 +886,11 @@ __global__ void __launch_bounds__(EPB * 8, ElemOccupancy<R>::waves) sld_element(
       for (int q = 0; q < 9; ++q) Fm[q] = H[q] + ((q % 4 == 0) ? R(1) : R(0));
     }
     if constexpr (SUR != 1) {
-    A.sdv_k[0][gp] = rm.xS;
-    A.sdv_k[1][gp] = rm.F_AS;
-    A.sdv_k[2][gp] = rm.F_SA;
+    if (!neo) {
+      A.sdv_k[0][gp] = rm.xS;
+      A.sdv_k[1][gp] = rm.F_AS;
+      A.sdv_k[2][gp] = rm.F_SA;
+    }
     fl = rm.flags;
 
     // Q = detJ * tau * j^-T

@@ -80,23 +80,31 @@ def kernel_name(mode, cfg):
 def algorithmic_bytes(prob, dev, mode, assembly):
     """Bytes one element launch must move at least (average over launches).
 
-    Per Gauss point: history read + written 48 B, connectivity 4 B, the
-    element's bulk and shear moduli 2 B.  Gather assembly: per Gauss point
-    the corner's slot index 4 B and its force written (3 x 8 B in FP64, 3 x 4
-    B in FP32), per node coordinates + displacement read 48 B.  Coloured
-    scatter: per node the launch touches coordinates + displacement 48 B and
-    f_int read-modify-write (2 x 3 x 8 B or 2 x 3 x 4 B)."""
+    Per Gauss point: connectivity 4 B, the element's bulk and shear moduli
+    2 B, and on SMA elements the history read + written 48 B (neo-Hookean
+    elements have none).  Gather assembly: per Gauss point the corner's slot
+    index 4 B and its force written (3 x 8 B in FP64, 3 x 4 B in FP32), per
+    node coordinates + displacement read 48 B.  Coloured scatter: per node
+    the launch touches coordinates + displacement 48 B and f_int
+    read-modify-write (2 x 3 x 8 B or 2 x 3 x 4 B)."""
+    from alya_sma_amd.problems import LAW_SMA
     fb = 8 if mode == "fp64" else 4
     lnods = prob["lnods"]
+    ne = lnods.shape[0]
+    per_gp = np.full(ne, 6.0 + 48.0)
+    if prob.get("law_of_mat") is not None:
+        mat = prob.get("mat_of_elem")
+        law = np.asarray(prob["law_of_mat"])[mat if mat is not None else np.zeros(ne, np.int64)]
+        per_gp = np.where(law == LAW_SMA, 6.0 + 48.0, 6.0)
     if assembly == "gather":
-        ne, npn = lnods.shape[0], prob["coord"].shape[0]
-        return float(ne * 8 * (54 + 4 + 3 * fb) + npn * 48)
+        npn = prob["coord"].shape[0]
+        return float(8 * per_gp.sum() + ne * 8 * (4 + 3 * fb) + npn * 48)
     ptr, order = dev.coloring()
     per = []
     for c in range(len(ptr) - 1):
         els = order[ptr[c]:ptr[c + 1]]
         npc = len(np.unique(lnods[els].ravel()))
-        per.append(len(els) * 8 * 54 + npc * (48 + 6 * fb))
+        per.append(8 * per_gp[els].sum() + npc * (48 + 6 * fb))
     return float(np.mean(per))
 
 

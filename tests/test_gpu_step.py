@@ -300,6 +300,12 @@ def test_tube_in_tissue_two_materials_match_oracle():
     dev, orc = _pair(prob)
     with dev:
         assert abs(dev.critical_dt() - orc.critical_dt()) <= 1e-13 * orc.critical_dt()
+        # the elements are stored grouped by law: within a colour the SMA
+        # elements come before the tissue
+        ptr, order = dev.coloring()
+        laws = prob["mat_of_elem"][order]
+        for c in range(len(ptr) - 1):
+            assert np.all(np.diff(laws[ptr[c]:ptr[c + 1]]) >= 0)
         for chunk in range(16):
             if chunk:
                 _sync_from_oracle(dev, orc)
