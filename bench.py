@@ -18,7 +18,9 @@ relative L2 difference of the Gauss-point PK2 stress after one step taken
 from the FP64 trajectory's state, at ten checkpoints along the run (max
 over checkpoints): the error the precision mode adds to a step; the
 largest Gauss-point difference relative to the largest stress component is
-reported beside it.  The difference between whole trajectories at the last step is reported too,
+reported beside it, with the number of points whose branch flags differ from
+the FP64 step's (a loading function within rounding of its threshold) and
+the largest difference over the points that took the same branch.  The difference between whole trajectories at the last step is reported too,
 next to the same difference between two FP64 runs whose initial states
 differ by 1e-13 (relative, in u at step 20): transformation fronts make the
 undamped load-unload cycle sensitive to any rounding, so the end-of-run
@@ -207,6 +209,9 @@ def accuracy(prob, modes, K, cfgs, sur=None, tol=1e-4):
         d.step(dt, nsteps=first)
     pert.set("u", pert.get("u") * (1.0 + 1e-13))
     done, one, onemax = first, {m: 0.0 for m in others}, {m: 0.0 for m in others}
+    # points whose branch flags differ from the FP64 step's, and the largest
+    # difference over the points that took the same branch
+    flips, same_max = {m: 0 for m in others}, {m: 0.0 for m in others}
     chunk = max(2, (K - first) // 10)
     while done < K:
         n = min(chunk, K - done)
@@ -215,6 +220,7 @@ def accuracy(prob, modes, K, cfgs, sur=None, tol=1e-4):
             st = {f: ref.get(f) for f in ("u", "v", "a", "sdv", "time")}
         ref.step(dt, nsteps=1)
         S = ref.get("stress")
+        fl = (ref.get("flags") & 3).reshape(-1)
         for m in others:
             for f, v in st.items():
                 probe[m].set(f, v)
@@ -222,14 +228,22 @@ def accuracy(prob, modes, K, cfgs, sur=None, tol=1e-4):
             Sm = probe[m].get("stress")
             one[m] = max(one[m], rel(Sm, S))
             onemax[m] = max(onemax[m], relmax(Sm, S))
+            same = (probe[m].get("flags") & 3).reshape(-1) == fl
+            flips[m] += int(np.count_nonzero(~same))
+            d9 = np.abs(Sm - S).reshape(-1, 9)[same]
+            if d9.size:
+                same_max[m] = max(same_max[m], float(d9.max()) / max(float(np.abs(S).max()), 1e-300))
         for d in [pert] + list(traj.values()):
             d.step(dt, nsteps=n)
         done += n
+        print(f"# accuracy pass: {done} of {K} steps", file=sys.stderr, flush=True)
     S = ref.get("stress")
     out = {"fp64": {"stress_rel_l2_one_step": 0.0, "stress_rel_max_one_step": 0.0,
                     "stress_rel_l2_trajectory_end": 0.0}}
     for m in others:
         out[m] = {"stress_rel_l2_one_step": one[m], "stress_rel_max_one_step": onemax[m],
+                  "stress_rel_max_one_step_same_branch": same_max[m],
+                  "branch_flips_one_step": flips[m],
                   "stress_rel_l2_trajectory_end": rel(traj[m].get("stress"), S)}
     sens = rel(pert.get("stress"), S)
     for d in [ref, pert] + list(traj.values()) + list(probe.values()):
