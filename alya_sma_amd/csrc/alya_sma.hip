@@ -100,6 +100,7 @@ struct sld_dev {
   double* E_of_elem = nullptr;
   double* kg = nullptr;              // [nelem][2] bulk, shear modulus
   uint8_t* law_of_elem = nullptr;    // [nelem] 1 = neo-Hookean; null when all SMA
+  int32_t nh_begin = -1;             // first stored neo-Hookean element (law-grouped), or -1
   double* sdv[2][3] = {};
   int cur = 1;                       // index of the TIME_N history and clock slot
   double* S_out = nullptr;
@@ -201,7 +202,7 @@ sld::NodalArgs nodal_args(sld_dev* d, int tn) {
 
 int g_num_cu = 0;
 
-template <typename R, int EPB, bool STORE, int SUR, bool GATHER>
+template <typename R, int EPB, bool STORE, int SUR, bool GATHER, int LAWS = 0>
 void launch_element(sld::ElemArgs A, hipStream_t s, int persistent) {
   const int ne = A.e_end - A.e_begin;
   if (ne <= 0) return;
@@ -216,20 +217,35 @@ void launch_element(sld::ElemArgs A, hipStream_t s, int persistent) {
       grid = (ntiles + A.tiles_per_block - 1) / A.tiles_per_block;
     }
   }
-  hipLaunchKernelGGL((sld::sld_element<R, EPB, STORE, SUR, GATHER>), dim3(grid), dim3(EPB * 8), 0,
-                     s, A);
+  hipLaunchKernelGGL((sld::sld_element<R, EPB, STORE, SUR, GATHER, LAWS>), dim3(grid), dim3(EPB * 8),
+                     0, s, A);
 }
 
-template <typename R, bool STORE, int SUR, bool GATHER>
+template <typename R, bool STORE, int SUR, bool GATHER, int LAWS = 0>
 void launch_element_epb(int epb, const sld::ElemArgs& A, hipStream_t s, int persistent) {
-  if (epb == 8) launch_element<R, 8, STORE, SUR, GATHER>(A, s, persistent);
-  else if (epb == 16) launch_element<R, 16, STORE, SUR, GATHER>(A, s, persistent);
+  if (epb == 8) launch_element<R, 8, STORE, SUR, GATHER, LAWS>(A, s, persistent);
+  else if (epb == 16) launch_element<R, 16, STORE, SUR, GATHER, LAWS>(A, s, persistent);
   else launch_element<R, 32, STORE, SUR, GATHER>(A, s, persistent);
 }
 
 template <typename R, int SUR>
 void launch_element_mode(const sld_dev* d, const sld::ElemArgs& A, hipStream_t s, bool gather) {
   const bool st = d->store != 0;
+  if constexpr (SUR == 0) {
+    if (gather && d->nh_begin >= 0 && A.e_end > d->nh_begin) {
+      // law-grouped mesh: the SMA range with the per-element kernel, the
+      // neo-Hookean range with the kernel that has only that law
+      sld::ElemArgs B = A;
+      B.e_end = std::min(A.e_end, d->nh_begin);
+      if (st) launch_element_epb<R, true, 0, true>(d->epb, B, s, d->persistent);
+      else launch_element_epb<R, false, 0, true>(d->epb, B, s, d->persistent);
+      B.e_begin = std::max(A.e_begin, d->nh_begin);
+      B.e_end = A.e_end;
+      if (st) launch_element_epb<R, true, 0, true, 2>(d->epb, B, s, d->persistent);
+      else launch_element_epb<R, false, 0, true, 2>(d->epb, B, s, d->persistent);
+      return;
+    }
+  }
   if (gather) {
     if (st) launch_element_epb<R, true, SUR, true>(d->epb, A, s, d->persistent);
     else launch_element_epb<R, false, SUR, true>(d->epb, A, s, d->persistent);
@@ -728,6 +744,8 @@ int sld_dev_create(const sld_mesh_desc* mesh, const sld_material_desc* mat,
     for (int64_t i = 0; i < ne; ++i)
       le[i] = mat->law[mat->mat_of_elem ? mat->mat_of_elem[d->order[i]] : 0] == SLD_LAW_NEOHOOKEAN102;
     TRY(upload(&d->law_of_elem, le.data(), le.size(), s)); track(d, d->law_of_elem);
+    // stored grouped by law: the neo-Hookean elements are a trailing range
+    d->nh_begin = (int32_t)(std::find(le.begin(), le.end(), (uint8_t)1) - le.begin());
   }
   hipLaunchKernelGGL(sld::sld_elem_moduli, dim3(nblocks(ne, 256)), dim3(256), 0, s, ne, d->props,
                      d->mat_of_elem, d->E_of_elem, d->kg);

@@ -647,8 +647,14 @@ __device__ __forceinline__ void count_flags(unsigned long long* ctr, uint32_t fl
 #ifndef SLD_WAVES_F32
 #define SLD_WAVES_F32 4
 #endif
-template <typename R>
-struct ElemOccupancy { static constexpr int waves = sizeof(R) == 8 ? SLD_WAVES_F64 : SLD_WAVES_F32; };
+// LAWS = 0: each element takes its own law (law_of_elem); LAWS = 2: every
+// element of the launch is neo-Hookean (a range of a law-grouped mesh),
+// without the SMA path and its registers.
+template <typename R, int LAWS = 0>
+struct ElemOccupancy {
+  static constexpr int waves = LAWS == 2 ? (sizeof(R) == 8 ? 3 : 6)
+                                         : (sizeof(R) == 8 ? SLD_WAVES_F64 : SLD_WAVES_F32);
+};
 
 // The value of x, opaque to the optimiser: quantities derived from the lane
 // index (shape-function derivatives at the lane's Gauss point or node) are
@@ -659,8 +665,8 @@ __device__ __forceinline__ int opaque(int x) {
   return x;
 }
 
-template <typename R, int EPB, bool STORE, int SUR = 0, bool GATHER = false>
-__global__ void __launch_bounds__(EPB * 8, ElemOccupancy<R>::waves) sld_element(ElemArgs A) {
+template <typename R, int EPB, bool STORE, int SUR = 0, bool GATHER = false, int LAWS = 0>
+__global__ void __launch_bounds__(EPB * 8, (ElemOccupancy<R, LAWS>::waves)) sld_element(ElemArgs A) {
   __shared__ double s_abs[EPB][8][6];     // gathered absolute x and u
   __shared__ R s_rel[EPB][6][8];          // x_a - x_0 (0..2), u_a - u_0 (3..5)
   __shared__ R s_v[EPB][8][8];            // V(h) of the 6 components (pad 2)
@@ -776,7 +782,7 @@ __global__ void __launch_bounds__(EPB * 8, ElemOccupancy<R>::waves) sld_element(
     // a neo-Hookean element has no history: its values stay as they are in
     // both slots (equal from creation and every set), so they are neither
     // read nor written
-    const bool neo = A.law_of_elem && A.law_of_elem[e];
+    const bool neo = LAWS == 2 || (A.law_of_elem && A.law_of_elem[e]);
     double xS0 = 0.0, FAS0 = 0.0, FSA0 = 0.0;
     if (!neo) {
       xS0 = A.sdv_n[0][gp];

@@ -76,7 +76,9 @@ def kernel_name(mode, cfg):
     r = "double" if mode == "fp64" else "float"
     sur = 2 if mode == "surrogate" else 0
     ga = "true" if cfg["assembly"] == "gather" else "false"
-    return f"sld_element<{r}, {cfg['elems_per_block']}, false, {sur}, {ga}>"
+    # the last template argument: 0 = each element's own law (2 = the
+    # neo-Hookean-only launch that follows it on a law-grouped mesh)
+    return f"sld_element<{r}, {cfg['elems_per_block']}, false, {sur}, {ga}, 0>"
 
 
 def algorithmic_bytes(prob, dev, mode, assembly):
@@ -111,19 +113,23 @@ def algorithmic_bytes(prob, dev, mode, assembly):
 
 
 def measured_traffic(workload, kernel):
-    """HBM bytes per launch of `kernel` on `workload` from the committed
+    """HBM bytes per element phase of `kernel` on `workload` from the committed
     rocprofv3 PMC summary (FETCH_SIZE and WRITE_SIZE scaled by a plain copy
-    kernel), or None when that pair was not measured."""
+    kernel): on a law-grouped mesh the phase is two launches, `kernel` over
+    the SMA range (last template argument 0) and its neo-Hookean-only
+    variant (2), whose bytes add.  None when the pair was not measured."""
+    nh = kernel[:-2] + "2>"
+    found = {}
     try:
         with open(TRAFFIC_FILE) as f:
             for line in f:
                 m = re.match(r"traffic_bytes_per_launch\s+(\S+)\s+(.+?)\s+=\s+([0-9.eE+]+)\s*$",
                              line.strip())
-                if m and m.group(1) == workload and m.group(2) == kernel:
-                    return float(m.group(3))
+                if m and m.group(1) == workload and m.group(2) in (kernel, nh):
+                    found[m.group(2)] = float(m.group(3))
     except OSError:
         pass
-    return None
+    return sum(found.values()) if kernel in found else None
 
 
 def cpu_baseline(prob, budget_s):
