@@ -667,7 +667,6 @@ __device__ __forceinline__ int opaque(int x) {
 
 template <typename R, int EPB, bool STORE, int SUR = 0, bool GATHER = false, int LAWS = 0>
 __global__ void __launch_bounds__(EPB * 8, (ElemOccupancy<R, LAWS>::waves)) sld_element(ElemArgs A) {
-  __shared__ double s_abs[EPB][8][6];     // gathered absolute x and u
   __shared__ R s_rel[EPB][6][8];          // x_a - x_0 (0..2), u_a - u_0 (3..5)
   __shared__ R s_v[EPB][8][8];            // V(h) of the 6 components (pad 2)
   __shared__ R s_q[EPB][9][8];            // Q(3i+k) at each Gauss point
@@ -694,21 +693,26 @@ __global__ void __launch_bounds__(EPB * 8, (ElemOccupancy<R, LAWS>::waves)) sld_
   const bool valid = i_el < A.e_end;
   const int e = !valid ? 0 : A.elem_list ? A.elem_list[i_el] : i_el;
 
-  // -- phase 1: gather the element's 8 nodes into LDS (thread g loads node g)
+  // -- phase 1: gather the element's 8 nodes (thread g loads node g); node
+  // 0's values come from the element's first lane by a wave shuffle
   int node = 0;
+  double xa[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   if (valid) {
     node = A.lnods[(int64_t)e * 8 + g];
-    s_abs[le][g][0] = A.coord[0][node];
-    s_abs[le][g][1] = A.coord[1][node];
-    s_abs[le][g][2] = A.coord[2][node];
-    s_abs[le][g][3] = A.u[0][node];
-    s_abs[le][g][4] = A.u[1][node];
-    s_abs[le][g][5] = A.u[2][node];
+    xa[0] = A.coord[0][node];
+    xa[1] = A.coord[1][node];
+    xa[2] = A.coord[2][node];
+    xa[3] = A.u[0][node];
+    xa[4] = A.u[1][node];
+    xa[5] = A.u[2][node];
   }
-  __syncthreads();
-  if (valid) {
+  {
+    const int lane0 = (tid & 63) & ~7;   // lane of the element's node 0
 #pragma unroll
-    for (int c = 0; c < 6; ++c) s_rel[le][c][g] = (R)(s_abs[le][g][c] - s_abs[le][0][c]);
+    for (int c = 0; c < 6; ++c) {
+      const double x0 = __shfl(xa[c], lane0, 64);
+      if (valid) s_rel[le][c][g] = (R)(xa[c] - x0);
+    }
   }
   __syncthreads();
   // thread c < 6: V(h) of component c, nodes taken in sign-bit order
