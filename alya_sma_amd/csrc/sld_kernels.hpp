@@ -668,7 +668,9 @@ __device__ __forceinline__ int opaque(int x) {
 template <typename R, int EPB, bool STORE, int SUR = 0, bool GATHER = false, int LAWS = 0>
 __global__ void __launch_bounds__(EPB * 8, (ElemOccupancy<R, LAWS>::waves)) sld_element(ElemArgs A) {
   __shared__ R s_rel[EPB][6][8];          // x_a - x_0 (0..2), u_a - u_0 (3..5)
-  __shared__ R s_v[EPB][8][8];            // V(h) of the 6 components (pad 2)
+  // V(h) of the 6 components; rows padded to 8 for aligned reads, except in
+  // the neo-Hookean-only kernel, whose occupancy the LDS bounds
+  __shared__ R s_v[EPB][8][LAWS == 2 ? 6 : 8];
   __shared__ R s_q[EPB][9][8];            // Q(3i+k) at each Gauss point
   __shared__ R s_u4[EPB][8][4];           // U0, U1, U2, U12 of Q(0..7)
 
