@@ -5,7 +5,8 @@
 //                 (mod_sld_solution_methods.f90:122-145 CD, :406-429 TW;
 //                 sld_updbcs.f90:79-114 + sld_funbou.f90:95-136); clears
 //                 f_int for the coloured scatter
-//   sld_element   SoA gather into LDS element tiles, reference Jacobian +
+//   sld_element   SoA gather into LDS element tiles (relative coordinates
+//                 and displacements), reference Jacobian +
 //                 deformation gradient at the 8 Gauss points, SMA return
 //                 map, Kirchhoff stress, internal force
 //                 (sld_element_operations.f90:119-256, elmca2.f90:105-111,
