@@ -318,6 +318,30 @@ def test_tube_in_tissue_two_materials_match_oracle():
         assert np.all(xs[mat == 1] == 0)       # the tissue has no history
 
 
+def test_neo_hookean_only_mesh_matches_oracle():
+    """Every element neo-Hookean (law 102): the whole mesh is the
+    neo-Hookean-only launch.  A soft cantilever with a small tip load."""
+    prob = problems.cantilever_problem(nx=16, ny=4, nz=4, lx=4.0, ly=1.0, lz=1.0, nsteps=400,
+                                       traction=0.002, E_noise=0.0)
+    prob["props"] = problems.TISSUE[None].copy()
+    prob["law_of_mat"] = np.array([problems.LAW_NEOHOOKEAN], np.int32)
+    prob["rho"] = np.array([problems.RHO_TISSUE])
+    prob["E_of_elem"] = None
+    with Solidz(prob) as probe:
+        dt = 0.5 * probe.critical_dt()
+    # the load ramps 0 -> 1 -> 0 over the 400 steps of this step size
+    prob["func_table"] = problems.smooth_table(400 * dt, [[0, 0, 0], [1, 0, 0], [0, 0, 0]])[None]
+    dev, orc = _pair(prob)
+    with dev:
+        assert abs(dev.critical_dt() - orc.critical_dt()) <= 1e-13 * orc.critical_dt()
+        for chunk in range(4):
+            dev.step(dt, nsteps=100, scheme=0)
+            orc.step(dt, scheme=0, n=100)
+            _compare(dev, orc, f"neo-Hookean cantilever {100 * (chunk + 1)}")
+        assert np.abs(orc.u).max() > 1e-3
+        assert np.all(dev.get("sdv") == 0.0)
+
+
 def test_tube_in_tissue_fp32_one_step_error():
     prob = problems.cylinder_problem(ntheta=24, nz=8, nr_tube=2, nr_tissue=4, nsteps=400)
     dt = prob["run"]["dt"]
