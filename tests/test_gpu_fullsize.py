@@ -4,7 +4,8 @@ The device runs into the loading, its state is copied into the FP64 CPU
 oracle (run in the device's colour order, OpenMP over each colour), and both
 take the next steps: every nodal field, the history and the branch flags
 agree to 1e-10 in FP64, and the FP32 mode's stress is within 1e-4 (relative
-L2) of the FP64 step from the same state."""
+L2) of the FP64 step from the same state; the cantilever also with the
+Tchamwa-Wielgosz scheme."""
 import numpy as np
 import pytest
 
@@ -37,21 +38,23 @@ CASES = {
 }
 
 
-@pytest.mark.parametrize("case", list(CASES))
-def test_full_size_step_matches_oracle(case):
+@pytest.mark.parametrize("case,scheme", [(c, 0) for c in CASES] + [("cantilever", 1)])
+def test_full_size_step_matches_oracle(case, scheme):
+    """scheme 0 = central difference, 1 = Tchamwa-Wielgosz."""
     build, warm = CASES[case]
     prob = build()
     r = prob["run"]
     n_gp = 8 * prob["lnods"].shape[0]
-    print(f"{case}: {n_gp} Gauss points, {warm} device steps, then 2 steps on both", flush=True)
+    print(f"{case} scheme {scheme}: {n_gp} Gauss points, {warm} device steps, then 2 steps on both",
+          flush=True)
     with Solidz(prob) as dev:
         dev.set_store(True)
-        dev.step(r["dt"], nsteps=warm, scheme=0, phi=r["phi"])
+        dev.step(r["dt"], nsteps=warm, scheme=scheme, phi=r["phi"])
         ptr, order = dev.coloring()
         orc = O.StepOracle(prob, elem_order=order, color_ptr=ptr)
         _load_state(orc, dev)
-        dev.step(r["dt"], nsteps=2, scheme=0, phi=r["phi"])
-        orc.step(r["dt"], scheme=0, phi=r["phi"], n=2)
+        dev.step(r["dt"], nsteps=2, scheme=scheme, phi=r["phi"])
+        orc.step(r["dt"], scheme=scheme, phi=r["phi"], n=2)
         u, f = dev.get("u"), dev.get("fint")
         a_scale = max(np.max(np.abs(orc.fint)) / np.min(orc.mass), np.max(np.abs(orc.a)))
         S_scale = max(np.max(np.abs(orc.S)), 1e-4 * orc.E_max)
@@ -74,9 +77,9 @@ def test_full_size_step_matches_oracle(case):
         # FP32 mode from the same state: one step against the FP64 step
         S64 = orc.S.copy()
         _load_state(orc, dev)       # the device state after the two steps
-        orc.step(r["dt"], scheme=0, phi=r["phi"], n=1)
+        orc.step(r["dt"], scheme=scheme, phi=r["phi"], n=1)
         dev.set_mode(1)
-        dev.step(r["dt"], nsteps=1, scheme=0, phi=r["phi"])
+        dev.step(r["dt"], nsteps=1, scheme=scheme, phi=r["phi"])
         S32 = dev.get("stress")
         l2 = np.linalg.norm(S32 - orc.S) / np.linalg.norm(orc.S)
         print(f"  FP32 one-step stress error {l2:.2e} (relative L2)", flush=True)
