@@ -41,6 +41,11 @@ int upload(T** p, const T* host, size_t count, hipStream_t) {
 
 unsigned nblocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
+// Workgroup size of the corrector without the energy balance.
+#ifndef SLD_CORRECT_THREADS
+#define SLD_CORRECT_THREADS 64
+#endif
+
 }  // namespace
 
 // Steps per captured graph of a long run, longest first (even lengths: see
@@ -425,12 +430,18 @@ int enqueue_step(sld_dev* d, double dt, int scheme, double phi, int tn, KernelTi
     TEND();
   }
   TBEGIN(2);
-  if (fuse) {
-    if (scheme == 0) hipLaunchKernelGGL((sld::sld_correct<0, true>), dim3(nb), dim3(256), 0, s, N, dt, phi);
-    else hipLaunchKernelGGL((sld::sld_correct<1, true>), dim3(nb), dim3(256), 0, s, N, dt, phi);
-  } else {
-    if (scheme == 0) hipLaunchKernelGGL((sld::sld_correct<0, false>), dim3(nb), dim3(256), 0, s, N, dt, phi);
-    else hipLaunchKernelGGL((sld::sld_correct<1, false>), dim3(nb), dim3(256), 0, s, N, dt, phi);
+  {
+    // small workgroups spread the nodes evenly over the CUs; the energy
+    // balance keeps 256 (its partial sums are per 256-node block)
+    const int cbs = d->energy ? 256 : SLD_CORRECT_THREADS;
+    const dim3 g(nblocks(d->npoin, cbs)), b(cbs);
+    if (fuse) {
+      if (scheme == 0) hipLaunchKernelGGL((sld::sld_correct<0, true>), g, b, 0, s, N, dt, phi);
+      else hipLaunchKernelGGL((sld::sld_correct<1, true>), g, b, 0, s, N, dt, phi);
+    } else {
+      if (scheme == 0) hipLaunchKernelGGL((sld::sld_correct<0, false>), g, b, 0, s, N, dt, phi);
+      else hipLaunchKernelGGL((sld::sld_correct<1, false>), g, b, 0, s, N, dt, phi);
+    }
   }
   if (d->energy)
     hipLaunchKernelGGL(sld::sld_energy_reduce, dim3(1), dim3(256), 0, s, d->energy_part, (int)nb,
