@@ -224,6 +224,60 @@ def test_element_with_repeated_node_gather_matches_oracle():
         _compare(dev, orc, "repeated node 200")
 
 
+def _fan_problem(ntheta=12, nr=2, nz=2, radius=1.0, height=1.0):
+    """Hexes around an axis (r, theta, z); the innermost ring collapses onto
+    one axis node per level, so each inner element lists the axis nodes
+    twice and an interior axis node gathers 4 x ntheta element corners."""
+    axis = lambda iz: iz
+    ring = lambda ir, it, iz: (nz + 1) + (ir - 1) + nr * ((it % ntheta) + ntheta * iz)
+    node = lambda ir, it, iz: axis(iz) if ir == 0 else ring(ir, it, iz)
+    npoin = (nz + 1) + nr * ntheta * (nz + 1)
+    coord = np.zeros((npoin, 3))
+    for iz in range(nz + 1):
+        coord[axis(iz)] = (0.0, 0.0, height * iz / nz)
+        for ir in range(1, nr + 1):
+            for it in range(ntheta):
+                a = 2 * np.pi * it / ntheta
+                r = radius * ir / nr
+                coord[ring(ir, it, iz)] = (r * np.cos(a), r * np.sin(a), height * iz / nz)
+    lnods = np.array([[node(ir, it, iz), node(ir + 1, it, iz), node(ir + 1, it + 1, iz),
+                       node(ir, it + 1, iz), node(ir, it, iz + 1), node(ir + 1, it, iz + 1),
+                       node(ir + 1, it + 1, iz + 1), node(ir, it + 1, iz + 1)]
+                      for iz in range(nz) for it in range(ntheta) for ir in range(nr)], np.int32)
+    fix = np.zeros(npoin, np.uint8)
+    fix[np.abs(coord[:, 2]) < 1e-12] = 7
+    top = np.abs(coord[:, 2] - height) < 1e-12
+    fext0 = np.zeros((npoin, 3))
+    fext0[top, 0] = 2.0
+    fext0[top, 2] = 3.0
+    prob = dict(coord=coord, lnods=lnods, props=problems.NITINOL[None].copy(),
+                rho=np.array([problems.RHO_NITINOL]), fixmask=fix,
+                func_of_node=np.full(npoin, -1, np.int32), E_of_elem=None, fext0=fext0)
+    return prob
+
+
+def test_fan_with_many_corners_at_a_node_gather_matches_oracle():
+    """More than 16 element corners at a node: the gather runs with per-node
+    slot ranges instead of padded slots (and with repeated nodes)."""
+    prob = _fan_problem()
+    assert np.bincount(prob["lnods"].ravel()).max() == 4 * 12
+    orc0 = O.StepOracle(dict(prob, func_table=np.zeros((1, 2, 4)), func_npts=np.array([2], np.int32),
+                             func_smooth=np.array([1], np.int32), fext_func=-1))
+    dt = 0.5 * orc0.critical_dt()
+    prob.update(func_table=problems.smooth_table(400 * dt, [[0, 0, 0], [1, 0, 0], [0, 0, 0]])[None],
+                func_npts=np.array([3], np.int32), func_smooth=np.array([1], np.int32), fext_func=0)
+    for k in range(len(prob["lnods"])):
+        _, gpvol, _, _ = O.hex8_kinematics(prob["coord"][prob["lnods"][k]], np.zeros((8, 3)))
+        assert np.all(gpvol > 0)
+    dev, orc = _pair(prob)
+    with dev:
+        for chunk in range(4):
+            dev.step(dt, nsteps=100)
+            orc.step(dt, n=100)
+            _compare(dev, orc, f"fan {100 * (chunk + 1)}")
+        assert np.abs(orc.u).max() > 0.0
+
+
 def test_autotune_leaves_state_unchanged():
     prob = problems.cantilever_problem(nx=40, ny=8, nz=8, lx=10.0, ly=2.0, lz=2.0, nsteps=100,
                                        traction=700.0)
