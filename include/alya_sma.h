@@ -177,9 +177,12 @@ int sld_dev_get(sld_dev *dev, int32_t field, void *host, int64_t count);
 int sld_dev_set(sld_dev *dev, int32_t field, const void *host, int64_t count);
 
 /* Colouring used by the handle: colour_ptr [ncolor+1] and elem_order
- * [nelem], the caller's elements sorted by colour (stable), colour c =
- * elem_order[colour_ptr[c] .. colour_ptr[c+1]-1].  Both assemblies add the
- * element forces at a node in this order. */
+ * [nelem], the caller's elements sorted by colour, colour c =
+ * elem_order[colour_ptr[c] .. colour_ptr[c+1]-1]; within a colour in the
+ * caller's order, except that a mesh mixing constitutive laws lists its SMA
+ * elements before its neo-Hookean ones (the handle stores elements grouped
+ * by law).  Both assemblies add the element forces at a node in this
+ * order. */
 int sld_dev_coloring(const sld_dev *dev, int32_t *ncolor, int32_t *color_ptr,
                      int32_t max_colors, int32_t *elem_order);
 
