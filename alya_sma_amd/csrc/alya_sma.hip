@@ -1077,7 +1077,8 @@ int sld_dev_time_kernels(sld_dev* d, double dt, int32_t scheme, double phi, int3
   hipEventDestroy(t1);
   out[0] = cnt[0] ? sum[0] / cnt[0] : 0.0;    // predictor, ms per launch
   // element kernel, ms per launch: the element phase (all its launches,
-  // back to back) divided by its number of launches
+  // back to back) divided by its number of launches; the gather phase counts
+  // as one launch, also where a law-grouped mesh runs it as two kernels
   out[1] = T.element_launches ? sum[1] / T.element_launches : 0.0;
   out[2] = cnt[2] ? sum[2] / cnt[2] : 0.0;    // corrector, ms per launch
   // element launches a step makes (the timing repeats excluded)
