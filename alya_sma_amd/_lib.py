@@ -81,6 +81,7 @@ def lib():
     L.sld_dev_coloring.argtypes = [p, p, p, i32, p]
     L.sld_dev_set_tile.argtypes = [p, i32, i32, i32]
     L.sld_dev_set_assembly.argtypes = [p, i32]
+    L.sld_dev_set_gravity.argtypes = [p, C.POINTER(C.c_double)]
     L.sld_dev_set_energy.argtypes = [p, i32]
     L.sld_dev_set_faces.argtypes = [p, p]
     L.sld_dev_energy.argtypes = [p, p]

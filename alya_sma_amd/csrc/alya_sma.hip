@@ -133,6 +133,8 @@ struct sld_dev {
     double* fface = nullptr;           // [4 nface][3]
   } faces;
   int mode = SLD_MODE_FP64, store = 0, epb = 8, xcd = 1, persistent = 0, use_graph = 1;
+  int gravity = 0;                   // sld_dev_set_gravity
+  double grav[3] = {0.0, 0.0, 0.0};
   int assembly = SLD_ASSEMBLY_GATHER;
   double dtcrit = 0.0;
   std::map<GraphKey, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
@@ -201,6 +203,11 @@ sld::NodalArgs nodal_args(sld_dev* d, int tn) {
   if (d->faces.nface > 0) {
     A.node_face_ptr = d->faces.node_ptr;
     A.fface = d->faces.fface;
+  }
+  if (d->gravity) {
+    A.gravity = 1;
+    A.mass = d->mass;
+    for (int k = 0; k < 3; ++k) A.grav[k] = d->grav[k];
   }
   return A;
 }
@@ -889,6 +896,17 @@ int sld_dev_set_assembly(sld_dev* d, int32_t assembly) {
   return SMA_OK;
 }
 
+int sld_dev_set_gravity(sld_dev* d, const double* g) {
+  if (!d) return fail(SMA_ERR_ARG, "null handle");
+  if (g && !(std::isfinite(g[0]) && std::isfinite(g[1]) && std::isfinite(g[2])))
+    return fail(SMA_ERR_ARG, "sld_dev_set_gravity: non-finite g");
+  HIPCHK(hipStreamSynchronize(d->stream));
+  free_graphs(d);   // captured steps hold the previous value
+  d->gravity = g ? 1 : 0;
+  for (int k = 0; k < 3; ++k) d->grav[k] = g ? g[k] : 0.0;
+  return SMA_OK;
+}
+
 int sld_dev_set_energy(sld_dev* d, int32_t on) {
   if (!d) return fail(SMA_ERR_ARG, "null handle");
   if (on != 0 && on != 1) return fail(SMA_ERR_ARG, "on must be 0 or 1");
@@ -1250,9 +1268,17 @@ int sld_dev_get(sld_dev* d, int32_t field, void* host, int64_t count) {
       if (rc) return rc;
       std::vector<uint8_t> fix(np);
       HIPCHK(hipMemcpy(fix.data(), d->fixmask, np, hipMemcpyDeviceToHost));
+      // frxid = elrhs = elfext - elfint on prescribed dofs: the element loads
+      // are gravity's m g (the Neumann and nodal loads are assembled apart)
+      std::vector<double> m(d->gravity ? np : 0);
+      if (d->gravity) HIPCHK(hipMemcpy(m.data(), d->mass, np * sizeof(double), hipMemcpyDeviceToHost));
       double* out = (double*)host;
       for (int64_t p = 0; p < np; ++p)
-        for (int k = 0; k < 3; ++k) out[3 * p + k] = ((fix[p] >> k) & 1) ? 0.0 - f[3 * p + k] : 0.0;
+        for (int k = 0; k < 3; ++k) {
+          double r = 0.0 - f[3 * p + k];
+          if (d->gravity) r = r + m[p] * d->grav[k];
+          out[3 * p + k] = ((fix[p] >> k) & 1) ? r : 0.0;
+        }
       return SMA_OK;
     }
     case SLD_FIELD_STRESS: case SLD_FIELD_PK1: {

@@ -185,6 +185,9 @@ struct NodalArgs {
   // and the face-corner forces of the step [slot][3]
   const int32_t* node_face_ptr;
   const double* fface;
+  // gravity (sld_dev_set_gravity): f_ext += m g at every dof; mass is set
+  int32_t gravity;
+  double grav[3];
 };
 
 // Boundary faces with Neumann loads (sld_bouope.f90), one QUA04 face per
@@ -393,6 +396,13 @@ __device__ __forceinline__ void correct_node(const NodalArgs& A, int64_t i, doub
                              A.func_npts[A.fext_func], A.func_smooth[A.fext_func], 0, cutim);
 #pragma unroll
     for (int d = 0; d < 3; ++d) fe[d] = fe[d] + fx * A.fext0[d][i];
+  }
+  if (A.gravity) {
+    // the element load sum_g N_a rho g w detJ (sld_elmmat.f90:152-157) of
+    // every element at the node: the lumped mass times g
+    const double m = A.mass[i];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) fe[d] = fe[d] + m * A.grav[d];
   }
   double an[3], vn[3];
 #pragma unroll

@@ -98,6 +98,8 @@ class Solidz:
         self._keep = []
         if prob.get("faces") is not None:
             self.set_faces(prob["faces"])
+        if prob.get("gravity") is not None:
+            self.set_gravity(prob["gravity"])
 
     def close(self):
         if getattr(self, "h", None):
@@ -126,6 +128,15 @@ class Solidz:
     def set_tile(self, elems_per_block, xcd_remap=True, persistent=0):
         check(lib().sld_dev_set_tile(self.h, int(elems_per_block), 1 if xcd_remap else 0,
                                      int(persistent)), "sld_dev_set_tile")
+
+    def set_gravity(self, g):
+        """Gravity acceleration g[3] (grnor_sld * gravi_sld): f_ext += m g at
+        every node; None removes it."""
+        if g is None:
+            check(lib().sld_dev_set_gravity(self.h, None), "sld_dev_set_gravity")
+            return
+        gv = (C.c_double * 3)(*[float(x) for x in g])
+        check(lib().sld_dev_set_gravity(self.h, gv), "sld_dev_set_gravity")
 
     def set_surrogate(self, surrogate, tol=1e-4):
         """Attach a trained alya_sma_amd.surrogate.Surrogate: mode 2 then

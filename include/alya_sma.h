@@ -231,6 +231,14 @@ typedef struct {
 } sld_face_desc;
 int sld_dev_set_faces(sld_dev *dev, const sld_face_desc *faces);
 
+/* Gravity (GRAVITY: NORM, GX, GY, GZ of the .sld.dat, sld_reaphy.f90:209-223;
+ * its element load in sld_elmmat.f90:152-157): g[3] = grnor_sld *
+ * gravi_sld, the acceleration (the caller normalises the direction).  Every
+ * dof, prescribed ones included, receives m_a g, which is the element load
+ * sum_g N_a rho g w detJ summed over the elements at the node (the lumped
+ * mass row sum); the reactions field includes it.  g = NULL removes it. */
+int sld_dev_set_gravity(sld_dev *dev, const double *g);
+
 /* Energy balance of the explicit run (replaces sld_energy,
  * mod_sld_energy.f90:71-162, called from sld_endste.f90): on = 1 starts a
  * fresh balance (allocated on first use), on = 0 stops accumulating.  Each

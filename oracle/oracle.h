@@ -107,6 +107,11 @@ typedef struct {
     const int32_t *face_func;  /* [nface] function id or -1; NULL = none     */
     int32_t face_transient;    /* 1: value = (function column x, 0, 0) or 0  */
     int32_t face_follower;     /* 1: integrate over X + u^{n+1}              */
+    /* gravity acceleration g = grnor_sld * gravi_sld (sld_reaphy.f90:209-223);
+     * its element load sum_g N_a rho g w detJ (sld_elmmat.f90:152-157) is the
+     * lumped mass times g */
+    int32_t has_gravity;
+    double gravity[3];
 } orc_model;
 
 /* Nodal forces of the boundary faces at time t over the geometry

@@ -139,7 +139,8 @@ class _Model(C.Structure):
                 ("law_of_mat", C.c_void_p),
                 ("nface", C.c_int64), ("lnodb", C.c_void_p), ("face_code", C.c_void_p),
                 ("face_value", C.c_void_p), ("face_func", C.c_void_p),
-                ("face_transient", C.c_int32), ("face_follower", C.c_int32)]
+                ("face_transient", C.c_int32), ("face_follower", C.c_int32),
+                ("has_gravity", C.c_int32), ("gravity", C.c_double * 3)]
 
 
 class _State(C.Structure):
@@ -205,6 +206,11 @@ class StepOracle:
             m.face_func = _ptr(arr(fc.get("func"), np.int32))
             m.face_transient = int(bool(fc.get("transient", False)))
             m.face_follower = int(bool(fc.get("follower", False)))
+        g = prob.get("gravity")
+        if g is not None:
+            m.has_gravity = 1
+            for d in range(3):
+                m.gravity[d] = float(g[d])
         self.m = m
         n, e = self.npoin, self.nelem
         self.u = np.zeros((n, 3)); self.v = np.zeros((n, 3)); self.a = np.zeros((n, 3))

@@ -311,6 +311,9 @@ int orc_sld_step(const orc_model *m, orc_state *s, double dt, int scheme, double
     if (m->fext0)
         for (int64_t p = 0; p < np; p++)
             for (int d = 0; d < 3; d++) fe[3 * p + d] = fe[3 * p + d] + fx * m->fext0[3 * p + d];
+    if (m->has_gravity)     /* on every dof, prescribed ones too (their reactions) */
+        for (int64_t p = 0; p < np; p++)
+            for (int d = 0; d < 3; d++) fe[3 * p + d] = fe[3 * p + d] + s->mass[p] * m->gravity[d];
     for (int64_t p = 0; p < 3 * np; p++) s->rhs[p] = s->rhs[p] + fe[p];
     /* solve + corrector */
     for (int64_t p = 0; p < np; p++) {
