@@ -13,7 +13,8 @@ one this path uses:
              [+ element]), MATERIALS
   .fix.dat   ON_BOUNDARIES (boundary -> code), ON_NODES (node -> code)
   .set.dat   NODE (node -> node set)
-  .sld.dat   PROPERTIES: MATERIAL, DENSITY, CONSTITUTIVE_MODEL NITINOL
+  .sld.dat   PROBLEM_DEFINITION: GRAVITY (NORM, GX, GY, GZ);
+             PROPERTIES: MATERIAL, DENSITY, CONSTITUTIVE_MODEL NITINOL
              (sld_reaphy.f90: parco(1:17)) or NEOHO ANSYS mu K
              (lawst 102), FORCE: FIELD = n (nodal forces ramped by
              t/(t1 - t0), sld_matrix.f90:249-262); NUMERICAL_TREATMENT:
@@ -343,6 +344,19 @@ def read_case(dat_path):
                 fext0=fext0, fext_func=fext_func)
     if faces is not None:
         prob["faces"] = faces
+    # ---- gravity (.sld.dat PROBLEM_DEFINITION, sld_reaphy.f90:209-223):
+    # g = NORM times the normalised (GX, GY, GZ), each 0 when absent
+    for l in _block(sld, "PROBLEM_DEFINITION", "END_PROBLEM_DEFINITION"):
+        if re.match(r"GRAVI\w*\s*:", l, re.I):
+            val = lambda k: next((_num(mm.group(1)) for mm in
+                                  [re.search(k + r"\w*\s*=\s*([^\s,]+)", l, re.I)] if mm), 0.0)
+            d = np.array([val("GX"), val("GY"), val("GZ")])
+            nrm = float(np.linalg.norm(d))
+            if nrm > 0.0:
+                d = d / nrm
+            g = val("NORM") * d
+            if np.any(g != 0.0):
+                prob["gravity"] = g
     # ---- witness points (.ker.dat coordinates, .sld.dat variables)
     wpts = []
     for i, l in enumerate(ker):

@@ -79,6 +79,14 @@ module mod_sld_dev
        integer(c_int) :: rc
      end function sld_dev_set_faces
 
+     ! g(3) = grnor_sld * gravi_sld (sld_reaphy.f90:209-223)
+     function sld_dev_set_gravity(dev, g) bind(C, name="sld_dev_set_gravity") result(rc)
+       import :: c_int, c_double, c_ptr
+       type(c_ptr), value :: dev
+       real(c_double), intent(in) :: g(3)
+       integer(c_int) :: rc
+     end function sld_dev_set_gravity
+
      function sld_dev_set_energy(dev, on) bind(C, name="sld_dev_set_energy") result(rc)
        import :: c_int, c_int32_t, c_ptr
        type(c_ptr), value :: dev

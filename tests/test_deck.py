@@ -86,3 +86,21 @@ def test_deck_neumann_keywords(tmp_path):
     np.testing.assert_allclose(ff.sum(axis=0), [0.0, 0.0, PRESSURE * (4.0 - 0.25)], atol=1e-12)
     assert (ff[clamped] == 0.0).all()
     assert (ff[np.abs(ref["coord"][:, 2] - LZ) > 1e-9] == 0.0).all()
+
+
+def test_deck_gravity(tmp_path):
+    """GRAVITY: NORM=..., GX=..., GY=..., GZ=... in PROBLEM_DEFINITION gives
+    g = NORM times the normalised direction (sld_reaphy.f90:209-223)."""
+    from _decks import write_beam_deck
+    path, _ = write_beam_deck(str(tmp_path))
+    sld = os.path.join(str(tmp_path), "beam.sld.dat")
+    text = open(sld).read()
+    assert "  PROBLEM_DEFINITION\n" in text
+    text = text.replace("  PROBLEM_DEFINITION\n",
+                        "  PROBLEM_DEFINITION\n    GRAVITY: NORM=9810.0, GX=0.0, GY=3.0, GZ=-4.0\n")
+    open(sld, "w").write(text)
+    p = deck.read_case(path)
+    np.testing.assert_allclose(p["gravity"], [0.0, 9810.0 * 0.6, -9810.0 * 0.8], rtol=1e-15)
+    os.makedirs(str(tmp_path / "plain"))
+    assert "gravity" not in deck.read_case(write_beam_deck(str(tmp_path / "plain"))[0])
+
