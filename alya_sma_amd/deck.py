@@ -385,14 +385,17 @@ def read_case(dat_path):
     return prob
 
 
-def reactions(prob, fint):
-    """Reaction forces on prescribed dofs (f_int on those dofs; no external
-    load acts there), zero elsewhere (sld_elmmat.f90:228-244)."""
+def reactions(prob, fint, mass=None):
+    """FRXIX..Z = -frxid_sld on prescribed dofs, zero elsewhere: frxid is
+    elrhs = elfext - elfint there (sld_elmmat.f90:228-244), so f_int, less
+    the gravity load m g when the case has gravity (mass = the lumped mass)."""
     mask = np.stack([(prob["fixmask"] >> d) & 1 for d in range(3)], axis=1).astype(bool)
-    return np.where(mask, fint, 0.0)
+    g = prob.get("gravity")
+    r = fint if g is None or mass is None else fint - np.asarray(mass)[:, None] * g
+    return np.where(mask, r, 0.0)
 
 
-def write_node_sets(path, prob, history):
+def write_node_sets(path, prob, history, mass=None):
     """history: list of (step, time, u [npoin,3], fint [npoin,3]).  One row
     per (step, set): sums over the set's nodes of DISPX..Z / their count
     (the set's mean displacement) and of the reactions FRXIX..Z."""
@@ -400,7 +403,7 @@ def write_node_sets(path, prob, history):
         f.write("# node sets of " + prob["name"] + "\n")
         f.write("# columns: step time set DISPX DISPY DISPZ FRXIX FRXIY FRXIZ\n")
         for step, t, u, fint in history:
-            r = reactions(prob, fint)
+            r = reactions(prob, fint, mass)
             for s, nodes in prob["node_sets"].items():
                 d = u[nodes].mean(axis=0)
                 q = r[nodes].sum(axis=0)
@@ -460,6 +463,7 @@ def run_case(dat_path, out_dir=".", every=None, mode=0):
             nsteps = min(nsteps, r["max_steps"])
         every = every or max(1, nsteps // 200)
         hist, wrows, done = [], [], 0
+        mass = dev.get("mass")
         while done < nsteps:
             n = min(every, nsteps - done)
             dev.step(dt, nsteps=n, scheme=r["scheme"], phi=r["phi"])
@@ -470,7 +474,7 @@ def run_case(dat_path, out_dir=".", every=None, mode=0):
                 wrows.append((done, t, W.values(prob["lnods"], welem, wxi, witness_fields(prob, dev),
                                                 wit["variables"])))
     path = os.path.join(out_dir, prob["name"] + "-nodes.sld.set")
-    write_node_sets(path, prob, hist)
+    write_node_sets(path, prob, hist, mass)
     if wit is not None:
         write_witness(os.path.join(out_dir, prob["name"] + "-witness.sld.wit"), prob, wrows)
     return path, prob, dt, nsteps
