@@ -19,7 +19,7 @@
 //                 corrector sums each node's contiguous slots in colour
 //                 order, i.e. the same additions in the same order)
 //   sld_face_load Neumann loads of boundary faces (sld_bouope.f90), when set
-//   sld_correct   [gather f_int], f_ext (faces + nodal pattern),
+//   sld_correct   [gather f_int], f_ext (faces + nodal pattern + m g),
 //                 a = (f_ext - f_int) / m, velocity corrector / TW Dirichlet
 //                 zeroing (solric.f90:294-330; mod_sld_solution_methods.f90:
 //                 181-210 CD, :494-527 TW); writes t^{n+1} = t^n + dt
