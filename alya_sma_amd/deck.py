@@ -438,6 +438,8 @@ def witness_fields(prob, dev):
         f["frx"] = -dev.get("reactions")
     if "sig" in keys:
         f["sig"] = W.nodal_cauchy(prob["coord"], prob["lnods"], u, dev.get("pk1"))
+    if "eps" in keys or "lep" in keys:
+        f["eps"], f["lep"] = W.nodal_strains(prob["coord"], prob["lnods"], u)
     return f
 
 

@@ -2,11 +2,14 @@
 
 A witness point is a coordinate where nodal results are interpolated with
 the shape functions of the element that contains it: DISPX..Z, VELOX..Z,
-ACCEX..Z, the reactions FRXIX..Z (-frxid_sld, sld_outwit.f90:184-211) and the
+ACCEX..Z, the reactions FRXIX..Z (-frxid_sld, sld_outwit.f90:184-211), the
 Cauchy stress SIGXX, SIGYY, SIGZZ, SIGYZ, SIGXZ, SIGXY of the nodal projection
 caust_sld (sld_calculate_pp_tensors.f90, sld_calculate_pp_tensors_ielem.f90):
 per Gauss point sigma = F P^T / det F, lumped onto the nodes with the weights
-N_a gpvol and divided by the lumped volume sum_g N_a gpvol.  The points come
+N_a gpvol and divided by the lumped volume sum_g N_a gpvol; the Green strain
+EPSXX..EPSXY (green_sld, E = (F^T F - I)/2) and the logarithmic strain
+LEPXX..LEPXY (lepsi_sld, the eigen form of log(F F^T)/2,
+sld_calculate_pp_tensors_ielem.f90:125-215) by the same projection.  The points come
 from the WITNESS_POINTS section of the case's .ker.dat (ker_readat.f90:
 3832-3870), the variables from the WITNESS_POINTS list of its .sld.dat.
 Host-side post-processing of the device's fields; HEX08 meshes.
@@ -24,7 +27,11 @@ VARIABLES = {"DISPX": ("u", 0), "DISPY": ("u", 1), "DISPZ": ("u", 2),
              "ACCEX": ("a", 0), "ACCEY": ("a", 1), "ACCEZ": ("a", 2),
              "FRXIX": ("frx", 0), "FRXIY": ("frx", 1), "FRXIZ": ("frx", 2),
              "SIGXX": ("sig", 0), "SIGYY": ("sig", 1), "SIGZZ": ("sig", 2),
-             "SIGYZ": ("sig", 3), "SIGXZ": ("sig", 4), "SIGXY": ("sig", 5)}
+             "SIGYZ": ("sig", 3), "SIGXZ": ("sig", 4), "SIGXY": ("sig", 5),
+             "EPSXX": ("eps", 0), "EPSYY": ("eps", 1), "EPSZZ": ("eps", 2),
+             "EPSYZ": ("eps", 3), "EPSXZ": ("eps", 4), "EPSXY": ("eps", 5),
+             "LEPXX": ("lep", 0), "LEPYY": ("lep", 1), "LEPZZ": ("lep", 2),
+             "LEPYZ": ("lep", 3), "LEPXZ": ("lep", 4), "LEPXY": ("lep", 5)}
 
 
 def shape(xi):
@@ -81,6 +88,32 @@ def gp_kinematics(coord, lnods, u):
     return gpvol, N, F
 
 
+def _project(coord, lnods, gpvol, N, tens):
+    """Nodal projection of a symmetric tensor per Gauss point tens [ne, 8,
+    3, 3]: Voigt (xx, yy, zz, yz, xz, xy) lumped with N_a gpvol, divided by
+    the lumped volume."""
+    voigt = np.stack([tens[..., 0, 0], tens[..., 1, 1], tens[..., 2, 2],
+                      tens[..., 1, 2], tens[..., 0, 2], tens[..., 0, 1]], axis=-1)
+    wgt = N[None, :, :] * gpvol[:, :, None]                    # [ne, 8 gp, 8 node]
+    npoin = len(coord)
+    out = np.zeros((npoin, 6))
+    vmass = np.zeros(npoin)
+    np.add.at(out, lnods, np.einsum("ega,egv->eav", wgt, voigt))
+    np.add.at(vmass, lnods, wgt.sum(axis=1))
+    return out / vmass[:, None]
+
+
+def nodal_strains(coord, lnods, u):
+    """(green, lepsi) [npoin, 6] each: the nodal projections of the Green
+    strain (F^T F - I)/2 and of the logarithmic strain log(F F^T)/2."""
+    lnods = np.asarray(lnods)
+    gpvol, N, F = gp_kinematics(coord, lnods, u)
+    green = 0.5 * (np.einsum("egki,egkj->egij", F, F) - np.eye(3))
+    w, v = np.linalg.eigh(np.einsum("egik,egjk->egij", F, F))
+    lep = np.einsum("egik,egk,egjk->egij", v, 0.5 * np.log(w), v)
+    return _project(coord, lnods, gpvol, N, green), _project(coord, lnods, gpvol, N, lep)
+
+
 def nodal_cauchy(coord, lnods, u, pk1):
     """caust [npoin, 6] (xx, yy, zz, yz, xz, xy) from the first Piola stress
     pk1 [ne, 8, 9] (column-major per Gauss point, the device's layout)."""
@@ -88,21 +121,13 @@ def nodal_cauchy(coord, lnods, u, pk1):
     gpvol, N, F = gp_kinematics(coord, lnods, u)
     P = np.asarray(pk1, dtype=np.float64).reshape(-1, 8, 3, 3).transpose(0, 1, 3, 2)
     sig = np.einsum("egij,egkj->egik", F, P) / np.linalg.det(F)[..., None, None]
-    voigt = np.stack([sig[..., 0, 0], sig[..., 1, 1], sig[..., 2, 2],
-                      sig[..., 1, 2], sig[..., 0, 2], sig[..., 0, 1]], axis=-1)   # [ne, 8, 6]
-    wgt = N[None, :, :] * gpvol[:, :, None]                    # [ne, 8 gp, 8 node]
-    npoin = len(coord)
-    caust = np.zeros((npoin, 6))
-    vmass = np.zeros(npoin)
-    np.add.at(caust, lnods, np.einsum("ega,egv->eav", wgt, voigt))
-    np.add.at(vmass, lnods, wgt.sum(axis=1))
-    return caust / vmass[:, None]
+    return _project(coord, lnods, gpvol, N, sig)
 
 
 def values(lnods, elem, xi, fields, variables):
     """[nw, nvar] values of the named variables at located points; fields
-    maps "u", "v", "a", "frx", "sig" to nodal arrays.  Points outside the mesh
-    give NaN."""
+    maps "u", "v", "a", "frx", "sig", "eps", "lep" to nodal arrays.  Points
+    outside the mesh give NaN."""
     lnods = np.asarray(lnods)
     out = np.full((len(elem), len(variables)), np.nan)
     N, _ = shape(xi)

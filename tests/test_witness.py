@@ -1,7 +1,8 @@
 """Witness points (sld_outwit.f90) pinned by closed forms: points found in
 a distorted mesh at the natural coordinates they were made from; trilinear
 interpolation reproduces an affine field exactly; the nodal Cauchy
-projection of a uniform stress is that stress at every node; the deck's
+projection of a uniform stress is that stress at every node, the Green and
+logarithmic strains of a homogeneous deformation are its closed forms; the deck's
 WITNESS_POINTS (with SCALE) and variable list are read."""
 import numpy as np
 
@@ -52,6 +53,28 @@ def test_nodal_cauchy_of_a_uniform_stress():
     caust = W.nodal_cauchy(coord, lnods, u, pk1)
     ref = [sig[0, 0], sig[1, 1], sig[2, 2], sig[1, 2], sig[0, 2], sig[0, 1]]
     np.testing.assert_allclose(caust, np.broadcast_to(ref, caust.shape), rtol=0, atol=1e-12)
+
+
+def test_nodal_strains_of_a_homogeneous_deformation():
+    """u = (F - I) X with F = R U (rotation times a stretch along rotated
+    axes): at every node the Green strain is (U^2 - I)/2 and the
+    logarithmic strain R log(U) R^T, whatever the mesh distortion."""
+    coord, lnods = _distorted(4)
+    a = 0.3
+    R = np.array([[np.cos(a), -np.sin(a), 0.0], [np.sin(a), np.cos(a), 0.0], [0.0, 0.0, 1.0]])
+    Q = np.linalg.qr(np.array([[1.0, 0.2, 0.1], [0.3, 1.0, -0.2], [0.1, 0.4, 1.0]]))[0]
+    lam = np.array([1.08, 0.97, 1.02])
+    U = Q @ np.diag(lam) @ Q.T
+    F = R @ U
+    u = coord @ (F - np.eye(3)).T
+    green, lep = W.nodal_strains(coord, lnods, u)
+    voigt = lambda T: [T[0, 0], T[1, 1], T[2, 2], T[1, 2], T[0, 2], T[0, 1]]
+    E = 0.5 * (U @ U - np.eye(3))
+    L = R @ Q @ np.diag(np.log(lam)) @ Q.T @ R.T
+    np.testing.assert_allclose(green, np.broadcast_to(voigt(E), green.shape), rtol=0, atol=1e-13)
+    np.testing.assert_allclose(lep, np.broadcast_to(voigt(L), lep.shape), rtol=0, atol=1e-13)
+    # witness names map to the two fields
+    assert W.VARIABLES["EPSXY"] == ("eps", 5) and W.VARIABLES["LEPZZ"] == ("lep", 2)
 
 
 def test_deck_witness_section(tmp_path):
