@@ -436,8 +436,9 @@ def witness_fields(prob, dev):
         f["a"] = dev.get("a")
     if "frx" in keys:
         f["frx"] = -dev.get("reactions")
-    if "sig" in keys:
+    if "sig" in keys or "vm" in keys:
         f["sig"] = W.nodal_cauchy(prob["coord"], prob["lnods"], u, dev.get("pk1"))
+        f["vm"] = W.von_mises(f["sig"])
     if "eps" in keys or "lep" in keys:
         f["eps"], f["lep"] = W.nodal_strains(prob["coord"], prob["lnods"], u)
     return f
@@ -457,7 +458,7 @@ def run_case(dat_path, out_dir=".", every=None, mode=0):
         welem, wxi = W.locate(prob["coord"], prob["lnods"], wit["points"])
     with Solidz(prob) as dev:
         dev.set_mode(mode)
-        if wit is not None and any(v.startswith("SIG") for v in wit["variables"]):
+        if wit is not None and any(v.startswith(("SIG", "SEQ")) for v in wit["variables"]):
             dev.set_store(True)
         dt = r["safety"] * dev.critical_dt()
         nsteps = int(np.ceil((r["t_end"] - r["t0"]) / dt))

@@ -9,7 +9,9 @@ per Gauss point sigma = F P^T / det F, lumped onto the nodes with the weights
 N_a gpvol and divided by the lumped volume sum_g N_a gpvol; the Green strain
 EPSXX..EPSXY (green_sld, E = (F^T F - I)/2) and the logarithmic strain
 LEPXX..LEPXY (lepsi_sld, the eigen form of log(F F^T)/2,
-sld_calculate_pp_tensors_ielem.f90:125-215) by the same projection.  The points come
+sld_calculate_pp_tensors_ielem.f90:125-215) by the same projection, and the
+von Mises stress SEQVM of the projected Cauchy stress (seqvm_sld,
+sld_calculate_pp_tensors.f90:136-141).  The points come
 from the WITNESS_POINTS section of the case's .ker.dat (ker_readat.f90:
 3832-3870), the variables from the WITNESS_POINTS list of its .sld.dat.
 Host-side post-processing of the device's fields; HEX08 meshes.
@@ -31,7 +33,8 @@ VARIABLES = {"DISPX": ("u", 0), "DISPY": ("u", 1), "DISPZ": ("u", 2),
              "EPSXX": ("eps", 0), "EPSYY": ("eps", 1), "EPSZZ": ("eps", 2),
              "EPSYZ": ("eps", 3), "EPSXZ": ("eps", 4), "EPSXY": ("eps", 5),
              "LEPXX": ("lep", 0), "LEPYY": ("lep", 1), "LEPZZ": ("lep", 2),
-             "LEPYZ": ("lep", 3), "LEPXZ": ("lep", 4), "LEPXY": ("lep", 5)}
+             "LEPYZ": ("lep", 3), "LEPXZ": ("lep", 4), "LEPXY": ("lep", 5),
+             "SEQVM": ("vm", 0)}
 
 
 def shape(xi):
@@ -114,6 +117,15 @@ def nodal_strains(coord, lnods, u):
     return _project(coord, lnods, gpvol, N, green), _project(coord, lnods, gpvol, N, lep)
 
 
+def von_mises(caust):
+    """[npoin, 1] von Mises stress of the nodal Cauchy stress caust [npoin, 6]
+    (xx, yy, zz, yz, xz, xy)."""
+    s = np.asarray(caust)
+    return np.sqrt(0.5 * ((s[:, 0] - s[:, 1]) ** 2 + (s[:, 1] - s[:, 2]) ** 2
+                          + (s[:, 2] - s[:, 0]) ** 2
+                          + 6.0 * (s[:, 3] ** 2 + s[:, 4] ** 2 + s[:, 5] ** 2)))[:, None]
+
+
 def nodal_cauchy(coord, lnods, u, pk1):
     """caust [npoin, 6] (xx, yy, zz, yz, xz, xy) from the first Piola stress
     pk1 [ne, 8, 9] (column-major per Gauss point, the device's layout)."""
@@ -126,7 +138,7 @@ def nodal_cauchy(coord, lnods, u, pk1):
 
 def values(lnods, elem, xi, fields, variables):
     """[nw, nvar] values of the named variables at located points; fields
-    maps "u", "v", "a", "frx", "sig", "eps", "lep" to nodal arrays.  Points
+    maps "u", "v", "a", "frx", "sig", "eps", "lep", "vm" to nodal arrays.  Points
     outside the mesh give NaN."""
     lnods = np.asarray(lnods)
     out = np.full((len(elem), len(variables)), np.nan)

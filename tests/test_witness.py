@@ -53,6 +53,9 @@ def test_nodal_cauchy_of_a_uniform_stress():
     caust = W.nodal_cauchy(coord, lnods, u, pk1)
     ref = [sig[0, 0], sig[1, 1], sig[2, 2], sig[1, 2], sig[0, 2], sig[0, 1]]
     np.testing.assert_allclose(caust, np.broadcast_to(ref, caust.shape), rtol=0, atol=1e-12)
+    # von Mises: sqrt(3/2 dev(sigma):dev(sigma))
+    d = sig - np.trace(sig) / 3.0 * np.eye(3)
+    np.testing.assert_allclose(W.von_mises(caust)[:, 0], np.sqrt(1.5 * np.sum(d * d)), rtol=1e-12)
 
 
 def test_nodal_strains_of_a_homogeneous_deformation():
