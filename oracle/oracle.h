@@ -140,6 +140,20 @@ typedef struct {
 } orc_state;
 
 int orc_sld_mass(const orc_model *m, orc_state *s);
+
+/* Planted mistakes, for tests that show a check catches them: 0 (the
+ * default) runs the restatement as written; the tests set another value,
+ * check that their closed form fails, and set it back.  Never set outside
+ * such a test. */
+#define ORC_MUT_NONE        0
+#define ORC_MUT_DT_YOUNG    1   /* sound speed sqrt(E/rho) for sqrt((lam+2mu)/rho) */
+#define ORC_MUT_DT_HMAX     2   /* hleng(1), the largest length, for hleng(ndime) */
+#define ORC_MUT_DT_NH_BULK  3   /* law 102: sqrt(K/rho), the 4 mu/3 dropped */
+#define ORC_MUT_TW_HALF     4   /* TW predictor with phi dt^2 a / 2 */
+#define ORC_MUT_TW_FIXVA    5   /* TW corrector keeps v, a on prescribed dofs */
+#define ORC_MUT_TW_VNEW     6   /* TW predictor u with v^{n+1} for v^n */
+extern int orc_mutation;
+
 double orc_sld_critical_dt(const orc_model *m);
 /* One explicit step. scheme 0 = central difference, 1 = Tchamwa-Wielgosz. */
 int orc_sld_step(const orc_model *m, orc_state *s, double dt, int scheme, double phi);

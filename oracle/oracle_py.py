@@ -58,6 +58,32 @@ def _ptr(a):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
 
 
+# Planted mistakes of the restatement (oracle.h ORC_MUT_*): a test runs its
+# check under ``planted(code)`` and expects it to fail.
+MUT_DT_YOUNG = 1
+MUT_DT_HMAX = 2
+MUT_DT_NH_BULK = 3
+MUT_TW_HALF = 4
+MUT_TW_FIXVA = 5
+MUT_TW_VNEW = 6
+
+
+class planted:
+    """Context manager: the oracle runs with mistake `code` planted."""
+
+    def __init__(self, code):
+        self.code = code
+
+    def __enter__(self):
+        self.var = C.c_int.in_dll(lib(), "orc_mutation")
+        self.var.value = self.code
+        return self
+
+    def __exit__(self, *exc):
+        self.var.value = 0
+        return False
+
+
 def sm160_eval(props, F, sdv_old, mat_of_gp=None, E_of_gp=None):
     """SoA law evaluation.  props [nmat,18]; F [9,n] col-major per point;
     sdv_old [3,n].  Returns (sdv_new [3,n], S [9,n], flags [n])."""

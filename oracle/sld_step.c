@@ -23,6 +23,8 @@
 #include <string.h>
 #include "oracle.h"
 
+int orc_mutation = ORC_MUT_NONE;
+
 static double elem_E(const orc_model *m, int64_t e, const double **props_out)
 {
     int32_t mat = m->mat_of_elem ? m->mat_of_elem[e] : 0;
@@ -96,12 +98,19 @@ double orc_sld_critical_dt(const orc_model *m)
         tr[1][2] = (-xm[0][0] * xm[1][2] + xm[1][0] * xm[0][2]) * dn;
         for (int i = 0; i < 3; i++)
             h[i] = 2.0 / sqrt(tr[i][0] * tr[i][0] + tr[i][1] * tr[i][1] + tr[i][2] * tr[i][2]);
+        /* elmlen sorts hleng descending; hmini = hleng(ndime) / porde,
+         * porde = 1 for HEX08 (sld_updtss.f90:143) */
         double hmin = fmin(h[0], fmin(h[1], h[2]));
+        if (orc_mutation == ORC_MUT_DT_HMAX) hmin = fmax(h[0], fmax(h[1], h[2]));
+        /* sld_velsnd.f90:117-124 (law 160): sqrt((lam + 2 mu)/rho) */
         double lam = nu * E / ((1.0 + nu) * (1.0 - 2.0 * nu));
         double mu = E / (2.0 * (1.0 + nu));
         double c = sqrt((lam + 2.0 * mu) / m->rho[mat]);
-        if (law == 102)   /* sld_velsnd.f90:75-78: sqrt((K0 + 4 mu0/3)/rho) */
+        if (orc_mutation == ORC_MUT_DT_YOUNG) c = sqrt(E / m->rho[mat]);
+        if (law == 102) {  /* sld_velsnd.f90:75-78: sqrt((K0 + 4 mu0/3)/rho) */
             c = sqrt((pr[1] + 4.0 * pr[0] / 3.0) / m->rho[mat]);
+            if (orc_mutation == ORC_MUT_DT_NH_BULK) c = sqrt(pr[1] / m->rho[mat]);
+        }
         double dt = hmin / c;
         if (dt < dtmin) dtmin = dt;
     }
@@ -295,8 +304,11 @@ int orc_sld_step(const orc_model *m, orc_state *s, double dt, int scheme, double
                 *v = vh;
                 *u = un;
             } else {
+                /* mod_sld_solution_methods.f90:411-421 */
                 double vn = *v + dt * *a;
                 double un = *u + dt * *v + phi * (dt * dt) * *a;
+                if (orc_mutation == ORC_MUT_TW_HALF) un = *u + dt * *v + 0.5 * phi * (dt * dt) * *a;
+                if (orc_mutation == ORC_MUT_TW_VNEW) un = *u + dt * vn + phi * (dt * dt) * *a;
                 if (fixed) un = bv;
                 *v = vn;
                 *u = un;
@@ -325,8 +337,12 @@ int orc_sld_step(const orc_model *m, orc_state *s, double dt, int scheme, double
                 s->a[3 * p + d] = acc;
                 s->v[3 * p + d] = s->v[3 * p + d] + (t_end - t_mid) * acc;
             } else {
+                /* mod_sld_solution_methods.f90:504-514 */
                 s->a[3 * p + d] = acc;
-                if (fixed) { s->v[3 * p + d] = 0.0; s->a[3 * p + d] = 0.0; }
+                if (fixed && orc_mutation != ORC_MUT_TW_FIXVA) {
+                    s->v[3 * p + d] = 0.0;
+                    s->a[3 * p + d] = 0.0;
+                }
             }
         }
     }
