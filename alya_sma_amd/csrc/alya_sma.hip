@@ -113,6 +113,9 @@ struct sld_dev {
   uint32_t* flags_out = nullptr;
   unsigned long long* counters = nullptr;
   double* clock = nullptr;           // [2] t^n in slot cur
+  sld::Health* health = nullptr;     // run health: set by the kernel that finds a failure
+  unsigned halted = 0;               // host copy of health->halted (last synchronising call)
+  std::string halt_msg;
   // energy balance, allocated when first switched on
   int energy = 0;
   double* du[3] = {};
@@ -209,7 +212,43 @@ sld::NodalArgs nodal_args(sld_dev* d, int tn) {
     A.mass = d->mass;
     for (int k = 0; k < 3; ++k) A.grav[k] = d->grav[k];
   }
+  A.health = d->health;
   return A;
+}
+
+// Clears the run health (creation and every restart from a set state).
+int clear_health(sld_dev* d) {
+  const sld::Health h{0ull, ~0ull, 0u, 0u};
+  HIPCHK(hipMemcpy(d->health, &h, sizeof(h), hipMemcpyHostToDevice));
+  d->halted = 0;
+  d->halt_msg.clear();
+  return 0;
+}
+
+// Reads the run health after the handle's stream has drained.  Returns
+// SMA_ERR_STATE (message naming the element or node) when the run halted.
+int read_health(sld_dev* d, sld::Health* out = nullptr) {
+  sld::Health h{};
+  HIPCHK(hipMemcpy(&h, d->health, sizeof(h), hipMemcpyDeviceToHost));
+  if (out) *out = h;
+  d->halted = h.halted;
+  if (!h.halted) return 0;
+  std::string msg = "run halted:";
+  if (h.halted & sld::HALT_DETF) {
+    const uint32_t lo = (uint32_t)(h.detf & 0xffffffffull);
+    const int64_t e = (int64_t)(0xffffffffu - lo);
+    uint32_t bits = (uint32_t)(h.detf >> 32);
+    float mag;
+    std::memcpy(&mag, &bits, sizeof(mag));
+    const int64_t ec = (e >= 0 && e < d->nelem) ? d->order[e] : e;
+    msg += " negative Jacobian det F = " + std::to_string(-(double)mag) + " at element " +
+           std::to_string(ec) + " (sld_detjac)";
+  }
+  if (h.halted & sld::HALT_NONFINITE)
+    msg += " NaN or infinite displacement or residual at node " + std::to_string(h.node) +
+           " (sld_cvgunk)";
+  d->halt_msg = msg;
+  return fail(SMA_ERR_STATE, msg);
 }
 
 int g_num_cu = 0;
@@ -380,6 +419,7 @@ int enqueue_step(sld_dev* d, double dt, int scheme, double phi, int tn, KernelTi
     E.P_out = d->P_out;
     E.flags_out = d->flags_out;
     E.counters = d->counters;
+    E.health = d->health;
     E.xcd_remap = d->xcd;
     const bool sur = d->mode == SLD_MODE_SURROGATE;
     if (sur) {
@@ -452,7 +492,7 @@ int enqueue_step(sld_dev* d, double dt, int scheme, double phi, int tn, KernelTi
   }
   if (d->energy)
     hipLaunchKernelGGL(sld::sld_energy_reduce, dim3(1), dim3(256), 0, s, d->energy_part, (int)nb,
-                       d->energy_acc);
+                       d->energy_acc, (const sld::Health*)d->health);
   TEND();
   HIPCHK(hipGetLastError());
   return 0;
@@ -776,6 +816,9 @@ int sld_dev_create(const sld_mesh_desc* mesh, const sld_material_desc* mat,
   HIPCHK(hipMemsetAsync(d->counters, 0, 8 * sld::COUNTER_SLOTS * sizeof(unsigned long long), s));
   TRY(dalloc(&d->clock, 2)); track(d, d->clock);
   HIPCHK(hipMemsetAsync(d->clock, 0, 2 * sizeof(double), s));
+  TRY(dalloc(&d->health, 1)); track(d, d->health);
+  HIPCHK(hipStreamSynchronize(s));
+  TRY(clear_health(d));
   TRY(dalloc(&d->mass, np)); track(d, d->mass);
   TRY(dalloc(&d->rmass, np)); track(d, d->rmass);
   HIPCHK(hipMemsetAsync(d->mass, 0, np * sizeof(double), s));
@@ -1017,6 +1060,7 @@ int sld_dev_step(sld_dev* d, double dt, int32_t scheme, double phi, int64_t nste
   if (nsteps < 0) return fail(SMA_ERR_ARG, "nsteps must be >= 0");
   if (d->mode == SLD_MODE_SURROGATE && !surrogate_attached(d))
     return fail(SMA_ERR_STATE, "surrogate mode without a surrogate");
+  if (d->halted) return fail(SMA_ERR_STATE, d->halt_msg);
   int64_t left = nsteps;
   // runs of 64, 16 and 2 steps replay captured graphs (an even length
   // returns the history and clock slots to where they started); inside a run
@@ -1057,6 +1101,7 @@ int sld_dev_time_kernels(sld_dev* d, double dt, int32_t scheme, double phi, int3
   if (!(dt > 0.0) || nsteps <= 0) return fail(SMA_ERR_ARG, "dt > 0 and nsteps > 0 required");
   if (d->mode == SLD_MODE_SURROGATE && !surrogate_attached(d))
     return fail(SMA_ERR_STATE, "surrogate mode without a surrogate");
+  if (d->halted) return fail(SMA_ERR_STATE, d->halt_msg);
   KernelTimer T;
   T.s = d->stream;
   T.elem_reps = 8;
@@ -1093,6 +1138,7 @@ int sld_dev_time_kernels(sld_dev* d, double dt, int32_t scheme, double phi, int3
   HIPCHK(hipEventElapsedTime(&tot, t0, t1));
   hipEventDestroy(t0);
   hipEventDestroy(t1);
+  if (int rh = read_health(d)) return rh;
   out[0] = cnt[0] ? sum[0] / cnt[0] : 0.0;    // predictor, ms per launch
   // element kernel, ms per launch: the element phase (all its launches,
   // back to back) divided by its number of launches; the gather phase counts
@@ -1117,17 +1163,33 @@ int sld_dev_autotune(sld_dev* d, double dt, int32_t scheme, double phi, int32_t 
     return fail(SMA_ERR_STATE, "surrogate mode without a surrogate");
   hipStream_t s = d->stream;
   HIPCHK(hipStreamSynchronize(s));
-  // snapshot of everything a step writes, restored before every trial and at the end
+  if (int rh = read_health(d)) return rh;
+  // snapshot of everything a step writes, restored before every trial and
+  // at the end: the state, the assembled and corner forces, the stored
+  // stress fields, the energy balance, the counters and the run health
   const int64_t np = d->npoin, ngp = d->nelem * 8;
   std::vector<std::pair<void*, size_t>> live;
   for (int k = 0; k < 3; ++k) {
     live.push_back({d->u[k], np * 8});
     live.push_back({d->v[k], np * 8});
     live.push_back({d->a[k], np * 8});
+    live.push_back({d->fint64[k], np * 8});
+    live.push_back({d->fint32[k], np * 4});
     for (int sl = 0; sl < 2; ++sl) live.push_back({d->sdv[sl][k], ngp * 8});
+    if (d->du[0]) {
+      live.push_back({d->du[k], np * 8});
+      live.push_back({d->fint_prev[k], np * 8});
+      live.push_back({d->fext_prev[k], np * 8});
+    }
   }
+  live.push_back({d->efor, d->efor_bytes});
+  if (d->S_out) live.push_back({d->S_out, 9 * ngp * 8});
+  if (d->P_out) live.push_back({d->P_out, 9 * ngp * 8});
+  if (d->flags_out) live.push_back({d->flags_out, ngp * 4});
+  if (d->energy_acc) live.push_back({d->energy_acc, 5 * 8});
   live.push_back({d->clock, 16});
   live.push_back({d->counters, 8 * sld::COUNTER_SLOTS * sizeof(unsigned long long)});
+  live.push_back({d->health, sizeof(sld::Health)});
   std::vector<void*> snap(live.size(), nullptr);
   auto cleanup = [&]() { for (void* p : snap) if (p) (void)hipFree(p); };
   for (size_t i = 0; i < live.size(); ++i) {
@@ -1204,7 +1266,7 @@ void* sld_dev_stream(sld_dev* d) { return d ? (void*)d->stream : nullptr; }
 int sld_dev_sync(sld_dev* d) {
   if (!d) return fail(SMA_ERR_ARG, "null handle");
   HIPCHK(hipStreamSynchronize(d->stream));
-  return SMA_OK;
+  return read_health(d);
 }
 
 int sld_dev_coloring(const sld_dev* d, int32_t* ncolor, int32_t* color_ptr, int32_t max_colors,
@@ -1224,6 +1286,14 @@ int sld_dev_get(sld_dev* d, int32_t field, void* host, int64_t count) {
   if (!d || !host) return fail(SMA_ERR_ARG, "null argument");
   HIPCHK(hipStreamSynchronize(d->stream));
   const int64_t np = d->npoin, ne = d->nelem, ngp = ne * 8;
+  // the fields stay readable after a halt (the state the failure left);
+  // the host copy of the halt is refreshed so the next step refuses to run
+  sld::Health h{};
+  {
+    const std::string keep = sma_abi::last_error;
+    (void)read_health(d, &h);
+    sma_abi::last_error = keep;
+  }
   auto need = [&](int64_t n) { return count == n ? 0 : fail(SMA_ERR_ARG, "count does not match field size"); };
   std::vector<double> tmp;
   switch (field) {
@@ -1308,6 +1378,24 @@ int sld_dev_get(sld_dev* d, int32_t field, void* host, int64_t count) {
       if (need(1)) return SMA_ERR_ARG;
       HIPCHK(hipMemcpy(host, d->clock + d->cur, sizeof(double), hipMemcpyDeviceToHost));
       return SMA_OK;
+    case SLD_FIELD_HEALTH: {
+      if (need(4)) return SMA_ERR_ARG;
+      double* out = (double*)host;
+      out[0] = (double)h.halted;
+      out[1] = -1.0;
+      out[2] = 0.0;
+      out[3] = -1.0;
+      if (h.halted & sld::HALT_DETF) {
+        const int64_t e = (int64_t)(0xffffffffu - (uint32_t)(h.detf & 0xffffffffull));
+        const uint32_t bits = (uint32_t)(h.detf >> 32);
+        float mag;
+        std::memcpy(&mag, &bits, sizeof(mag));
+        out[1] = (double)((e >= 0 && e < ne) ? d->order[e] : e);
+        out[2] = -(double)mag;
+      }
+      if (h.halted & sld::HALT_NONFINITE) out[3] = (double)h.node;
+      return SMA_OK;
+    }
     case SLD_FIELD_COUNTERS: {
       if (need(8)) return SMA_ERR_ARG;
       std::vector<unsigned long long> slots(8 * sld::COUNTER_SLOTS);
@@ -1329,6 +1417,10 @@ int sld_dev_set(sld_dev* d, int32_t field, const void* host, int64_t count) {
   HIPCHK(hipStreamSynchronize(d->stream));
   const int64_t np = d->npoin, ne = d->nelem, ngp = ne * 8;
   std::vector<double> tmp;
+  // a state set by the caller (restart) starts a healthy run
+  if (field == SLD_FIELD_U || field == SLD_FIELD_V || field == SLD_FIELD_A ||
+      field == SLD_FIELD_SDV || field == SLD_FIELD_TIME)
+    if (int rc = clear_health(d)) return rc;
   switch (field) {
     case SLD_FIELD_U: case SLD_FIELD_V: case SLD_FIELD_A: {
       if (count != 3 * np) return fail(SMA_ERR_ARG, "count does not match field size");

@@ -145,6 +145,35 @@ __device__ __forceinline__ double funbou(const double* __restrict__ tab, int npt
   return x;
 }
 
+// Run health.  The reference stops the run on a non-positive Jacobian det F
+// at a Gauss point (sld_elmpre.f90:229-240 keeps the smallest one and its
+// element, sld_elmope.f90:106 -> sld_detjac.f90:93 calls runend) and on a
+// NaN or infinite displacement or force residual (sld_cvgunk.f90:195-199).
+// Here the kernel that meets such a state records it and sets `halted`;
+// every step kernel returns at once while `halted` is set, so the device
+// state stays as it was when the failure was found and the host reports it.
+struct Health {
+  // 0 = none; else (|det F| as FP32 bits) << 32 | (0xffffffff - element):
+  // atomicMax keeps the most negative det F, ties to the lowest element
+  unsigned long long detf;
+  unsigned long long node;   // ~0 = none; else the lowest node with a non-finite value
+  unsigned halted;           // bit 0: det F <= 0, bit 1: non-finite state
+  unsigned pad;
+};
+constexpr unsigned HALT_DETF = 1u, HALT_NONFINITE = 2u;
+
+__device__ __forceinline__ void report_detf(Health* h, int64_t e, double detF) {
+  const float mag = fabsf((float)detF);
+  const unsigned long long key = ((unsigned long long)__float_as_uint(mag) << 32) |
+                                 (unsigned long long)(0xffffffffu - (uint32_t)e);
+  atomicMax(&h->detf, key);
+  atomicOr(&h->halted, HALT_DETF);
+}
+__device__ __forceinline__ void report_nonfinite(Health* h, int64_t node) {
+  atomicMin(&h->node, (unsigned long long)node);
+  atomicOr(&h->halted, HALT_NONFINITE);
+}
+
 // fixmask bits: 1, 2, 4 = prescribed x, y, z; FIX_FORCE = the node carries
 // a nonzero entry of the nodal force pattern fext0 (set by the library).
 constexpr int FIX_FORCE = 8;
@@ -188,6 +217,7 @@ struct NodalArgs {
   // gravity (sld_dev_set_gravity): f_ext += m g at every dof; mass is set
   int32_t gravity;
   double grav[3];
+  Health* health;              // run health (never null)
 };
 
 // Boundary faces with Neumann loads (sld_bouope.f90), one QUA04 face per
@@ -272,6 +302,7 @@ __device__ __forceinline__ void store_prediction(const NodalArgs& A, int64_t i, 
 
 template <int SCHEME>  // 0 = central difference, 1 = Tchamwa-Wielgosz
 __global__ void __launch_bounds__(256) sld_predict(NodalArgs A, double dt, double phi) {
+  if (A.health->halted) return;
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= A.npoin) return;
   double u[3], v[3], a[3], un[3], vn[3];
@@ -283,6 +314,7 @@ __global__ void __launch_bounds__(256) sld_predict(NodalArgs A, double dt, doubl
   }
   predict_vals(A, i, SCHEME, dt, phi, A.clock[A.tn] + dt, A.fixmask[i], u, v, a, un, vn);
   store_prediction(A, i, u, un, vn);
+  if (!(isfinite(un[0]) && isfinite(un[1]) && isfinite(un[2]))) report_nonfinite(A.health, i);
 }
 
 // Gather assembly: f_int at node i = 0 + f_(e1) + f_(e2) + ... over the
@@ -405,11 +437,13 @@ __device__ __forceinline__ void correct_node(const NodalArgs& A, int64_t i, doub
     for (int d = 0; d < 3; ++d) fe[d] = fe[d] + m * A.grav[d];
   }
   double an[3], vn[3];
+  bool finite = true;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     double fi = fg[d];
     double r = 0.0 - fi;
     r = r + fe[d];
+    finite = finite && isfinite(r);
     double acc = 0.0 + r * rm;
     if (SCHEME == 0) {
       an[d] = acc;
@@ -425,10 +459,12 @@ __device__ __forceinline__ void correct_node(const NodalArgs& A, int64_t i, doub
     }
   }
   if (A.energy_part) node_energy(A, i, fg, fe, vn, e);
+  if (!finite) report_nonfinite(A.health, i);
   if (FUSE) {
     double un[3], vp[3];
     predict_vals(A, i, SCHEME, dt, phi, cutim + dt, fix, u0, vn, an, un, vp);
     store_prediction(A, i, u0, un, vp);
+    if (!(isfinite(un[0]) && isfinite(un[1]) && isfinite(un[2]))) report_nonfinite(A.health, i);
   }
   if (i == 0) {
     A.clock[1 - A.tn] = cutim;
@@ -459,6 +495,7 @@ __device__ __forceinline__ void block_sum3(double e[3], double* part) {
 template <int SCHEME, bool FUSE>
 __global__ void __launch_bounds__(256) sld_correct(NodalArgs A, double dt, double phi) {
 #pragma clang fp contract(off)
+  if (A.health->halted) return;
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double e[3] = {0.0, 0.0, 0.0};
   if (i < A.npoin) correct_node<SCHEME, FUSE>(A, i, dt, phi, e);
@@ -470,8 +507,9 @@ __global__ void __launch_bounds__(256) sld_correct(NodalArgs A, double dt, doubl
 // first step after a reset takes the full increment, later ones half the
 // trapezoid.
 __global__ void __launch_bounds__(256) sld_energy_reduce(const double* part, int nparts,
-                                                         double* acc) {
+                                                         double* acc, const Health* health) {
 #pragma clang fp contract(off)
+  if (health->halted) return;
   double e[3] = {0.0, 0.0, 0.0};
   for (int b = threadIdx.x; b < nparts; b += 256) {
 #pragma unroll
@@ -611,6 +649,7 @@ struct ElemArgs {
   double* P_out;                // optional first Piola F S [9][ngp] (with S_out)
   uint32_t* flags_out;          // optional [ngp]
   unsigned long long* counters; // [COUNTER_SLOTS][8] per-flag Gauss-point counts
+  Health* health;               // run health (never null)
   int32_t xcd_remap;            // 1 = spread consecutive tiles per XCD
   int32_t tiles_per_block;      // > 0: persistent launch, tiles per workgroup
   // surrogate (mode 2): per-GP status (0 exact, 1 accepted, 2 fallback) and
@@ -685,6 +724,7 @@ __global__ void __launch_bounds__(EPB * 8, (ElemOccupancy<R, LAWS>::waves)) sld_
   __shared__ R s_q[EPB][9][8];            // Q(3i+k) at each Gauss point
   __shared__ R s_u4[EPB][8][4];           // U0, U1, U2, U12 of Q(0..7)
 
+  if (A.health->halted) return;
   const int tid = threadIdx.x;
   const int le = tid >> 3, g = tid & 7;
   int bid = blockIdx.x;
@@ -919,7 +959,12 @@ __global__ void __launch_bounds__(EPB * 8, (ElemOccupancy<R, LAWS>::waves)) sld_
     // Q = detJ * tau * j^-T
     // det F = detj / detJ > 0 (false for NaN)
     const bool detf_pos = detj > R(0) ? detJ > R(0) : (detj < R(0) && detJ < R(0));
-    if (!detf_pos) fl |= sma::FLAG_DETF_NONPOS;
+    if (!detf_pos) {
+      fl |= sma::FLAG_DETF_NONPOS;
+      // det F <= 0 stops the run (NaN is left to the residual check)
+      if ((detj <= R(0) && detJ > R(0)) || (detj >= R(0) && detJ < R(0)))
+        report_detf(A.health, e, (double)detj / (double)detJ);
+    }
     const R T[9] = {tau[0], tau[1], tau[2], tau[1], tau[3], tau[4], tau[2], tau[4], tau[5]};
 #pragma unroll
     for (int i = 0; i < 3; ++i)

@@ -232,13 +232,14 @@ class Solidz:
     # ----------------------------------------------------------------- fields
     def get(self, field):
         f = {"u": 0, "v": 1, "a": 2, "fint": 3, "mass": 4, "sdv": 5, "stress": 6,
-             "flags": 7, "time": 8, "counters": 9, "pk1": 10, "reactions": 11}[field]
+             "flags": 7, "time": 8, "counters": 9, "pk1": 10, "reactions": 11,
+             "health": 12}[field]
         n, e = self.npoin, self.nelem
         shape, dt = {0: ((n, 3), np.float64), 1: ((n, 3), np.float64), 2: ((n, 3), np.float64),
                      3: ((n, 3), np.float64), 4: ((n,), np.float64), 5: ((e, 8, 3), np.float64),
                      6: ((e, 8, 9), np.float64), 7: ((e, 8), np.uint32), 8: ((1,), np.float64),
                      9: ((8,), np.uint64), 10: ((e, 8, 9), np.float64),
-                     11: ((n, 3), np.float64)}[f]
+                     11: ((n, 3), np.float64), 12: ((4,), np.float64)}[f]
         out = np.zeros(shape, dtype=dt)
         check(lib().sld_dev_get(self.h, f, _p(out), out.size), f"sld_dev_get({field})")
         return out
@@ -247,6 +248,13 @@ class Solidz:
         f = {"u": 0, "v": 1, "a": 2, "sdv": 5, "time": 8}[field]
         a = np.ascontiguousarray(value, dtype=np.float64)
         check(lib().sld_dev_set(self.h, f, _p(a), a.size), f"sld_dev_set({field})")
+
+    def health(self):
+        """(halt bits, element of the most negative det F or -1, that det F,
+        lowest node with a non-finite value or -1); bits 1 = det F <= 0
+        (sld_detjac), 2 = non-finite displacement or residual (sld_cvgunk)."""
+        h = self.get("health")
+        return int(h[0]), int(h[1]), float(h[2]), int(h[3])
 
     def counters(self):
         c = self.get("counters")

@@ -130,7 +130,16 @@ int sld_dev_set_store(sld_dev *dev, int32_t store);
 
 /* Advance nsteps explicit steps of size dt.  scheme 0 = central difference
  * (sld_explicit), 1 = Tchamwa-Wielgosz with parameter phi (sld_explicit_tw).
- * Asynchronous on the handle's stream; steps are replayed from a hipGraph. */
+ * Asynchronous on the handle's stream; steps are replayed from a hipGraph.
+ * Failure stops the run as the reference's runend does: det F <= 0 at a
+ * Gauss point (sld_elmope.f90:106 -> sld_detjac.f90:93) or a NaN/infinite
+ * displacement or residual (sld_cvgunk.f90:195-199).  The kernel that finds
+ * it halts the device (later steps do nothing, the state stays as the
+ * failure left it) and the next synchronising call -- sld_dev_sync,
+ * sld_dev_time_kernels, sld_dev_autotune -- returns SMA_ERR_STATE naming the
+ * element (most negative det F) or node; from then on sld_dev_step returns
+ * SMA_ERR_STATE at once.  sld_dev_get still reads the fields (and
+ * SLD_FIELD_HEALTH); sld_dev_set of the state clears the halt. */
 int sld_dev_step(sld_dev *dev, double dt, int32_t scheme, double phi, int64_t nsteps);
 
 /* Runs nsteps steps eagerly with HIP events around the predictor, the
@@ -170,10 +179,17 @@ double sld_dev_critical_dt(const sld_dev *dev);
                                   (sld_elmmat.f90:228-237; no element body forces
                                   here, so -f_int); Alya's node-set FRXIX is its
                                   negative (sld_nodset.f90:100) */
+#define SLD_FIELD_HEALTH   12  /* [4] run health: {halt bits (1 = det F <= 0 at a Gauss
+                                  point, sld_elmpre.f90:229-240 -> sld_detjac.f90:93;
+                                  2 = NaN or infinite displacement or residual,
+                                  sld_cvgunk.f90:195-199), element of the most
+                                  negative det F (caller numbering) or -1, that
+                                  det F (FP32 precision) or 0, lowest node with a
+                                  non-finite value or -1} */
 int sld_dev_get(sld_dev *dev, int32_t field, void *host, int64_t count);
 
 /* Overwrite state (restart, sld_restar.f90:63-65): fields U, V, A, SDV
- * (both history slots) and TIME. */
+ * (both history slots) and TIME.  Setting any of them clears a halt. */
 int sld_dev_set(sld_dev *dev, int32_t field, const void *host, int64_t count);
 
 /* Colouring used by the handle: colour_ptr [ncolor+1] and elem_order

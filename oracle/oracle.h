@@ -137,6 +137,14 @@ typedef struct {
     double *fint_prev;
     double *fext_prev;         /* [npoin][3] f_ext of the previous step      */
     double allie, allwk, allke, etota;
+    /* run stop (sld_detjac / sld_cvgunk runend): 0, or 1 = det F <= 0 at a
+     * Gauss point (the step ends after the element loop), 2 = a NaN or
+     * infinite predicted displacement (the step ends after the predictor)
+     * or residual (after the corrector) */
+    int32_t halt;
+    int64_t halt_elem;         /* element of the smallest det F (first in loop order) */
+    double halt_detf;
+    int64_t halt_node;         /* lowest node with a non-finite value */
 } orc_state;
 
 int orc_sld_mass(const orc_model *m, orc_state *s);
@@ -155,7 +163,8 @@ int orc_sld_mass(const orc_model *m, orc_state *s);
 extern int orc_mutation;
 
 double orc_sld_critical_dt(const orc_model *m);
-/* One explicit step. scheme 0 = central difference, 1 = Tchamwa-Wielgosz. */
+/* One explicit step. scheme 0 = central difference, 1 = Tchamwa-Wielgosz.
+ * Returns s->halt (0 = completed). */
 int orc_sld_step(const orc_model *m, orc_state *s, double dt, int scheme, double phi);
 /* Evaluate load function f at time t (sld_funbou.f90 itask 2, component c). */
 double orc_funbou(const orc_model *m, int32_t f, int c, double t);

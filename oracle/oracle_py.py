@@ -176,7 +176,9 @@ class _State(C.Structure):
                 ("t", C.c_double), ("nsteps", C.c_int64),
                 ("fint_prev", C.c_void_p), ("fext_prev", C.c_void_p),
                 ("allie", C.c_double), ("allwk", C.c_double),
-                ("allke", C.c_double), ("etota", C.c_double)]
+                ("allke", C.c_double), ("etota", C.c_double),
+                ("halt", C.c_int32), ("halt_elem", C.c_int64),
+                ("halt_detf", C.c_double), ("halt_node", C.c_int64)]
 
 
 class StepOracle:
@@ -260,8 +262,20 @@ class StepOracle:
         return lib().orc_sld_critical_dt(C.byref(self.m))
 
     def step(self, dt, scheme=0, phi=1.033, n=1):
+        """n steps; returns the run-stop code (0, 1 = det F <= 0, 2 = non-finite)."""
+        rc = 0
         for _ in range(n):
-            lib().orc_sld_step(C.byref(self.m), C.byref(self.s), dt, scheme, phi)
+            rc = lib().orc_sld_step(C.byref(self.m), C.byref(self.s), dt, scheme, phi)
+            if rc:
+                break
+        return rc
+
+    @property
+    def halt(self):
+        """(code, element, det F, node) of a stopped run."""
+        s = self.s
+        return (s.halt, s.halt_elem if s.halt == 1 else -1, s.halt_detf if s.halt == 1 else 0.0,
+                s.halt_node if s.halt == 2 else -1)
 
     @property
     def t(self):

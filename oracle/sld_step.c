@@ -248,6 +248,16 @@ static void element_loop(const orc_model *m, orc_state *s)
                     fl = orc_sm160_point(props, Fg, sn, sk, S, 0);
                 }
                 if (!(detF[g] > 0.0)) fl |= ORC_FLAG_DETF_NONPOS;
+                /* sld_elmpre.f90:229-240: the smallest det F <= 0 and its element */
+                if (detF[g] <= 0.0) {
+                    #pragma omp critical (orc_detf_min)
+                    if (!s->halt || detF[g] < s->halt_detf ||
+                        (detF[g] == s->halt_detf && e < s->halt_elem)) {
+                        s->halt = 1;
+                        s->halt_detf = detF[g];
+                        s->halt_elem = e;
+                    }
+                }
                 if (s->flags) s->flags[8 * e + g] = fl;
                 if (s->S) memcpy(s->S + 72 * e + 9 * g, S, sizeof(S));
                 /* P_jk = sum_i F_ji S_ik  (sld_builtin_materials.f90:359-369) */
@@ -278,6 +288,7 @@ static void element_loop(const orc_model *m, orc_state *s)
 
 int orc_sld_step(const orc_model *m, orc_state *s, double dt, int scheme, double phi)
 {
+    if (s->halt) return s->halt;   /* a stopped run stays stopped */
     int64_t np = m->npoin;
     double cutim = s->t + dt;             /* setgts.f90:98 */
     double t_end = cutim + dt, t_mid = 0.5 * (cutim + t_end);
@@ -315,7 +326,24 @@ int orc_sld_step(const orc_model *m, orc_state *s, double dt, int scheme, double
             }
         }
     }
+    /* a non-finite predicted displacement stops the run (sld_cvgunk.f90:
+     * 195-199, the displacement residual) */
+    for (int64_t p = 0; p < np; p++)
+        for (int d = 0; d < 3; d++)
+            if (!isfinite(s->u[3 * p + d]) && !s->halt) {
+                s->halt = 2;
+                s->halt_node = p;
+            }
+    if (s->halt) {
+        free(u_old);
+        return s->halt;
+    }
     element_loop(m, s);
+    /* det F <= 0 stops the run after the element loop (sld_elmope.f90:106) */
+    if (s->halt) {
+        free(u_old);
+        return s->halt;
+    }
     /* f_ext = face loads (sld_bouope, assembled after the element loop in
      * sld_matrix.f90:221) + the nodal force pattern */
     double *fe = (double *)calloc((size_t)3 * np, sizeof(double));
@@ -327,6 +355,11 @@ int orc_sld_step(const orc_model *m, orc_state *s, double dt, int scheme, double
         for (int64_t p = 0; p < np; p++)
             for (int d = 0; d < 3; d++) fe[3 * p + d] = fe[3 * p + d] + s->mass[p] * m->gravity[d];
     for (int64_t p = 0; p < 3 * np; p++) s->rhs[p] = s->rhs[p] + fe[p];
+    /* a non-finite residual stops the run after the step (sld_cvgunk.f90:195-199) */
+    int64_t bad_node = -1;
+    for (int64_t p = np - 1; p >= 0; p--)
+        for (int d = 0; d < 3; d++)
+            if (!isfinite(s->rhs[3 * p + d])) bad_node = p;
     /* solve + corrector */
     for (int64_t p = 0; p < np; p++) {
         double fact1 = 1.0 / s->mass[p];
@@ -372,5 +405,9 @@ int orc_sld_step(const orc_model *m, orc_state *s, double dt, int scheme, double
     memcpy(s->sdv + 24 * m->nelem, s->sdv, sizeof(double) * 24 * m->nelem);
     s->t = cutim;
     s->nsteps++;
-    return 0;
+    if (bad_node >= 0) {
+        s->halt = 2;
+        s->halt_node = bad_node;
+    }
+    return s->halt;
 }
