@@ -149,6 +149,16 @@ typedef struct {
 
 int orc_sld_mass(const orc_model *m, orc_state *s);
 
+/* Phases of the step, for the FLOP-counting build (oracle/flopcount.cpp,
+ * which defines ORC_PHASE); a no-op in the oracle library. */
+#ifndef ORC_PHASE
+#define ORC_PHASE(k) ((void)0)
+#endif
+#define ORC_PH_NODAL 0   /* predictor, loads, solve, corrector */
+#define ORC_PH_KIN   1   /* element gather, Jacobian, gpcar, F, det F */
+#define ORC_PH_LAW   2   /* constitutive law at a Gauss point */
+#define ORC_PH_FORCE 3   /* P = F S, element internal force, assembly */
+
 /* Planted mistakes, for tests that show a check catches them: 0 (the
  * default) runs the restatement as written; the tests set another value,
  * check that their closed form fails, and set it back.  Never set outside
@@ -163,6 +173,8 @@ int orc_sld_mass(const orc_model *m, orc_state *s);
 extern int orc_mutation;
 
 double orc_sld_critical_dt(const orc_model *m);
+/* OpenMP threads of the element loop (no-op without OpenMP). */
+void orc_set_threads(int n);
 /* One explicit step. scheme 0 = central difference, 1 = Tchamwa-Wielgosz.
  * Returns s->halt (0 = completed). */
 int orc_sld_step(const orc_model *m, orc_state *s, double dt, int scheme, double phi);

@@ -51,11 +51,17 @@ def lib():
         _lib.orc_funbou.argtypes = [p, C.c_int32, C.c_int, d]
         _lib.orc_funbou.restype = d
         _lib.orc_face_loads.argtypes = [p, p, d, p]
+        _lib.orc_set_threads.argtypes = [C.c_int]
     return _lib
 
 
 def _ptr(a):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def set_threads(n):
+    """OpenMP threads of the oracle's colour-parallel element loop."""
+    lib().orc_set_threads(int(n))
 
 
 # Planted mistakes of the restatement (oracle.h ORC_MUT_*): a test runs its

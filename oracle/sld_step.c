@@ -25,6 +25,20 @@
 
 int orc_mutation = ORC_MUT_NONE;
 
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+/* Threads of the colour-parallel element loop (the CPU baseline times 1 and
+ * all of the host's cores). */
+void orc_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
 static double elem_E(const orc_model *m, int64_t e, const double **props_out)
 {
     int32_t mat = m->mat_of_elem ? m->mat_of_elem[e] : 0;
@@ -232,6 +246,7 @@ static void element_loop(const orc_model *m, orc_state *s)
             const int law = m->law_of_mat ? m->law_of_mat[mat] : 160;
             memcpy(props, pr, sizeof(props));
             if (law == 160) props[0] = E;
+            ORC_PHASE(ORC_PH_KIN);
             gather(m, m->coord, e, elcod);
             gather(m, s->u, e, eldis);
             orc_hex8_kinematics(elcod, eldis, gpcar, gpvol, F, detF);
@@ -240,6 +255,7 @@ static void element_loop(const orc_model *m, orc_state *s)
                 const double *Fg = F + 9 * g;
                 double *sk = sdv_k + 24 * e + 3 * g, *sn = sdv_n + 24 * e + 3 * g;
                 uint32_t fl = 0;
+                ORC_PHASE(ORC_PH_LAW);
                 if (law == 102) {
                     /* no history: TIME_N carries over to ITER_K unchanged */
                     orc_nh102_point(props, Fg, detF[g], S);
@@ -247,6 +263,7 @@ static void element_loop(const orc_model *m, orc_state *s)
                 } else {
                     fl = orc_sm160_point(props, Fg, sn, sk, S, 0);
                 }
+                ORC_PHASE(ORC_PH_FORCE);
                 if (!(detF[g] > 0.0)) fl |= ORC_FLAG_DETF_NONPOS;
                 /* sld_elmpre.f90:229-240: the smallest det F <= 0 and its element */
                 if (detF[g] <= 0.0) {
@@ -282,6 +299,7 @@ static void element_loop(const orc_model *m, orc_state *s)
                     s->fint[3 * p + i] += elfint[3 * a + i];
                 }
             }
+            ORC_PHASE(ORC_PH_NODAL);
         }
     }
 }

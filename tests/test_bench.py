@@ -20,21 +20,36 @@ def test_more_than_one_gpu_exits_with_a_message():
     assert "one GPU only" in r.stdout
 
 
-def test_algorithmic_bytes_gather():
-    # per GP: 48 history + 4 connectivity + 2 moduli + 4 slot + 3 x 4 (FP32)
-    # or 3 x 8 (FP64) corner force; per node 48
+def test_algorithmic_bytes():
+    # per GP: 48 history + 4 connectivity + 1 per-element modulus; per node
+    # 48 read + the internal force written (3 x 8 or 3 x 4)
     prob = problems.cantilever_problem(nx=10, ny=2, nz=2, lx=10.0, ly=2.0, lz=2.0)
     ne, npn = 40, 11 * 3 * 3
-    assert bench.algorithmic_bytes(prob, None, "fp32", "gather") == ne * 8 * 70 + npn * 48
-    assert bench.algorithmic_bytes(prob, None, "fp64", "gather") == ne * 8 * 82 + npn * 48
+    assert bench.algorithmic_bytes(prob, "fp64") == ne * 8 * 53 + npn * 72
+    assert bench.algorithmic_bytes(prob, "fp32") == ne * 8 * 53 + npn * 60
+    # the design's gather layout: 2 B moduli, 4 B slot and the corner force
+    assert bench.design_bytes(prob, "fp32", "gather") == ne * 8 * 70 + npn * 48
+    assert bench.design_bytes(prob, "fp64", "gather") == ne * 8 * 82 + npn * 48
 
 
 def test_algorithmic_bytes_without_history_on_neo_hookean_elements():
     prob = problems.cylinder_problem(ntheta=8, nz=2, nr_tube=1, nr_tissue=2)
     ne, npn = prob["lnods"].shape[0], prob["coord"].shape[0]
     n_sma = int(np.count_nonzero(prob["mat_of_elem"] == 0))
-    want = 8 * (n_sma * 54 + (ne - n_sma) * 6) + ne * 8 * (4 + 12) + npn * 48
-    assert bench.algorithmic_bytes(prob, None, "fp32", "gather") == want
+    assert prob.get("E_of_elem") is None
+    assert bench.algorithmic_bytes(prob, "fp32") == 8 * (n_sma * 52 + (ne - n_sma) * 4) + npn * 60
+
+
+def test_algorithmic_flops_mix():
+    prob = problems.cylinder_problem(ntheta=8, nz=2, nr_tube=1, nr_tissue=2)
+    ne = prob["lnods"].shape[0]
+    n_sma = 8 * int(np.count_nonzero(prob["mat_of_elem"] == 0))
+    f = bench.FLOPS_PER_GP
+    base = 8 * ne * (f["kinematics"] + f["force"]) + n_sma * f["law_elastic"] \
+        + (8 * ne - n_sma) * f["law_nh102"]
+    assert bench.algorithmic_flops(prob, 0, 0) == base
+    assert bench.algorithmic_flops(prob, 10, 4) == \
+        base + 10 * (f["law_AS"] - f["law_elastic"]) + 4 * (f["law_SA"] - f["law_elastic"])
 
 
 def test_kernel_names_and_committed_traffic():
