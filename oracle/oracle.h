@@ -112,6 +112,15 @@ typedef struct {
      * lumped mass times g */
     int32_t has_gravity;
     double gravity[3];
+    /* Rayleigh damping dampi_sld(1:2, imate) = (alpha, beta) per material
+     * (RAYLEIGH_DAMPING: RAYALPHA, RAYBETA, sld_reaphy.f90:1046-1049), or
+     * NULL; terms of sld_elmmat.f90:160-163 and :174-194 */
+    const double *damping;     /* [nmat][2] */
+    /* body force bodyf_sld(t^{n+1}) of load function body_func (BODYF,
+     * sld_reabcs.f90:523; evaluated by sld_funbou.f90 itask 3 in
+     * sld_updbcs.f90:96-99), -1 = none; element load of sld_elmmat.f90:
+     * 156-157, the lumped mass times the body force */
+    int32_t body_func;
 } orc_model;
 
 /* Nodal forces of the boundary faces at time t over the geometry
@@ -170,6 +179,10 @@ int orc_sld_mass(const orc_model *m, orc_state *s);
 #define ORC_MUT_TW_HALF     4   /* TW predictor with phi dt^2 a / 2 */
 #define ORC_MUT_TW_FIXVA    5   /* TW corrector keeps v, a on prescribed dofs */
 #define ORC_MUT_TW_VNEW     6   /* TW predictor u with v^{n+1} for v^n */
+#define ORC_MUT_DAMP_VHALF  7   /* CD damping with v^{n+1/2} for veloc_sld(ITER_K) = v^n */
+#define ORC_MUT_DAMP_LUMPED 8   /* alpha term with the lumped mass (node velocity) */
+#define ORC_MUT_BETA_T      9   /* beta term with the velocity gradient transposed */
+#define ORC_MUT_BODY_LE     10  /* body-force table with t <= t2 for t < t2 */
 extern int orc_mutation;
 
 double orc_sld_critical_dt(const orc_model *m);
@@ -180,6 +193,9 @@ void orc_set_threads(int n);
 int orc_sld_step(const orc_model *m, orc_state *s, double dt, int scheme, double phi);
 /* Evaluate load function f at time t (sld_funbou.f90 itask 2, component c). */
 double orc_funbou(const orc_model *m, int32_t f, int c, double t);
+/* Body-force value of load function f at time t (sld_funbou.f90 itask 3,
+ * component c): linear between table rows, zero outside the table. */
+double orc_funbou_body(const orc_model *m, int32_t f, int c, double t);
 
 #ifdef __cplusplus
 }

@@ -50,6 +50,8 @@ def lib():
         _lib.orc_sld_step.argtypes = [p, p, d, C.c_int, d]
         _lib.orc_funbou.argtypes = [p, C.c_int32, C.c_int, d]
         _lib.orc_funbou.restype = d
+        _lib.orc_funbou_body.argtypes = [p, C.c_int32, C.c_int, d]
+        _lib.orc_funbou_body.restype = d
         _lib.orc_face_loads.argtypes = [p, p, d, p]
         _lib.orc_set_threads.argtypes = [C.c_int]
     return _lib
@@ -72,6 +74,10 @@ MUT_DT_NH_BULK = 3
 MUT_TW_HALF = 4
 MUT_TW_FIXVA = 5
 MUT_TW_VNEW = 6
+MUT_DAMP_VHALF = 7
+MUT_DAMP_LUMPED = 8
+MUT_BETA_T = 9
+MUT_BODY_LE = 10
 
 
 class planted:
@@ -172,7 +178,8 @@ class _Model(C.Structure):
                 ("nface", C.c_int64), ("lnodb", C.c_void_p), ("face_code", C.c_void_p),
                 ("face_value", C.c_void_p), ("face_func", C.c_void_p),
                 ("face_transient", C.c_int32), ("face_follower", C.c_int32),
-                ("has_gravity", C.c_int32), ("gravity", C.c_double * 3)]
+                ("has_gravity", C.c_int32), ("gravity", C.c_double * 3),
+                ("damping", C.c_void_p), ("body_func", C.c_int32)]
 
 
 class _State(C.Structure):
@@ -245,6 +252,9 @@ class StepOracle:
             m.has_gravity = 1
             for d in range(3):
                 m.gravity[d] = float(g[d])
+        # Rayleigh damping [nmat][2] = (alpha, beta) and the body-force function
+        m.damping = _ptr(arr(prob.get("damping"), np.float64))
+        m.body_func = int(prob.get("body_func", -1))
         self.m = m
         n, e = self.npoin, self.nelem
         self.u = np.zeros((n, 3)); self.v = np.zeros((n, 3)); self.a = np.zeros((n, 3))
@@ -293,6 +303,9 @@ class StepOracle:
 
     def funbou(self, f, c, t):
         return lib().orc_funbou(C.byref(self.m), f, c, t)
+
+    def funbou_body(self, f, c, t):
+        return lib().orc_funbou_body(C.byref(self.m), f, c, t)
 
     def face_loads(self, u, t):
         """Nodal forces [npoin, 3] of the boundary faces at time t with

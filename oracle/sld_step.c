@@ -155,6 +155,27 @@ double orc_funbou(const orc_model *m, int32_t f, int c, double t)
     return x;
 }
 
+/* sld_funbou.f90:142-167 (itask 3, body forces): the row pair with
+ * t1 <= t < t2, m = (p2 - p1)/(t2 - t1), b = p1 - m t1, value m t + b;
+ * zero when t lies outside the table (the caller's fonew = 0). */
+double orc_funbou_body(const orc_model *m, int32_t f, int c, double t)
+{
+    const double *tab = m->func_table + (size_t)f * m->func_maxpts * 4;
+    int np = m->func_npts[f];
+    for (int i = 0; i < np - 1; i++) {
+        int upper = orc_mutation == ORC_MUT_BODY_LE ? t <= tab[4 * (i + 1) + 3]
+                                                     : t < tab[4 * (i + 1) + 3];
+        if (t >= tab[4 * i + 3] && upper) {
+            double t1 = tab[4 * i + 3], t2 = tab[4 * (i + 1) + 3];
+            double p1 = tab[4 * i + c], p2 = tab[4 * (i + 1) + c];
+            double mm = (p2 - p1) / (t2 - t1);
+            double b = p1 - mm * t1;
+            return mm * t + b;
+        }
+    }
+    return 0.0;
+}
+
 /* Boundary faces (sld_bouope.f90:88-277): per face, the geometry bocod of
  * its 4 nodes (reference, or current with follower loads), the value of the
  * step (sld_updbcs.f90:153-181: constant, or under transient conditions the
@@ -223,11 +244,27 @@ void orc_face_loads(const orc_model *m, const double *u, double t, double *ff)
     }
 }
 
-/* Element loop: fint, rhs and the new history/stress for every element. */
-static void element_loop(const orc_model *m, orc_state *s)
+/* HEX08 shape function N_a at Gauss point g (mod_lagrange.f90 shape3) */
+static double shapf_at(int g, int a)
+{
+    static double shapf[64];
+    static int ready = 0;
+    if (!ready) {
+        double posgp[24], weigp[8], deriv[192];
+        orc_hex8_tables(posgp, weigp, shapf, deriv);
+        ready = 1;
+    }
+    return shapf[8 * g + a];
+}
+
+/* Element loop: fint, rhs and the new history/stress for every element.
+ * vel = veloc_sld(:,:,ITER_K) at the time of the element loop (the Rayleigh
+ * damping's velocity), NULL without damping. */
+static void element_loop(const orc_model *m, orc_state *s, const double *vel)
 {
     double *sdv_k = s->sdv, *sdv_n = s->sdv + 24 * m->nelem;
     int ncol = m->color_ptr ? m->ncolor : 1;
+    (void)shapf_at(0, 0);   /* fill the table before the parallel region */
     memset(s->fint, 0, sizeof(double) * 3 * m->npoin);
     memset(s->rhs, 0, sizeof(double) * 3 * m->npoin);
     for (int c = 0; c < ncol; c++) {
@@ -238,7 +275,7 @@ static void element_loop(const orc_model *m, orc_state *s)
         for (ee = lo; ee < hi; ee++) {
             int64_t e = m->elem_order ? m->elem_order[ee] : ee;
             double elcod[24], eldis[24], gpcar[192], gpvol[8], F[72], detF[8];
-            double elfint[24] = {0};
+            double elfint[24] = {0}, elrayl[24] = {0}, Sg[72];
             double props[18];
             const double *pr;
             double E = elem_E(m, e, &pr);
@@ -250,6 +287,31 @@ static void element_loop(const orc_model *m, orc_state *s)
             gather(m, m->coord, e, elcod);
             gather(m, s->u, e, eldis);
             orc_hex8_kinematics(elcod, eldis, gpcar, gpvol, F, detF);
+            /* Rayleigh damping (sld_elmmat.f90:160-163, :174-194) with the
+             * velocity interpolated at the Gauss points, gpvel, and its
+             * material gradient, gprat (sld_elmpre.f90:110-114, :150-157) */
+            const double alpha = m->damping ? m->damping[2 * mat] : 0.0;
+            const double beta = m->damping ? m->damping[2 * mat + 1] : 0.0;
+            double gpvel[24], gprat[72];
+            if (vel && (alpha > 0.0 || beta > 0.0)) {
+                double elvel[24], posgp[24], weigp[8], shapf[64], deriv[192];
+                orc_hex8_tables(posgp, weigp, shapf, deriv);
+                gather(m, vel, e, elvel);
+                for (int i = 0; i < 3; i++)
+                    for (int g = 0; g < 8; g++) {
+                        gpvel[3 * g + i] = 0.0;
+                        for (int a = 0; a < 8; a++)
+                            gpvel[3 * g + i] = gpvel[3 * g + i] + elvel[3 * a + i] * shapf[8 * g + a];
+                    }
+                for (int g = 0; g < 8; g++)
+                    for (int i = 0; i < 3; i++)
+                        for (int j = 0; j < 3; j++) {
+                            double r = 0.0;
+                            for (int a = 0; a < 8; a++)
+                                r = r + elvel[3 * a + i] * gpcar[24 * g + 3 * a + j];
+                            gprat[9 * g + i + 3 * j] = r;
+                        }
+            }
             for (int g = 0; g < 8; g++) {
                 double S[9], P[9];
                 const double *Fg = F + 9 * g;
@@ -277,6 +339,7 @@ static void element_loop(const orc_model *m, orc_state *s)
                 }
                 if (s->flags) s->flags[8 * e + g] = fl;
                 if (s->S) memcpy(s->S + 72 * e + 9 * g, S, sizeof(S));
+                memcpy(Sg + 9 * g, S, sizeof(S));
                 /* P_jk = sum_i F_ji S_ik  (sld_builtin_materials.f90:359-369) */
                 for (int q = 0; q < 9; q++) P[q] = 0.0;
                 for (int k = 0; k < 3; k++)
@@ -286,16 +349,44 @@ static void element_loop(const orc_model *m, orc_state *s)
                 /* sld_elmmat.f90:137-150 */
                 double volux = gpvol[g];
                 for (int a = 0; a < 8; a++)
-                    for (int i = 0; i < 3; i++)
+                    for (int i = 0; i < 3; i++) {
                         for (int j = 0; j < 3; j++)
                             elfint[3 * a + i] = elfint[3 * a + i]
                                 + (P[i + 3 * j] + 0.0) * gpcar[24 * g + 3 * a + j] * volux;
+                        /* sld_elmmat.f90:160-163 */
+                        if (vel && alpha > 0.0) {
+                            double vg = gpvel[3 * g + i];
+                            if (orc_mutation == ORC_MUT_DAMP_LUMPED)
+                                vg = vel[3 * m->lnods[8 * e + a] + i];
+                            elrayl[3 * a + i] = elrayl[3 * a + i]
+                                + alpha * vg * shapf_at(g, a) * m->rho[mat] * volux;
+                        }
+                    }
             }
+            /* elrhs = elfext - elfint - elrayl_alpha (sld_elmmat.f90:170;
+             * the element loads of gravity and the body force are the
+             * lumped mass times the acceleration, added per node) */
+            double elrhs[24];
+            for (int q = 0; q < 24; q++) elrhs[q] = 0.0 - elfint[q] - elrayl[q];
+            /* the beta term as written (sld_elmmat.f90:174-194): the same
+             * sum over (j, k) is added to every component of node a */
+            if (vel && beta > 0.0)
+                for (int g = 0; g < 8; g++) {
+                    double volux = gpvol[g];
+                    for (int a = 0; a < 8; a++)
+                        for (int i = 0; i < 3; i++)
+                            for (int j = 0; j < 3; j++)
+                                for (int k = 0; k < 3; k++)
+                                    elrhs[3 * a + i] = elrhs[3 * a + i]
+                                        + beta * gpcar[24 * g + 3 * a + j]
+                                          * gprat[orc_mutation == ORC_MUT_BETA_T ? 9 * g + j + 3 * k
+                                                                                 : 9 * g + k + 3 * j]
+                                          * Sg[9 * g + j + 3 * k] * volux;
+                }
             for (int a = 0; a < 8; a++) {
                 int64_t p = m->lnods[8 * e + a];
                 for (int i = 0; i < 3; i++) {
-                    double r = 0.0 - elfint[3 * a + i] - 0.0;
-                    s->rhs[3 * p + i] += r;
+                    s->rhs[3 * p + i] += elrhs[3 * a + i];
                     s->fint[3 * p + i] += elfint[3 * a + i];
                 }
             }
@@ -312,7 +403,11 @@ int orc_sld_step(const orc_model *m, orc_state *s, double dt, int scheme, double
     double t_end = cutim + dt, t_mid = 0.5 * (cutim + t_end);
     double fx = 0.0;
     if (m->fext0) fx = orc_funbou(m, m->fext_func, 0, cutim);
-    double *u_old = NULL;
+    double *u_old = NULL, *v_damp = NULL;
+    if (m->damping) {
+        v_damp = (double *)malloc(sizeof(double) * 3 * np);
+        memcpy(v_damp, s->v, sizeof(double) * 3 * np);
+    }
     if (s->fint_prev) {
         u_old = (double *)malloc(sizeof(double) * 3 * np);
         memcpy(u_old, s->u, sizeof(double) * 3 * np);
@@ -354,9 +449,16 @@ int orc_sld_step(const orc_model *m, orc_state *s, double dt, int scheme, double
             }
     if (s->halt) {
         free(u_old);
+        free(v_damp);
         return s->halt;
     }
-    element_loop(m, s);
+    /* veloc_sld(ITER_K) at the element loop: v^n in the central difference
+     * (its predictor keeps v^{n+1/2} in veloctmp_sld), the predicted
+     * v^{n+1} in Tchamwa-Wielgosz */
+    element_loop(m, s, m->damping ? ((scheme == 0 && orc_mutation != ORC_MUT_DAMP_VHALF)
+                                     ? v_damp : s->v) : NULL);
+    free(v_damp);
+    v_damp = NULL;
     /* det F <= 0 stops the run after the element loop (sld_elmope.f90:106) */
     if (s->halt) {
         free(u_old);
@@ -369,9 +471,18 @@ int orc_sld_step(const orc_model *m, orc_state *s, double dt, int scheme, double
     if (m->fext0)
         for (int64_t p = 0; p < np; p++)
             for (int d = 0; d < 3; d++) fe[3 * p + d] = fe[3 * p + d] + fx * m->fext0[3 * p + d];
-    if (m->has_gravity)     /* on every dof, prescribed ones too (their reactions) */
+    /* gravity and the body force: (grnor gravi + bodyf) times the lumped
+     * mass, on every dof, prescribed ones too (their reactions) */
+    if (m->has_gravity || m->body_func >= 0) {
+        double gb[3];
+        for (int d = 0; d < 3; d++) {
+            double g = m->has_gravity ? m->gravity[d] : 0.0;
+            double b = m->body_func >= 0 ? orc_funbou_body(m, m->body_func, d, cutim) : 0.0;
+            gb[d] = g + b;
+        }
         for (int64_t p = 0; p < np; p++)
-            for (int d = 0; d < 3; d++) fe[3 * p + d] = fe[3 * p + d] + s->mass[p] * m->gravity[d];
+            for (int d = 0; d < 3; d++) fe[3 * p + d] = fe[3 * p + d] + s->mass[p] * gb[d];
+    }
     for (int64_t p = 0; p < 3 * np; p++) s->rhs[p] = s->rhs[p] + fe[p];
     /* a non-finite residual stops the run after the step (sld_cvgunk.f90:195-199) */
     int64_t bad_node = -1;
