@@ -83,6 +83,8 @@ def lib():
     L.sld_dev_set_assembly.argtypes = [p, i32]
     L.sld_dev_set_gravity.argtypes = [p, C.POINTER(C.c_double)]
     L.sld_dev_set_energy.argtypes = [p, i32]
+    L.sld_dev_set_damping.argtypes = [p, p]
+    L.sld_dev_set_body_force.argtypes = [p, i32]
     L.sld_dev_set_faces.argtypes = [p, p]
     L.sld_dev_energy.argtypes = [p, p]
     L.sld_dev_autotune.argtypes = [p, d, i32, d, i32, p, p]

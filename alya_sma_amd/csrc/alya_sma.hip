@@ -54,11 +54,12 @@ constexpr int GRAPH_RUNS[] = {64, 16, 2};
 
 struct GraphKey {
   double dt, phi;
-  int scheme, mode, store, epb, xcd, persistent, assembly, tn, energy, len;
+  int scheme, mode, store, epb, xcd, persistent, assembly, tn, energy, len, damp, body;
   bool operator<(const GraphKey& o) const {
-    return std::tie(dt, phi, scheme, mode, store, epb, xcd, persistent, assembly, tn, energy, len) <
+    return std::tie(dt, phi, scheme, mode, store, epb, xcd, persistent, assembly, tn, energy, len,
+                    damp, body) <
            std::tie(o.dt, o.phi, o.scheme, o.mode, o.store, o.epb, o.xcd, o.persistent,
-                    o.assembly, o.tn, o.energy, o.len);
+                    o.assembly, o.tn, o.energy, o.len, o.damp, o.body);
   }
 };
 
@@ -138,6 +139,15 @@ struct sld_dev {
   int mode = SLD_MODE_FP64, store = 0, epb = 8, xcd = 1, persistent = 0, use_graph = 1;
   int gravity = 0;                   // sld_dev_set_gravity
   double grav[3] = {0.0, 0.0, 0.0};
+  int32_t body_func = -1;            // sld_dev_set_body_force
+  std::vector<double> func_table_host;   // host copies of the load tables
+  std::vector<int32_t> func_npts_host;
+  // Rayleigh damping (sld_dev_set_damping), allocated on first use
+  int damp = 0;
+  double* damping = nullptr;         // [nmat][2] (alpha, beta)
+  double* vdamp[3] = {};             // v^n kept by the central-difference predictor
+  void* edamp = nullptr;             // corner damping forces, laid out as efor
+  double* fdamp[3] = {};             // nodal damping force of the last step
   int assembly = SLD_ASSEMBLY_GATHER;
   double dtcrit = 0.0;
   std::map<GraphKey, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
@@ -166,7 +176,7 @@ int track(sld_dev* d, T* p) {
   return 0;
 }
 
-sld::NodalArgs nodal_args(sld_dev* d, int tn) {
+sld::NodalArgs nodal_args(sld_dev* d, int tn, int scheme = 0) {
   sld::NodalArgs A{};
   A.npoin = d->npoin;
   const bool f32 = d->mode != SLD_MODE_FP64;
@@ -212,6 +222,15 @@ sld::NodalArgs nodal_args(sld_dev* d, int tn) {
     A.mass = d->mass;
     for (int k = 0; k < 3; ++k) A.grav[k] = d->grav[k];
   }
+  A.body_func = d->body_func;
+  if (d->body_func >= 0) A.mass = d->mass;
+  if (d->damp) {
+    for (int k = 0; k < 3; ++k) {
+      A.vdamp[k] = scheme == 0 ? d->vdamp[k] : nullptr;
+      A.fdamp[k] = d->fdamp[k];
+    }
+    A.edamp = d->edamp;
+  }
   A.health = d->health;
   return A;
 }
@@ -253,7 +272,7 @@ int read_health(sld_dev* d, sld::Health* out = nullptr) {
 
 int g_num_cu = 0;
 
-template <typename R, int EPB, bool STORE, int SUR, bool GATHER, int LAWS = 0>
+template <typename R, int EPB, bool STORE, int SUR, bool GATHER, int LAWS = 0, bool DAMP = false>
 void launch_element(sld::ElemArgs A, hipStream_t s, int persistent) {
   const int ne = A.e_end - A.e_begin;
   if (ne <= 0) return;
@@ -268,34 +287,52 @@ void launch_element(sld::ElemArgs A, hipStream_t s, int persistent) {
       grid = (ntiles + A.tiles_per_block - 1) / A.tiles_per_block;
     }
   }
-  hipLaunchKernelGGL((sld::sld_element<R, EPB, STORE, SUR, GATHER, LAWS>), dim3(grid), dim3(EPB * 8),
-                     0, s, A);
+  hipLaunchKernelGGL((sld::sld_element<R, EPB, STORE, SUR, GATHER, LAWS, DAMP>), dim3(grid),
+                     dim3(EPB * 8), 0, s, A);
 }
 
-template <typename R, bool STORE, int SUR, bool GATHER, int LAWS = 0>
+// Damping kernels come in one tile shape (8 elements per workgroup).
+template <typename R, bool STORE, int SUR, bool GATHER, int LAWS = 0, bool DAMP = false>
 void launch_element_epb(int epb, const sld::ElemArgs& A, hipStream_t s, int persistent) {
-  if (epb == 8) launch_element<R, 8, STORE, SUR, GATHER, LAWS>(A, s, persistent);
-  else if (epb == 16) launch_element<R, 16, STORE, SUR, GATHER, LAWS>(A, s, persistent);
-  else launch_element<R, 32, STORE, SUR, GATHER>(A, s, persistent);
+  if constexpr (DAMP) {
+    launch_element<R, 8, STORE, SUR, GATHER, LAWS, true>(A, s, persistent);
+  } else {
+    if (epb == 8) launch_element<R, 8, STORE, SUR, GATHER, LAWS>(A, s, persistent);
+    else if (epb == 16) launch_element<R, 16, STORE, SUR, GATHER, LAWS>(A, s, persistent);
+    else launch_element<R, 32, STORE, SUR, GATHER>(A, s, persistent);
+  }
+}
+
+template <typename R, int SUR, bool DAMP>
+void launch_element_damp(const sld_dev* d, const sld::ElemArgs& A, hipStream_t s) {
+  const bool st = d->store != 0;
+  if constexpr (SUR == 0) {
+    if (d->nh_begin >= 0 && A.e_end > d->nh_begin) {
+      // law-grouped mesh: the SMA range with the per-element kernel, the
+      // neo-Hookean range with the kernel that has only that law
+      sld::ElemArgs B = A;
+      B.e_end = std::min(A.e_end, d->nh_begin);
+      if (st) launch_element_epb<R, true, 0, true, 0, DAMP>(d->epb, B, s, d->persistent);
+      else launch_element_epb<R, false, 0, true, 0, DAMP>(d->epb, B, s, d->persistent);
+      B.e_begin = std::max(A.e_begin, d->nh_begin);
+      B.e_end = A.e_end;
+      if (st) launch_element_epb<R, true, 0, true, 2, DAMP>(d->epb, B, s, d->persistent);
+      else launch_element_epb<R, false, 0, true, 2, DAMP>(d->epb, B, s, d->persistent);
+      return;
+    }
+  }
+  if (st) launch_element_epb<R, true, SUR, true, 0, DAMP>(d->epb, A, s, d->persistent);
+  else launch_element_epb<R, false, SUR, true, 0, DAMP>(d->epb, A, s, d->persistent);
 }
 
 template <typename R, int SUR>
 void launch_element_mode(const sld_dev* d, const sld::ElemArgs& A, hipStream_t s, bool gather) {
   const bool st = d->store != 0;
-  if constexpr (SUR == 0) {
-    if (gather && d->nh_begin >= 0 && A.e_end > d->nh_begin) {
-      // law-grouped mesh: the SMA range with the per-element kernel, the
-      // neo-Hookean range with the kernel that has only that law
-      sld::ElemArgs B = A;
-      B.e_end = std::min(A.e_end, d->nh_begin);
-      if (st) launch_element_epb<R, true, 0, true>(d->epb, B, s, d->persistent);
-      else launch_element_epb<R, false, 0, true>(d->epb, B, s, d->persistent);
-      B.e_begin = std::max(A.e_begin, d->nh_begin);
-      B.e_end = A.e_end;
-      if (st) launch_element_epb<R, true, 0, true, 2>(d->epb, B, s, d->persistent);
-      else launch_element_epb<R, false, 0, true, 2>(d->epb, B, s, d->persistent);
-      return;
-    }
+  if (gather) {
+    // damping runs with the gather assembly only (its corner forces)
+    if (d->damp) launch_element_damp<R, SUR, true>(d, A, s);
+    else launch_element_damp<R, SUR, false>(d, A, s);
+    return;
   }
   if (gather) {
     if (st) launch_element_epb<R, true, SUR, true>(d->epb, A, s, d->persistent);
@@ -361,7 +398,7 @@ struct KernelTimer {
 int enqueue_step(sld_dev* d, double dt, int scheme, double phi, int tn, KernelTimer* T = nullptr,
                  bool predict = true, bool fuse = false) {
   hipStream_t s = d->stream;
-  sld::NodalArgs N = nodal_args(d, tn);
+  sld::NodalArgs N = nodal_args(d, tn, scheme);
   const bool gather = d->assembly == SLD_ASSEMBLY_GATHER;
   const unsigned nb = nblocks(d->npoin, 256);
   TBEGIN(0);
@@ -420,6 +457,14 @@ int enqueue_step(sld_dev* d, double dt, int scheme, double phi, int tn, KernelTi
     E.flags_out = d->flags_out;
     E.counters = d->counters;
     E.health = d->health;
+    if (d->damp) {
+      // veloc_sld(ITER_K) at the element loop: v^n (central difference,
+      // kept by the predictor) or the predicted v^{n+1} (Tchamwa-Wielgosz)
+      for (int k = 0; k < 3; ++k) E.vel[k] = scheme == 0 ? d->vdamp[k] : d->v[k];
+      E.damping = d->damping;
+      E.rho = d->rho;
+      E.edamp = d->edamp;
+    }
     E.xcd_remap = d->xcd;
     const bool sur = d->mode == SLD_MODE_SURROGATE;
     if (sur) {
@@ -504,6 +549,21 @@ void free_graphs(sld_dev* d) {
     hipGraphDestroy(kv.second.first);
   }
   d->graphs.clear();
+}
+
+// Body-force table value (sld_funbou.f90:142-167, itask 3), as the corrector
+// evaluates it on the device.
+double host_funbou_body(const double* tab, int npts, int c, double t) {
+  for (int i = 0; i < npts - 1; ++i) {
+    const double t1 = tab[4 * i + 3], t2 = tab[4 * (i + 1) + 3];
+    if (t >= t1 && t < t2) {
+      const double p1 = tab[4 * i + c], p2 = tab[4 * (i + 1) + c];
+      const double m = (p2 - p1) / (t2 - t1);
+      const double b = p1 - m * t1;
+      return m * t + b;
+    }
+  }
+  return 0.0;
 }
 
 double host_critical_dt(const double* coord, const int32_t* lnods, int64_t nelem,
@@ -760,6 +820,8 @@ int sld_dev_create(const sld_mesh_desc* mesh, const sld_material_desc* mat,
   d->nfunc = bc->nfunc;
   d->func_maxpts = bc->func_maxpts;
   if (bc->nfunc > 0) {
+    d->func_table_host.assign(bc->func_table, bc->func_table + (size_t)bc->nfunc * bc->func_maxpts * 4);
+    d->func_npts_host.assign(bc->func_npts, bc->func_npts + bc->nfunc);
     TRY(upload(&d->func_table, bc->func_table, (size_t)bc->nfunc * bc->func_maxpts * 4, s)); track(d, d->func_table);
     TRY(upload(&d->func_npts, bc->func_npts, (size_t)bc->nfunc, s)); track(d, d->func_npts);
     std::vector<int32_t> sm((size_t)bc->nfunc, 0);
@@ -890,6 +952,7 @@ int sld_dev_set_mode(sld_dev* d, int32_t mode) {
     // so that the padding slots read as zero in the new layout
     if ((mode == SLD_MODE_FP64) != (d->mode == SLD_MODE_FP64)) {
       HIPCHK(hipMemsetAsync(d->efor, 0, d->efor_bytes, d->stream));
+      if (d->edamp) HIPCHK(hipMemsetAsync(d->edamp, 0, d->efor_bytes, d->stream));
       HIPCHK(hipStreamSynchronize(d->stream));
     }
     d->mode = mode;
@@ -931,6 +994,8 @@ int sld_dev_set_assembly(sld_dev* d, int32_t assembly) {
     return fail(SMA_ERR_ARG, "assembly must be SLD_ASSEMBLY_SCATTER or SLD_ASSEMBLY_GATHER");
   if (assembly == SLD_ASSEMBLY_SCATTER && d->repeated_nodes)
     return fail(SMA_ERR_STATE, "an element lists a node twice: only the gather assembly is exact");
+  if (assembly == SLD_ASSEMBLY_SCATTER && d->damp)
+    return fail(SMA_ERR_STATE, "Rayleigh damping runs with the gather assembly only");
   if (assembly != d->assembly) {
     // f_int is complete after every step in both assemblies: nothing to carry
     HIPCHK(hipStreamSynchronize(d->stream));
@@ -947,6 +1012,52 @@ int sld_dev_set_gravity(sld_dev* d, const double* g) {
   free_graphs(d);   // captured steps hold the previous value
   d->gravity = g ? 1 : 0;
   for (int k = 0; k < 3; ++k) d->grav[k] = g ? g[k] : 0.0;
+  return SMA_OK;
+}
+
+int sld_dev_set_body_force(sld_dev* d, int32_t func) {
+  if (!d) return fail(SMA_ERR_ARG, "null handle");
+  if (func < -1 || func >= d->nfunc)
+    return fail(SMA_ERR_ARG, "sld_dev_set_body_force: function index out of range");
+  HIPCHK(hipStreamSynchronize(d->stream));
+  free_graphs(d);
+  d->body_func = func;
+  return SMA_OK;
+}
+
+int sld_dev_set_damping(sld_dev* d, const double* alpha_beta) {
+  if (!d) return fail(SMA_ERR_ARG, "null handle");
+  bool any = false;
+  if (alpha_beta)
+    for (int k = 0; k < 2 * d->nmat; ++k) {
+      if (!std::isfinite(alpha_beta[k]) || alpha_beta[k] < 0.0)
+        return fail(SMA_ERR_ARG, "sld_dev_set_damping: alpha and beta must be finite and >= 0");
+      any |= alpha_beta[k] > 0.0;
+    }
+  HIPCHK(hipStreamSynchronize(d->stream));
+  free_graphs(d);
+  if (!any) {
+    d->damp = 0;
+    return SMA_OK;
+  }
+  const int64_t np = d->npoin;
+  if (!d->damping) {
+    int rc = 0;
+    if ((rc = dalloc(&d->damping, (size_t)2 * d->nmat))) return rc;
+    track(d, d->damping);
+    for (int k = 0; k < 3; ++k) {
+      if ((rc = dalloc(&d->vdamp[k], np)) || (rc = dalloc(&d->fdamp[k], np))) return rc;
+      track(d, d->vdamp[k]);
+      track(d, d->fdamp[k]);
+      HIPCHK(hipMemset(d->fdamp[k], 0, np * sizeof(double)));
+    }
+    if ((rc = dalloc((double**)&d->edamp, d->efor_bytes / sizeof(double)))) return rc;
+    track(d, d->edamp);
+    HIPCHK(hipMemset(d->edamp, 0, d->efor_bytes));
+  }
+  HIPCHK(hipMemcpy(d->damping, alpha_beta, 2 * d->nmat * sizeof(double), hipMemcpyHostToDevice));
+  d->damp = 1;
+  d->assembly = SLD_ASSEMBLY_GATHER;
   return SMA_OK;
 }
 
@@ -1068,7 +1179,7 @@ int sld_dev_step(sld_dev* d, double dt, int32_t scheme, double phi, int64_t nste
   for (int len : GRAPH_RUNS) {
     if (!d->use_graph || left < len) continue;
     GraphKey key{dt, phi, scheme, d->mode, d->store, d->epb, d->xcd, d->persistent, d->assembly, d->cur,
-                 d->energy, len};
+                 d->energy, len, d->damp, d->body_func};
     auto it = d->graphs.find(key);
     if (it == d->graphs.end()) {
       hipGraph_t g = nullptr;
@@ -1183,6 +1294,13 @@ int sld_dev_autotune(sld_dev* d, double dt, int32_t scheme, double phi, int32_t 
     }
   }
   live.push_back({d->efor, d->efor_bytes});
+  if (d->edamp) {
+    live.push_back({d->edamp, d->efor_bytes});
+    for (int k = 0; k < 3; ++k) {
+      live.push_back({d->vdamp[k], np * 8});
+      live.push_back({d->fdamp[k], np * 8});
+    }
+  }
   if (d->S_out) live.push_back({d->S_out, 9 * ngp * 8});
   if (d->P_out) live.push_back({d->P_out, 9 * ngp * 8});
   if (d->flags_out) live.push_back({d->flags_out, ngp * 4});
@@ -1213,7 +1331,7 @@ int sld_dev_autotune(sld_dev* d, double dt, int32_t scheme, double phi, int32_t 
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
   int rc = 0;
-  const int asm_first = d->repeated_nodes ? SLD_ASSEMBLY_GATHER : SLD_ASSEMBLY_SCATTER;
+  const int asm_first = (d->repeated_nodes || d->damp) ? SLD_ASSEMBLY_GATHER : SLD_ASSEMBLY_SCATTER;
   for (int as = asm_first; as <= SLD_ASSEMBLY_GATHER && !rc; ++as)
   for (int ie = 0; ie < 3 && !rc; ++ie)
     for (int x = 0; x < 2 && !rc; ++x)
@@ -1338,15 +1456,35 @@ int sld_dev_get(sld_dev* d, int32_t field, void* host, int64_t count) {
       if (rc) return rc;
       std::vector<uint8_t> fix(np);
       HIPCHK(hipMemcpy(fix.data(), d->fixmask, np, hipMemcpyDeviceToHost));
-      // frxid = elrhs = elfext - elfint on prescribed dofs: the element loads
-      // are gravity's m g (the Neumann and nodal loads are assembled apart)
-      std::vector<double> m(d->gravity ? np : 0);
-      if (d->gravity) HIPCHK(hipMemcpy(m.data(), d->mass, np * sizeof(double), hipMemcpyDeviceToHost));
+      // frxid = elrhs = elfext - elfint - the damping force on prescribed
+      // dofs: the element loads are m (g + b(t)) of gravity and the body
+      // force (the Neumann and nodal loads are assembled apart)
+      const bool load = d->gravity || d->body_func >= 0;
+      std::vector<double> m(load ? np : 0), fd(d->damp ? 3 * np : 0);
+      if (load) HIPCHK(hipMemcpy(m.data(), d->mass, np * sizeof(double), hipMemcpyDeviceToHost));
+      if (d->damp)
+        for (int k = 0; k < 3; ++k) {
+          std::vector<double> t(np);
+          HIPCHK(hipMemcpy(t.data(), d->fdamp[k], np * sizeof(double), hipMemcpyDeviceToHost));
+          for (int64_t p = 0; p < np; ++p) fd[3 * p + k] = t[p];
+        }
+      double gb[3] = {0.0, 0.0, 0.0};
+      if (load) {
+        double t = 0.0;
+        HIPCHK(hipMemcpy(&t, d->clock + d->cur, sizeof(double), hipMemcpyDeviceToHost));
+        for (int k = 0; k < 3; ++k) {
+          gb[k] = d->gravity ? d->grav[k] : 0.0;
+          if (d->body_func >= 0)
+            gb[k] = gb[k] + host_funbou_body(&d->func_table_host[(size_t)d->body_func * d->func_maxpts * 4],
+                                             d->func_npts_host[d->body_func], k, t);
+        }
+      }
       double* out = (double*)host;
       for (int64_t p = 0; p < np; ++p)
         for (int k = 0; k < 3; ++k) {
           double r = 0.0 - f[3 * p + k];
-          if (d->gravity) r = r + m[p] * d->grav[k];
+          if (d->damp) r = r - fd[3 * p + k];
+          if (load) r = r + m[p] * gb[k];
           out[3 * p + k] = ((fix[p] >> k) & 1) ? r : 0.0;
         }
       return SMA_OK;

@@ -217,8 +217,36 @@ struct NodalArgs {
   // gravity (sld_dev_set_gravity): f_ext += m g at every dof; mass is set
   int32_t gravity;
   double grav[3];
+  // body force (sld_dev_set_body_force): f_ext += m b(t^{n+1}), b from load
+  // function body_func by the rule of sld_funbou.f90 itask 3; mass is set
+  int32_t body_func;           // -1 = none
+  // Rayleigh damping (sld_dev_set_damping): the central-difference
+  // predictor keeps v^n in vdamp (the element's veloc_sld(ITER_K)); the
+  // corrector sums each node's corner damping forces (edamp, laid out as
+  // efor) into the residual and stores them in fdamp at a run's last step
+  double* vdamp[3];            // null: no damping or Tchamwa-Wielgosz
+  const void* edamp;           // null: no damping
+  double* fdamp[3];
   Health* health;              // run health (never null)
 };
+
+// Body-force value of component c at time t (sld_funbou.f90:142-167, itask
+// 3): linear between the rows with t1 <= t < t2, value m t + b with
+// m = (p2 - p1)/(t2 - t1), b = p1 - m t1; zero outside the table.
+__device__ __forceinline__ double funbou_body(const double* __restrict__ tab, int npts, int c,
+                                              double t) {
+#pragma clang fp contract(off)
+  for (int i = 0; i < npts - 1; ++i) {
+    const double t1 = tab[4 * i + 3], t2 = tab[4 * (i + 1) + 3];
+    if (t >= t1 && t < t2) {
+      const double p1 = tab[4 * i + c], p2 = tab[4 * (i + 1) + c];
+      const double m = (p2 - p1) / (t2 - t1);
+      const double b = p1 - m * t1;
+      return m * t + b;
+    }
+  }
+  return 0.0;
+}
 
 // Boundary faces with Neumann loads (sld_bouope.f90), one QUA04 face per
 // thread; corner forces go to node-sorted slots that the corrector sums.
@@ -287,9 +315,11 @@ __device__ __forceinline__ void predict_node(const NodalArgs& A, int64_t i, int 
 // Stores the predicted state (and du for the energy balance); the coloured
 // scatter also clears f_int for the element launches that add into it.
 __device__ __forceinline__ void store_prediction(const NodalArgs& A, int64_t i, const double u0[3],
-                                                 const double un[3], const double vn[3]) {
+                                                 const double un[3], const double vn[3],
+                                                 const double vprev[3]) {
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
+    if (A.vdamp[0]) A.vdamp[d][i] = vprev[d];
     if (A.du[0]) A.du[d][i] = un[d] - u0[d];
     A.u[d][i] = un[d];
     A.v[d][i] = vn[d];
@@ -313,7 +343,7 @@ __global__ void __launch_bounds__(256) sld_predict(NodalArgs A, double dt, doubl
     a[d] = A.a[d][i];
   }
   predict_vals(A, i, SCHEME, dt, phi, A.clock[A.tn] + dt, A.fixmask[i], u, v, a, un, vn);
-  store_prediction(A, i, u, un, vn);
+  store_prediction(A, i, u, un, vn, v);
   if (!(isfinite(un[0]) && isfinite(un[1]) && isfinite(un[2]))) report_nonfinite(A.health, i);
 }
 
@@ -327,11 +357,11 @@ __global__ void __launch_bounds__(256) sld_predict(NodalArgs A, double dt, doubl
 // Without padding, a node with 8 corners still loads them in one round trip.
 template <typename R>
 __device__ __forceinline__ void gather_fint(const NodalArgs& A, int64_t i, double fi[3],
-                                            bool store) {
+                                            bool store, const void* slots = nullptr) {
   const int W = A.slot_width;
   const int32_t k0 = W ? W * (int32_t)i : A.node_ptr[i];
   const int32_t k1 = W ? k0 + W : A.node_ptr[i + 1];
-  const R* ef = (const R*)A.efor + 3 * (int64_t)k0;
+  const R* ef = (const R*)(slots ? slots : A.efor) + 3 * (int64_t)k0;
   R s0 = R(0), s1 = R(0), s2 = R(0);
   if (k1 - k0 == 8) {
     R c[24];
@@ -350,7 +380,7 @@ __device__ __forceinline__ void gather_fint(const NodalArgs& A, int64_t i, doubl
       s2 += ef[3 * k + 2];
     }
   }
-  if (store) {
+  if (store && !slots) {
     ((R*)A.fint[0])[i] = s0;
     ((R*)A.fint[1])[i] = s1;
     ((R*)A.fint[2])[i] = s2;
@@ -429,12 +459,28 @@ __device__ __forceinline__ void correct_node(const NodalArgs& A, int64_t i, doub
 #pragma unroll
     for (int d = 0; d < 3; ++d) fe[d] = fe[d] + fx * A.fext0[d][i];
   }
-  if (A.gravity) {
-    // the element load sum_g N_a rho g w detJ (sld_elmmat.f90:152-157) of
-    // every element at the node: the lumped mass times g
+  if (A.gravity || A.body_func >= 0) {
+    // the element load sum_g N_a rho (g + b) w detJ (sld_elmmat.f90:152-157)
+    // of every element at the node: the lumped mass times g + b(t^{n+1})
     const double m = A.mass[i];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) fe[d] = fe[d] + m * A.grav[d];
+    for (int d = 0; d < 3; ++d) {
+      double gb = A.gravity ? A.grav[d] : 0.0;
+      if (A.body_func >= 0)
+        gb = gb + funbou_body(A.func_table + (size_t)A.body_func * A.func_maxpts * 4,
+                              A.func_npts[A.body_func], d, cutim);
+      fe[d] = fe[d] + m * gb;
+    }
+  }
+  // Rayleigh damping forces of the node's element corners
+  double fd[3] = {0.0, 0.0, 0.0};
+  if (A.edamp) {
+    if (A.fint_fp32) gather_fint<float>(A, i, fd, false, A.edamp);
+    else gather_fint<double>(A, i, fd, false, A.edamp);
+    if (!FUSE) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) A.fdamp[d][i] = fd[d];
+    }
   }
   double an[3], vn[3];
   bool finite = true;
@@ -442,6 +488,7 @@ __device__ __forceinline__ void correct_node(const NodalArgs& A, int64_t i, doub
   for (int d = 0; d < 3; ++d) {
     double fi = fg[d];
     double r = 0.0 - fi;
+    r = r - fd[d];
     r = r + fe[d];
     finite = finite && isfinite(r);
     double acc = 0.0 + r * rm;
@@ -463,7 +510,7 @@ __device__ __forceinline__ void correct_node(const NodalArgs& A, int64_t i, doub
   if (FUSE) {
     double un[3], vp[3];
     predict_vals(A, i, SCHEME, dt, phi, cutim + dt, fix, u0, vn, an, un, vp);
-    store_prediction(A, i, u0, un, vp);
+    store_prediction(A, i, u0, un, vp, vn);
     if (!(isfinite(un[0]) && isfinite(un[1]) && isfinite(un[2]))) report_nonfinite(A.health, i);
   }
   if (i == 0) {
@@ -650,6 +697,13 @@ struct ElemArgs {
   uint32_t* flags_out;          // optional [ngp]
   unsigned long long* counters; // [COUNTER_SLOTS][8] per-flag Gauss-point counts
   Health* health;               // run health (never null)
+  // Rayleigh damping (DAMP kernels): veloc_sld(ITER_K) of the step (v^n in
+  // the central difference, the predicted v^{n+1} in Tchamwa-Wielgosz),
+  // (alpha, beta) and density per material, corner damping forces
+  const double* vel[3];
+  const double* damping;        // [nmat][2]
+  const double* rho;            // [nmat]
+  void* edamp;                  // [slot][3] (double or float)
   int32_t xcd_remap;            // 1 = spread consecutive tiles per XCD
   int32_t tiles_per_block;      // > 0: persistent launch, tiles per workgroup
   // surrogate (mode 2): per-GP status (0 exact, 1 accepted, 2 fallback) and
@@ -700,10 +754,11 @@ __device__ __forceinline__ void count_flags(unsigned long long* ctr, uint32_t fl
 // LAWS = 0: each element takes its own law (law_of_elem); LAWS = 2: every
 // element of the launch is neo-Hookean (a range of a law-grouped mesh),
 // without the SMA path and its registers.
-template <typename R, int LAWS = 0>
+template <typename R, int LAWS = 0, bool DAMP = false>
 struct ElemOccupancy {
-  static constexpr int waves = LAWS == 2 ? (sizeof(R) == 8 ? 3 : 6)
-                                         : (sizeof(R) == 8 ? SLD_WAVES_F64 : SLD_WAVES_F32);
+  static constexpr int waves = DAMP ? (sizeof(R) == 8 ? 1 : 2)
+                               : LAWS == 2 ? (sizeof(R) == 8 ? 3 : 6)
+                                           : (sizeof(R) == 8 ? SLD_WAVES_F64 : SLD_WAVES_F32);
 };
 
 // The value of x, opaque to the optimiser: quantities derived from the lane
@@ -715,14 +770,25 @@ __device__ __forceinline__ int opaque(int x) {
   return x;
 }
 
-template <typename R, int EPB, bool STORE, int SUR = 0, bool GATHER = false, int LAWS = 0>
-__global__ void __launch_bounds__(EPB * 8, (ElemOccupancy<R, LAWS>::waves)) sld_element(ElemArgs A) {
-  __shared__ R s_rel[EPB][6][8];          // x_a - x_0 (0..2), u_a - u_0 (3..5)
-  // V(h) of the 6 components; rows padded to 8 for aligned reads, except in
+// DAMP: Rayleigh damping (sld_elmmat.f90:160-163, :174-194) -- the
+// element also gathers the velocity, forms at each Gauss point the
+// consistent-mass term q = alpha rho detJ v(g) and the beta term's
+// z_k = detJ sum_j J^-1_kj w_j, w_j = sum_k L_kj S_jk (L the velocity
+// gradient, S the PK2 stress), and writes per corner the damping force
+// alpha sum_g N_a q(g) - sum_g sum_k dN_a/dxi_k z_k to edamp.
+template <typename R, int EPB, bool STORE, int SUR = 0, bool GATHER = false, int LAWS = 0,
+          bool DAMP = false>
+__global__ void __launch_bounds__(EPB * 8, (ElemOccupancy<R, LAWS, DAMP>::waves))
+    sld_element(ElemArgs A) {
+  constexpr int NC = DAMP ? 9 : 6;        // gathered components: x, u (, v)
+  __shared__ R s_rel[EPB][NC][8];         // x_a - x_0 (0..2), u_a - u_0 (3..5), v_a - v_0 (6..8)
+  // V(h) of the components; rows padded to 8 for aligned reads, except in
   // the neo-Hookean-only kernel, whose occupancy the LDS bounds
-  __shared__ R s_v[EPB][8][LAWS == 2 ? 6 : 8];
-  __shared__ R s_q[EPB][9][8];            // Q(3i+k) at each Gauss point
-  __shared__ R s_u4[EPB][8][4];           // U0, U1, U2, U12 of Q(0..7)
+  __shared__ R s_v[EPB][8][DAMP ? 10 : (LAWS == 2 ? 6 : 8)];
+  __shared__ R s_q[EPB][DAMP ? 15 : 9][8];   // Q(3i+k) (, q_i, z_k) at each Gauss point
+  __shared__ R s_u4[EPB][DAMP ? 11 : 8][4];  // U0, U1, U2, U12 of Q(0..7) (, z_0..2)
+  __shared__ R s_wq[DAMP ? EPB : 1][3][8];   // Walsh-Hadamard coefficients of q_i
+  __shared__ double s_v0[DAMP ? EPB : 1][3]; // node 0's velocity
 
   if (A.health->halted) return;
   const int tid = threadIdx.x;
@@ -749,7 +815,9 @@ __global__ void __launch_bounds__(EPB * 8, (ElemOccupancy<R, LAWS>::waves)) sld_
   // -- phase 1: gather the element's 8 nodes (thread g loads node g); node
   // 0's values come from the element's first lane by a wave shuffle
   int node = 0;
-  double xa[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  double xa[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) xa[c] = 0.0;
   if (valid) {
     node = A.lnods[(int64_t)e * 8 + g];
     xa[0] = A.coord[0][node];
@@ -758,27 +826,35 @@ __global__ void __launch_bounds__(EPB * 8, (ElemOccupancy<R, LAWS>::waves)) sld_
     xa[3] = A.u[0][node];
     xa[4] = A.u[1][node];
     xa[5] = A.u[2][node];
+    if constexpr (DAMP) {
+      xa[6] = A.vel[0][node];
+      xa[7] = A.vel[1][node];
+      xa[8] = A.vel[2][node];
+    }
   }
   {
     const int lane0 = (tid & 63) & ~7;   // lane of the element's node 0
 #pragma unroll
-    for (int c = 0; c < 6; ++c) {
+    for (int c = 0; c < NC; ++c) {
       const double x0 = __shfl(xa[c], lane0, 64);
       if (valid) s_rel[le][c][g] = (R)(xa[c] - x0);
+      if constexpr (DAMP)
+        if (c >= 6 && valid && g == 0) s_v0[le][c - 6] = x0;
     }
   }
   __syncthreads();
-  // thread c < 6: V(h) of component c, nodes taken in sign-bit order
-  if (valid && g < 6) {
-    const int c = opaque(g);
-    R v[8];
+  // thread c: V(h) of component c, nodes taken in sign-bit order
+  if (valid && g < NC) {
+    for (int c = opaque(g); c < NC; c += 8) {
+      R v[8];
 #pragma unroll
-    for (int b = 0; b < 8; ++b) v[b] = s_rel[le][c][hex_node((b >> 2) & 1, (b >> 1) & 1, b & 1)];
-    wht_stage(v, 4);
-    wht_stage(v, 2);
-    wht_stage(v, 1);
+      for (int b = 0; b < 8; ++b) v[b] = s_rel[le][c][hex_node((b >> 2) & 1, (b >> 1) & 1, b & 1)];
+      wht_stage(v, 4);
+      wht_stage(v, 2);
+      wht_stage(v, 1);
 #pragma unroll
-    for (int h = 0; h < 8; ++h) s_v[le][h][c] = v[h];
+      for (int h = 0; h < 8; ++h) s_v[le][h][c] = v[h];
+    }
   }
   __syncthreads();
 
@@ -973,6 +1049,76 @@ __global__ void __launch_bounds__(EPB * 8, (ElemOccupancy<R, LAWS>::waves)) sld_
         s_q[le][3 * i + k][g] =
             detJ * (T[i] * jci[k] + T[i + 3] * jci[k + 3] + T[i + 6] * jci[k + 6]);
 
+    if constexpr (DAMP) {
+      // Rayleigh damping at Gauss point g.  J and the velocity's natural
+      // derivatives are formed again from the V(h) rows (they are not kept
+      // through the constitutive update); the velocity interpolated at g is
+      // gpvel = v_0 + 1/8 sum_h c^|h| s_g(h) V_v(h) (sld_elmpre.f90:110-114)
+      const int gd = opaque(g);
+      const R c1 = (R)(0.125 * HEX_GPOS), c2 = (R)(0.125 * HEX_GPOS * HEX_GPOS);
+      const R c3 = (R)(0.125 * HEX_GPOS * HEX_GPOS * HEX_GPOS), k0 = (R)0.125;
+      const R ax = (gd & 4) ? c1 : -c1, ay = (gd & 2) ? c1 : -c1, az = (gd & 1) ? c1 : -c1;
+      const R byz = ((gd >> 1) ^ gd) & 1 ? -c2 : c2;
+      const R bxz = ((gd >> 2) ^ gd) & 1 ? -c2 : c2;
+      const R bxy = ((gd >> 2) ^ (gd >> 1)) & 1 ? -c2 : c2;
+      const R bxyz = ((gd >> 2) ^ (gd >> 1) ^ gd) & 1 ? c3 : -c3;
+      // weights of V(h): h = 0, z, y, yz, x, xz, xy, xyz (bits x = 4, y = 2, z = 1)
+      const R wv[8] = {k0, az, ay, byz, ax, bxz, bxy, bxyz};
+      const double alpha = A.damping[2 * mat], beta = A.damping[2 * mat + 1];
+      const R arho = (R)(alpha * A.rho[mat]);
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        R gv = R(0);
+#pragma unroll
+        for (int h = 0; h < 8; ++h) gv += wv[h] * s_v[le][h][6 + i];
+        gv += (R)s_v0[le][i];
+        s_q[le][9 + i][g] = arho * detJ * gv;
+      }
+      R z[3] = {R(0), R(0), R(0)};
+      if (beta > 0.0) {
+        // the beta term as written (sld_elmmat.f90:174-194):
+        // z_k = beta detJ sum_j J^-1_kj w_j, w_j = sum_k L_kj S_jk with the
+        // velocity gradient L = G_v J^-1 (gprat, sld_elmpre.f90:150-157)
+        R Jd[9], Gv[9];
+        const R* V7 = s_v[le][7];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          const int hk = 4 >> k, h1 = k == 0 ? 2 : 4, h2 = k == 2 ? 2 : 1;
+          const R w1 = k == 0 ? ay : ax, w2 = k == 2 ? ay : az, w3 = k == 0 ? byz : k == 1 ? bxz : bxy;
+          const R* Vk = s_v[le][hk];
+          const R* V1 = s_v[le][hk | h1];
+          const R* V2 = s_v[le][hk | h2];
+#pragma unroll
+          for (int i = 0; i < 3; ++i) {
+            Jd[i + 3 * k] = k0 * Vk[i] + w1 * V1[i] + w2 * V2[i] + w3 * V7[i];
+            Gv[i + 3 * k] = k0 * Vk[6 + i] + w1 * V1[6 + i] + w2 * V2[6 + i] + w3 * V7[6 + i];
+          }
+        }
+        R Jid[9];
+        sma::inv3<R>(Jd, Jid);
+        R L[9], Fi[9], S[9];
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            L[r + 3 * c] = Gv[r] * Jid[3 * c] + Gv[r + 3] * Jid[1 + 3 * c] + Gv[r + 6] * Jid[2 + 3 * c];
+            // F^-1 = J j^-1 (j = F J)
+            Fi[r + 3 * c] = Jd[r] * jci[3 * c] + Jd[r + 3] * jci[1 + 3 * c] + Jd[r + 6] * jci[2 + 3 * c];
+          }
+        sma::pk2<R>(Fi, tau, S);
+        R w[3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+          w[j] = L[3 * j] * S[j] + L[1 + 3 * j] * S[j + 3] + L[2 + 3 * j] * S[j + 6];
+        const R bd = (R)beta * detJ;
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+          z[k] = bd * (Jid[k] * w[0] + Jid[k + 3] * w[1] + Jid[k + 6] * w[2]);
+      }
+#pragma unroll
+      for (int k = 0; k < 3; ++k) s_q[le][12 + k][g] = z[k];
+    }
+
     if constexpr (STORE) {
       R Fi[9], S[9];
       sma::inv3<R>(Fm, Fi);
@@ -994,10 +1140,26 @@ __global__ void __launch_bounds__(EPB * 8, (ElemOccupancy<R, LAWS>::waves)) sld_
   if constexpr (SUR == 1) continue;   // uniform: the prepass has no scatter
   count_flags(A.counters, fl, valid);
   __syncthreads();
-  // thread g: U of Q(j = g) (Q(8) is formed in phase 3 by every thread)
+  // thread g: U of Q(j = g) (Q(8) is formed in phase 3 by every thread);
+  // with damping threads 0-2 also form U of z_0..2 and threads 3-5 the
+  // Walsh-Hadamard coefficients of q_0..2
   if (valid) {
     const int j = opaque(g);
     u4_of_q(s_q[le][j], j % 3, s_u4[le][j]);
+    if constexpr (DAMP) {
+      if (j < 3) {
+        u4_of_q(s_q[le][12 + j], j, s_u4[le][8 + j]);
+      } else if (j < 6) {
+        R v[8];
+#pragma unroll
+        for (int b = 0; b < 8; ++b) v[b] = s_q[le][9 + j - 3][b];
+        wht_stage(v, 4);
+        wht_stage(v, 2);
+        wht_stage(v, 1);
+#pragma unroll
+        for (int h = 0; h < 8; ++h) s_wq[le][j - 3][h] = v[h];
+      }
+    }
   }
   __syncthreads();
 
@@ -1027,6 +1189,40 @@ __global__ void __launch_bounds__(EPB * 8, (ElemOccupancy<R, LAWS>::waves)) sld_
       }
     }
     const R f0 = f[0], f1 = f[1], f2 = f[2];
+    if constexpr (DAMP) {
+      // damping force of corner a: alpha sum_g N_a(g) q(g) =
+      // 1/8 sum_h c^|h| s_a(h) Q_q(h), less the beta sum
+      // sum_g sum_k dN_a/dxi_k z_k (one value for the three components)
+      R cb = R(0);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const int sk = k == 0 ? sx : k == 1 ? sy : sz;
+        const int s1 = k == 0 ? sy : sx, s2 = k == 2 ? sy : sz;
+        const R l0 = sk ? c0 : -c0;
+        const R l1 = (sk ^ s1 ^ 1) ? c1 : -c1;
+        const R l2 = (sk ^ s2 ^ 1) ? c1 : -c1;
+        const R l3 = (sk ^ s1 ^ s2) ? c2 : -c2;
+        const R* u = s_u4[le][8 + k];
+        cb += l0 * u[0] + l1 * u[1] + l2 * u[2] + l3 * u[3];
+      }
+      const R c3 = (R)(0.125 * HEX_GPOS * HEX_GPOS * HEX_GPOS);
+      const R px = sx ? c1 : -c1, py = sy ? c1 : -c1, pz = sz ? c1 : -c1;
+      const R pyz = (sy ^ sz) ? -c2 : c2, pxz = (sx ^ sz) ? -c2 : c2, pxy = (sx ^ sy) ? -c2 : c2;
+      const R pxyz = (sx ^ sy ^ sz) ? c3 : -c3;
+      const R wa[8] = {c0, pz, py, pyz, px, pxz, pxy, pxyz};
+      R fd[3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        R r = R(0);
+#pragma unroll
+        for (int h = 0; h < 8; ++h) r += wa[h] * s_wq[le][i][h];
+        fd[i] = r - cb;
+      }
+      R* ed = (R*)A.edamp + 3 * (int64_t)A.corner_slot[(int64_t)e * 8 + a];
+      ed[0] = fd[0];
+      ed[1] = fd[1];
+      ed[2] = fd[2];
+    }
     if constexpr (GATHER) {
       R* ef = (R*)A.efor + 3 * (int64_t)A.corner_slot[(int64_t)e * 8 + a];
       ef[0] = f0;

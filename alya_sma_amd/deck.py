@@ -16,14 +16,16 @@ one this path uses:
   .sld.dat   PROBLEM_DEFINITION: GRAVITY (NORM, GX, GY, GZ);
              PROPERTIES: MATERIAL, DENSITY, CONSTITUTIVE_MODEL NITINOL
              (sld_reaphy.f90: parco(1:17)) or NEOHO ANSYS mu K
-             (lawst 102), FORCE: FIELD = n (nodal forces ramped by
-             t/(t1 - t0), sld_matrix.f90:249-262); NUMERICAL_TREATMENT:
+             (lawst 102), RAYLEIGH_DAMPING: RAYALPHA, RAYBETA
+             (sld_reaphy.f90:1046-1049), FORCE: FIELD = n (nodal forces
+             ramped by t/(t1 - t0), sld_matrix.f90:249-262); NUMERICAL_TREATMENT:
              TIME_INTEGRATION (TCHAMWA_WIELGOSZ with PHITW, or central
              difference), SAFETY_FACTOR; BOUNDARY_CONDITIONS[, TRANSIENT |
              CONSTANT] with CODES, NODES rules ("c1 & c2 ... fixity values
              [, MODULE_FUNCTION= f]"), CODES, BOUNDARIES rules ("code fixbo
              v1 v2 v3 [, MODULE_FUNCTION= f]", fixbo 3 pressure or 6
-             traction: sld_bouope.f90), FOLLOWER_LOADS: ON, and FUNCTIONS
+             traction: sld_bouope.f90), BODYF = function (body force,
+             sld_reabcs.f90:523), FOLLOWER_LOADS: ON, and FUNCTIONS
              (DISCRETE[, SMOOTH] shape tables, sld_funbou.f90)
 
 A node's codes are the codes of the boundaries whose faces contain it
@@ -165,18 +167,33 @@ def read_case(dat_path):
         v = l.split()
         node_codes[nidx[int(v[0])]].add(int(v[1]))
 
-    # ---- node sets (.set.dat)
-    node_sets = {}
+    # ---- node sets (.set.dat): each node takes the last set it is listed
+    # in (lnset), and a set's output node is the highest-numbered node
+    # carrying it (mod_reaset.f90:502-507, lnsec(lnset(ipoin)) = ipoin over
+    # ascending ipoin)
+    lnset = {}
     for l in _block(dom, "NODE", "END_NODE"):
         v = l.split()
         if len(v) >= 2 and v[0].isdigit():
-            node_sets.setdefault(int(v[1]), []).append(nidx[int(v[0])])
+            lnset[int(v[0])] = int(v[1])
+    node_sets = {}
+    for nid in sorted(lnset):
+        if lnset[nid] > 0:
+            node_sets.setdefault(lnset[nid], []).append(nidx[nid])
 
     # ---- materials (.sld.dat)
-    props, rho, law = {}, {}, {}
+    props, rho, law, damp = {}, {}, {}, {}
     force_field = None
     cur = 1
     for l in _block(sld, "PROPERTIES", "END_PROPERTIES"):
+        # RAYLEIGH_DAMPING: RAYALPHA=a, RAYBETA=b (sld_reaphy.f90:1046-1049;
+        # Alya matches keywords on their first five letters, 'RAYLE', and
+        # takes absent values as 0)
+        if re.match(r"RAYLE\w*", l, re.I):
+            def rv(key):
+                mm = re.search(key + r"\w*\s*=\s*([^\s,]+)", l, re.I)
+                return _num(mm.group(1)) if mm else 0.0
+            damp[cur] = (rv("RAYAL"), rv("RAYBE"))
         m = re.match(r"FORCE\s*:\s*FIELD\s*=\s*(\d+)", l, re.I)
         if m:
             force_field = int(m.group(1))
@@ -203,6 +220,9 @@ def read_case(dat_path):
             props[cur] = p
     mat_ids = sorted(props)
     midx = {m: i for i, m in enumerate(mat_ids)}
+    damping = None
+    if any(a > 0 or b > 0 for a, b in damp.values()):
+        damping = np.array([damp.get(m, (0.0, 0.0)) for m in mat_ids], np.float64)
     mat_of_elem = np.array([midx.get(mats.get(e, mat_ids[0]), 0) for e in elem_ids], dtype=np.int32)
 
     # ---- integrator (.sld.dat NUMERICAL_TREATMENT)
@@ -344,6 +364,21 @@ def read_case(dat_path):
                 fext0=fext0, fext_func=fext_func)
     if faces is not None:
         prob["faces"] = faces
+    if damping is not None:
+        prob["damping"] = damping
+    # ---- body force (BOUNDARY_CONDITIONS: BODYF = function, sld_reabcs.f90:
+    # 523): bodyf_sld follows the function (sld_funbou.f90 itask 3) when the
+    # conditions are transient and a node carries a solidz function
+    # (sld_updbcs.f90:78-99); otherwise it stays zero
+    for l in sld:
+        mb = re.match(r"BODYF\w*\s*[:=]?\s*(\d+)", l, re.I)
+        if mb:
+            fb = int(mb.group(1))
+            if fb > 0 and transient and np.any(func_of_node >= 0):
+                if fb not in fidx:
+                    raise ValueError(f"{name}: BODYF = {fb} names no function")
+                prob["body_func"] = fidx[fb]
+            break
     # ---- gravity (.sld.dat PROBLEM_DEFINITION, sld_reaphy.f90:209-223):
     # g = NORM times the normalised (GX, GY, GZ), each 0 when absent
     for l in _block(sld, "PROBLEM_DEFINITION", "END_PROBLEM_DEFINITION"):
@@ -379,6 +414,7 @@ def read_case(dat_path):
                                variables=[v for v in wvars if v in VARIABLES])
     prob["run"] = dict(t0=t0, t_end=t1, max_steps=nmax, scheme=scheme, phi=phi, safety=safety)
     prob["node_sets"] = {k: np.array(v, np.int64) for k, v in sorted(node_sets.items())}
+    prob["set_node"] = {k: int(v[-1]) for k, v in sorted(node_sets.items())}
     prob["node_codes"] = node_codes
     prob["node_ids"] = np.array(node_ids)
     prob["name"] = name
@@ -395,18 +431,17 @@ def reactions(prob, fint, mass=None):
     return np.where(mask, r, 0.0)
 
 
-def write_node_sets(path, prob, history, mass=None):
-    """history: list of (step, time, u [npoin,3], fint [npoin,3]).  One row
-    per (step, set): sums over the set's nodes of DISPX..Z / their count
-    (the set's mean displacement) and of the reactions FRXIX..Z."""
+def write_node_sets(path, prob, history):
+    """history: list of (step, time, u [npoin,3], frx [npoin,3]) with frx =
+    -frxid_sld (FRXIX..Z, sld_nodset.f90:98-103).  One row per (step, set):
+    DISPX..Z and FRXIX..Z of the set's output node (sld_nodset.f90:74-103)."""
     with open(path, "w") as f:
         f.write("# node sets of " + prob["name"] + "\n")
         f.write("# columns: step time set DISPX DISPY DISPZ FRXIX FRXIY FRXIZ\n")
-        for step, t, u, fint in history:
-            r = reactions(prob, fint, mass)
-            for s, nodes in prob["node_sets"].items():
-                d = u[nodes].mean(axis=0)
-                q = r[nodes].sum(axis=0)
+        for step, t, u, frx in history:
+            for s, node in prob["set_node"].items():
+                d = u[node]
+                q = frx[node]
                 f.write(f"{step:d} {t:.15e} {s:d} " + " ".join(f"{x:.15e}" for x in (*d, *q)) + "\n")
 
 
@@ -466,18 +501,20 @@ def run_case(dat_path, out_dir=".", every=None, mode=0):
             nsteps = min(nsteps, r["max_steps"])
         every = every or max(1, nsteps // 200)
         hist, wrows, done = [], [], 0
-        mass = dev.get("mass")
         while done < nsteps:
             n = min(every, nsteps - done)
             dev.step(dt, nsteps=n, scheme=r["scheme"], phi=r["phi"])
             done += n
             t = float(dev.get("time")[0])
-            hist.append((done, t, dev.get("u"), dev.get("fint")))
+            # FRXIX = -frxid: the device's reactions hold frxid (f_ext - f_int
+            # - the damping force on prescribed dofs, gravity and body force
+            # included)
+            hist.append((done, t, dev.get("u"), -dev.get("reactions")))
             if wit is not None:
                 wrows.append((done, t, W.values(prob["lnods"], welem, wxi, witness_fields(prob, dev),
                                                 wit["variables"])))
     path = os.path.join(out_dir, prob["name"] + "-nodes.sld.set")
-    write_node_sets(path, prob, hist, mass)
+    write_node_sets(path, prob, hist)
     if wit is not None:
         write_witness(os.path.join(out_dir, prob["name"] + "-witness.sld.wit"), prob, wrows)
     return path, prob, dt, nsteps

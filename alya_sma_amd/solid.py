@@ -100,6 +100,10 @@ class Solidz:
             self.set_faces(prob["faces"])
         if prob.get("gravity") is not None:
             self.set_gravity(prob["gravity"])
+        if prob.get("damping") is not None:
+            self.set_damping(prob["damping"])
+        if prob.get("body_func", -1) >= 0:
+            self.set_body_force(prob["body_func"])
 
     def close(self):
         if getattr(self, "h", None):
@@ -137,6 +141,19 @@ class Solidz:
             return
         gv = (C.c_double * 3)(*[float(x) for x in g])
         check(lib().sld_dev_set_gravity(self.h, gv), "sld_dev_set_gravity")
+
+    def set_damping(self, alpha_beta):
+        """Rayleigh damping (alpha, beta) per material, [nmat, 2]
+        (RAYLEIGH_DAMPING: RAYALPHA, RAYBETA); None removes it."""
+        if alpha_beta is None:
+            check(lib().sld_dev_set_damping(self.h, None), "sld_dev_set_damping")
+            return
+        ab = np.ascontiguousarray(np.atleast_2d(alpha_beta), dtype=np.float64)
+        check(lib().sld_dev_set_damping(self.h, _p(ab)), "sld_dev_set_damping")
+
+    def set_body_force(self, func):
+        """Body force from load function `func` (BODYF); -1 removes it."""
+        check(lib().sld_dev_set_body_force(self.h, int(func)), "sld_dev_set_body_force")
 
     def set_surrogate(self, surrogate, tol=1e-4):
         """Attach a trained alya_sma_amd.surrogate.Surrogate: mode 2 then

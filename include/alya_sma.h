@@ -255,6 +255,29 @@ int sld_dev_set_faces(sld_dev *dev, const sld_face_desc *faces);
  * mass row sum); the reactions field includes it.  g = NULL removes it. */
 int sld_dev_set_gravity(sld_dev *dev, const double *g);
 
+/* Body force (BODYF = func in the .sld.dat boundary conditions,
+ * sld_reabcs.f90:523; its element load in sld_elmmat.f90:156-157 with
+ * bodyf_sld from sld_updbcs.f90:96-99): every dof, prescribed ones included,
+ * receives m_a b(t^{n+1}), where b is load function func evaluated by the
+ * rule of sld_funbou.f90 itask 3 (linear between the rows with
+ * t1 <= t < t2, zero outside the table) and m_a the lumped mass (the
+ * element load summed over the node's elements).  Adds to gravity.
+ * func = -1 removes it. */
+int sld_dev_set_body_force(sld_dev *dev, int32_t func);
+
+/* Rayleigh damping (RAYLEIGH_DAMPING: RAYALPHA, RAYBETA of the .sld.dat
+ * material, dampi_sld(1:2, imate), sld_reaphy.f90:1046-1049), the element
+ * terms of sld_elmmat.f90:160-163 and :174-194: the residual of node a
+ * loses alpha sum_g N_a rho w detJ v(g) (v interpolated from
+ * veloc_sld(ITER_K): v^n under central differences, the predicted v^{n+1}
+ * under Tchamwa-Wielgosz) and gains, in each of its three components, the
+ * beta term as written, beta sum_g sum_jk dN_a/dX_j L_kj S_jk w detJ (L the
+ * velocity gradient, S the PK2 stress).  alpha_beta [nmat][2] (>= 0);
+ * NULL or all zero removes it.  The internal force field stays f_int
+ * alone (finte_sld); the reactions include the damping force.  Damping
+ * runs with the gather assembly only (selected here). */
+int sld_dev_set_damping(sld_dev *dev, const double *alpha_beta);
+
 /* Energy balance of the explicit run (replaces sld_energy,
  * mod_sld_energy.f90:71-162, called from sld_endste.f90): on = 1 starts a
  * fresh balance (allocated on first use), on = 0 stops accumulating.  Each

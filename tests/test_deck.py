@@ -104,3 +104,48 @@ def test_deck_gravity(tmp_path):
     os.makedirs(str(tmp_path / "plain"))
     assert "gravity" not in deck.read_case(write_beam_deck(str(tmp_path / "plain"))[0])
 
+
+
+def _copy_case(tmp_path):
+    import shutil
+    src = os.path.dirname(CASE)
+    dst = str(tmp_path / "case")
+    shutil.copytree(src, dst)
+    return os.path.join(dst, "sm154_d1C.dat"), dst
+
+
+def test_deck_node_sets_take_one_node_each(tmp_path):
+    """A set id listed for several nodes outputs its highest-numbered node
+    (mod_reaset.f90:502-507), and a node listed twice keeps its last set."""
+    path, d = _copy_case(tmp_path)
+    with open(os.path.join(d, "sm154_d1C.set.dat"), "w") as f:
+        f.write("NODE\n  4 1\n  2 1\n  8 2\n  6 2\n  5 2\n  5 1\nEND_NODE\n")
+    p = deck.read_case(path)
+    assert {k: v.tolist() for k, v in p["node_sets"].items()} == {1: [1, 3, 4], 2: [5, 7]}
+    assert p["set_node"] == {1: 4, 2: 7}
+
+
+def test_deck_rayleigh_damping_and_body_force(tmp_path):
+    """RAYLEIGH_DAMPING: RAYALPHA/RAYBETA per material (sld_reaphy.f90:
+    1046-1049; 'RAYLE' matches on five letters, absent values are 0) and
+    BODYF = function under transient conditions (sld_reabcs.f90:523)."""
+    path, d = _copy_case(tmp_path)
+    sld = os.path.join(d, "sm154_d1C.sld.dat")
+    text = open(sld).read()
+    text = text.replace("    DENSITY           = 6.4e-9\n",
+                        "    DENSITY           = 6.4e-9\n    RAYLEIGH_DUMPING, RAYALPHA= 100000\n")
+    text = text.replace("BOUNDARY_CONDITIONS, TRANSIENT\n",
+                        "BOUNDARY_CONDITIONS, TRANSIENT\n  BODYF = 1\n")
+    open(sld, "w").write(text)
+    p = deck.read_case(path)
+    np.testing.assert_array_equal(p["damping"], [[1.0e5, 0.0]])
+    assert p["body_func"] == 0
+    text = text.replace("RAYLEIGH_DUMPING, RAYALPHA= 100000",
+                        "RAYLEIGH_DAMPING: RAYALPHA=2.5, RAYBETA=0.125")
+    text = text.replace("BOUNDARY_CONDITIONS, TRANSIENT", "BOUNDARY_CONDITIONS, CONSTANT")
+    open(sld, "w").write(text)
+    p = deck.read_case(path)
+    np.testing.assert_array_equal(p["damping"], [[2.5, 0.125]])
+    assert "body_func" not in p           # constant conditions: bodyf_sld stays 0
+    q = deck.read_case(CASE)
+    assert "damping" not in q and "body_func" not in q

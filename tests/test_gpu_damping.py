@@ -1,0 +1,153 @@
+"""Rayleigh damping (sld_elmmat.f90:160-194) and the body force
+(sld_elmmat.f90:156-157, sld_funbou.f90 itask 3) on the GPU against the
+oracle run in the same colour order.  With alpha = 1e5 (RAYALPHA of the
+reference's cantilever deck, cantiliver_nitinol.sld.dat:28) the
+load-unload cycle of a cantilever runs free -- no restart from the oracle --
+and stays within 1e-10 of it; the beta term, the reactions (which include
+the damping force) and the FP32 mode are checked on the same mesh."""
+import numpy as np
+import pytest
+
+from alya_sma_amd import problems
+from alya_sma_amd.solid import Solidz
+from oracle import oracle_py as O
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-10
+
+
+def _rel(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+def _cantilever(alpha=1.0e5, beta=0.0, nsteps=2000):
+    p = problems.cantilever_problem(nx=48, ny=4, nz=4, lx=30.0, ly=2.5, lz=2.5, traction=600.0,
+                                    nsteps=nsteps)
+    p["damping"] = np.array([[alpha, beta]])
+    return p
+
+
+def _pair(prob):
+    dev = Solidz(prob)
+    dev.set_store(True)
+    ptr, order = dev.coloring()
+    orc = O.StepOracle(prob, elem_order=order, color_ptr=ptr)
+    return dev, orc
+
+
+def _errors(dev, orc):
+    a_scale = max(np.max(np.abs(orc.fint)) / np.min(orc.mass), np.max(np.abs(orc.a)))
+    S_scale = max(np.max(np.abs(orc.S)), 1e-4 * orc.E_max)
+    return dict(u=_rel(dev.get("u"), orc.u), v=_rel(dev.get("v"), orc.v),
+                a=np.max(np.abs(dev.get("a") - orc.a)) / a_scale,
+                fint=_rel(dev.get("fint"), orc.fint),
+                S=np.max(np.abs(dev.get("stress") - orc.S)) / S_scale)
+
+
+@pytest.mark.parametrize("scheme", [0, 1])
+def test_damped_cantilever_free_run_matches_oracle(scheme):
+    prob = _cantilever()
+    r = prob["run"]
+    dev, orc = _pair(prob)
+    transforming = 0
+    with dev:
+        for chunk in range(8):
+            dev.step(r["dt"], nsteps=250, scheme=scheme, phi=r["phi"])
+            orc.step(r["dt"], scheme=scheme, phi=r["phi"], n=250)
+            errs = _errors(dev, orc)
+            fl = dev.get("flags")
+            transforming = max(transforming, int(np.count_nonzero(fl & 3)))
+            print(f"scheme {scheme} step {250 * (chunk + 1)}: "
+                  f"{ {k: f'{x:.1e}' for k, x in errs.items()} }, transforming {transforming}",
+                  flush=True)
+            for k, x in errs.items():
+                assert x < TOL, (chunk, k, x)
+            np.testing.assert_array_equal(fl & 3, orc.flags & 3)
+            sdv = dev.get("sdv")
+            assert np.max(np.abs(sdv[..., 0] - orc.sdv[0, ..., 0])) < 1e-10
+            assert _rel(sdv[..., 1:], orc.sdv[0, ..., 1:]) < TOL
+        assert transforming > 0
+        # reactions: f_ext - f_int - the damping force on prescribed dofs
+        R = dev.get("reactions")
+        fixed = np.stack([(prob["fixmask"] >> k) & 1 for k in range(3)], axis=1).astype(bool)
+        np.testing.assert_array_equal(R[~fixed], 0.0)
+        assert _rel(R[fixed], orc.rhs[fixed]) < TOL
+
+
+def test_damping_takes_energy_out():
+    """The free vibration after the load cycle decays with damping and
+    does not without it (same run, same state at the cycle's end)."""
+    prob = _cantilever(nsteps=600)
+    r = prob["run"]
+    und = dict(prob)
+    und.pop("damping")
+    ke = {}
+    for name, p in (("damped", prob), ("undamped", und)):
+        with Solidz(p) as dev:
+            dev.step(r["dt"], nsteps=600)
+            m = dev.get("mass")
+            k0 = 0.5 * np.sum(m[:, None] * dev.get("v") ** 2)
+            dev.step(r["dt"], nsteps=1200)
+            ke[name] = (k0, 0.5 * np.sum(m[:, None] * dev.get("v") ** 2))
+    print("kinetic energy before / after 1200 free steps:", ke, flush=True)
+    assert ke["damped"][1] < 0.2 * ke["damped"][0]
+    assert ke["undamped"][1] > 0.2 * ke["undamped"][0]
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_beta_term_matches_oracle(mode):
+    prob = _cantilever(alpha=2.0e4, beta=3.0e-7, nsteps=800)
+    r = prob["run"]
+    dev, orc = _pair(prob)
+    with dev:
+        dev.set_mode(mode)
+        dev.step(r["dt"], nsteps=400)
+        orc.step(r["dt"], n=400)
+        if mode == 0:
+            errs = _errors(dev, orc)
+            print("beta term, 400 steps:", {k: f"{x:.1e}" for k, x in errs.items()}, flush=True)
+            for k, x in errs.items():
+                assert x < TOL, (k, x)
+        else:
+            # FP32 kinematics and forces: one step from the FP64 state
+            dev.set("u", orc.u)
+            dev.set("v", orc.v)
+            dev.set("a", orc.a)
+            dev.set("sdv", orc.sdv[1])
+            dev.set("time", np.array([orc.t]))
+            dev.step(r["dt"], nsteps=1)
+            orc.step(r["dt"], n=1)
+            eS = np.linalg.norm(dev.get("stress") - orc.S) / np.linalg.norm(orc.S)
+            ea = np.linalg.norm(dev.get("a") - orc.a) / np.linalg.norm(orc.a)
+            print(f"FP32 damped step: stress {eS:.1e}, a {ea:.1e}", flush=True)
+            assert eS < 1e-4 and ea < 1e-3
+
+
+@pytest.mark.parametrize("scheme", [0, 1])
+def test_body_force_matches_oracle(scheme):
+    prob = problems.bar_problem(nsteps=400)
+    r = prob["run"]
+    T = 400 * r["dt"]
+    old = prob["func_table"]
+    mp = max(old.shape[1], 3)
+    tab = np.zeros((2, mp, 4))
+    tab[0, :old.shape[1]] = old[0]
+    # body force (mm/s^2): a ramp, then a change of direction, zero from 0.9 T on
+    tab[1, 0] = [0.0, 0.0, 0.0, 0.0]
+    tab[1, 1] = [2.0e10, -1.0e10, 5.0e9, 0.4 * T]
+    tab[1, 2] = [0.0, 3.0e10, 0.0, 0.9 * T]
+    prob["func_table"] = tab
+    prob["func_npts"] = np.array([prob["func_npts"][0], 3], np.int32)
+    prob["func_smooth"] = np.array([prob["func_smooth"][0], 0], np.int32)
+    prob["body_func"] = 1
+    dev, orc = _pair(prob)
+    with dev:
+        dev.step(r["dt"], nsteps=400, scheme=scheme, phi=r["phi"])
+        orc.step(r["dt"], scheme=scheme, phi=r["phi"], n=400)
+        errs = _errors(dev, orc)
+        print(f"body force scheme {scheme}:", {k: f"{x:.1e}" for k, x in errs.items()}, flush=True)
+        for k, x in errs.items():
+            assert x < TOL, (k, x)
+        R = dev.get("reactions")
+        fixed = np.stack([(prob["fixmask"] >> k) & 1 for k in range(3)], axis=1).astype(bool)
+        assert _rel(R[fixed], orc.rhs[fixed]) < TOL

@@ -32,12 +32,12 @@ def test_deck_on_gpu_matches_oracle(tmp_path):
         o.step(dt, scheme=1, phi=prob["run"]["phi"], n=m)
         done += m
         r = deck.reactions(prob, o.fint)
-        for s, nodes in prob["node_sets"].items():
+        for s, node in prob["set_node"].items():
             row = rows[2 * k + (s - 1)]
             assert row[0] == done and row[2] == s
             if done <= 40000:
-                np.testing.assert_allclose(row[3:6], o.u[nodes].mean(axis=0), rtol=0, atol=1e-12)
-                np.testing.assert_allclose(row[6:9], r[nodes].sum(axis=0), rtol=0,
+                np.testing.assert_allclose(row[3:6], o.u[node], rtol=0, atol=1e-12)
+                np.testing.assert_allclose(row[6:9], r[node], rtol=0,
                                            atol=1e-10 * max(1.0, np.abs(o.fint).max()))
         k += 1
     # closing state: the prescribed face is back at x = 0, both runs finite
@@ -67,9 +67,9 @@ def test_neumann_deck_on_gpu_matches_oracle(tmp_path):
         o.step(dt, scheme=1, phi=prob["run"]["phi"], n=m)
         done += m
         row = rows[k]
-        nodes = prob["node_sets"][1]
+        node = prob["set_node"][1]
         assert row[0] == done
-        ref = o.u[nodes].mean(axis=0)
+        ref = o.u[node]
         moved = max(moved, np.abs(ref).max())
         np.testing.assert_allclose(row[3:6], ref, rtol=0, atol=1e-10 * max(moved, 1e-30))
         # witness points: the oracle's fields through the same interpolation
