@@ -941,10 +941,18 @@ __global__ void __launch_bounds__(EPB * 8, (ElemOccupancy<R, LAWS, DAMP>::waves)
       double F[9];
 #pragma unroll
       for (int q = 0; q < 9; ++q) F[q] = H[q] + ((q % 4 == 0) ? 1.0 : 0.0);
-      sma::Spectral<double> sp = sma::spectral_fp64(F);
-      rm = sma::return_map(m, sp.theta, sp.e_n, xS0, FAS0, FSA0);
-      sma::kirchhoff<double>(m, sp, rm.xS, tau);
-      if (!sp.eig_ok) rm.flags |= sma::FLAG_EIGEN_FAIL;
+      // the Hencky tensor by series (theta, e' and e_n, no eigenvectors);
+      // large stretches take the reference's spectral path
+      const sma::Hencky64 hk = sma::hencky_fp64(H);
+      if (hk.ok) {
+        rm = sma::return_map(m, hk.theta, hk.e_n, xS0, FAS0, FSA0);
+        sma::kirchhoff_hencky64(m, hk, rm.xS, tau);
+      } else {
+        sma::Spectral<double> sp = sma::spectral_fp64(F);
+        rm = sma::return_map(m, sp.theta, sp.e_n, xS0, FAS0, FSA0);
+        sma::kirchhoff<double>(m, sp, rm.xS, tau);
+        if (!sp.eig_ok) rm.flags |= sma::FLAG_EIGEN_FAIL;
+      }
 #pragma unroll
       for (int q = 0; q < 9; ++q) Fm[q] = F[q];
     } else {

@@ -434,6 +434,109 @@ __host__ __device__ __forceinline__ Hencky32 hencky_fp32(const float (&H)[9]) {
   return h;
 }
 
+// FP64 Hencky strain by the same series, for the plain FP64 element step:
+// E = 1/2 log b = sum_k Z^(2k+1)/(2k+1), Z = (b - I)(b + I)^-1, b - I formed
+// from H = F - I.  theta = tr E and e' = dev E are what the loading functions
+// and the stress consume (theta = log J, |e'| = e_n of compute_kirchoff_stress
+// :313-322), so no eigenvectors, square roots of eigenvalues, logarithms or
+// powers are needed; the terms run until the remainder is below 2^-54 of
+// |E| (per lane).  Stretches with |Z|_F > 0.3 take the reference's spectral
+// path instead (ok = false).
+struct Hencky64 {
+  double theta, e_n;
+  double ed[6];        // e' (00, 01, 02, 11, 12, 22)
+  bool ok;
+};
+
+__host__ __device__ __forceinline__ void sym_mul_commuting64(const double (&a)[6],
+                                                            const double (&b)[6], double (&c)[6]) {
+  c[0] = a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+  c[1] = a[0] * b[1] + a[1] * b[3] + a[2] * b[4];
+  c[2] = a[0] * b[2] + a[1] * b[4] + a[2] * b[5];
+  c[3] = a[1] * b[1] + a[3] * b[3] + a[4] * b[4];
+  c[4] = a[1] * b[2] + a[3] * b[4] + a[4] * b[5];
+  c[5] = a[2] * b[2] + a[4] * b[4] + a[5] * b[5];
+}
+
+__host__ __device__ __forceinline__ Hencky64 hencky_fp64(const double (&H)[9]) {
+  Hencky64 h;
+  double c[6];
+  c[0] = 2.0 * H[0] + (H[0] * H[0] + H[3] * H[3] + H[6] * H[6]);
+  c[3] = 2.0 * H[4] + (H[1] * H[1] + H[4] * H[4] + H[7] * H[7]);
+  c[5] = 2.0 * H[8] + (H[2] * H[2] + H[5] * H[5] + H[8] * H[8]);
+  c[1] = H[3] + H[1] + (H[0] * H[1] + H[3] * H[4] + H[6] * H[7]);
+  c[2] = H[6] + H[2] + (H[0] * H[2] + H[3] * H[5] + H[6] * H[8]);
+  c[4] = H[7] + H[5] + (H[1] * H[2] + H[4] * H[5] + H[7] * H[8]);
+  const double m00 = 2.0 + c[0], m11 = 2.0 + c[3], m22 = 2.0 + c[5];
+  double a[6];
+  a[0] = m11 * m22 - c[4] * c[4];
+  a[1] = c[2] * c[4] - c[1] * m22;
+  a[2] = c[1] * c[4] - c[2] * m11;
+  a[3] = m00 * m22 - c[2] * c[2];
+  a[4] = c[1] * c[2] - m00 * c[4];
+  a[5] = m00 * m11 - c[1] * c[1];
+  const double r = 1.0 / (m00 * a[0] + c[1] * a[1] + c[2] * a[2]);
+#pragma unroll
+  for (int q = 0; q < 6; ++q) a[q] *= r;
+  double z[6];
+  sym_mul_commuting64(c, a, z);
+  const double zf2 = z[0] * z[0] + z[3] * z[3] + z[5] * z[5] +
+                     2.0 * (z[1] * z[1] + z[2] * z[2] + z[4] * z[4]);
+  h.ok = zf2 <= 0.09;
+  double e[6];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) e[q] = z[q];
+  if (h.ok && zf2 > 0.0) {
+    // n terms after Z: the remainder |Z|^(2n+3)/((2n+3)(1 - |Z|^2)) stays
+    // below 2^-54 |Z| (|Z| <= |Z|_F)
+    const double lim = 5.551115123125783e-17 * (1.0 - zf2);
+    double z2[6], t[6];
+    sym_mul_commuting64(z, z, z2);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) t[q] = z[q];
+    double pw = zf2;
+    for (int k = 1; k <= 16 && pw > lim * (double)(2 * k + 1); ++k) {
+      double tn[6];
+      sym_mul_commuting64(t, z2, tn);
+      const double w = 1.0 / (double)(2 * k + 1);
+#pragma unroll
+      for (int q = 0; q < 6; ++q) {
+        t[q] = tn[q];
+        e[q] += w * tn[q];
+      }
+      pw *= zf2;
+    }
+  }
+  const double th = e[0] + e[3] + e[5];
+  const double t3 = th * (1.0 / 3.0);
+  h.ed[0] = e[0] - t3;
+  h.ed[1] = e[1];
+  h.ed[2] = e[2];
+  h.ed[3] = e[3] - t3;
+  h.ed[4] = e[4];
+  h.ed[5] = e[5] - t3;
+  h.theta = th;
+  h.e_n = sqrt(h.ed[0] * h.ed[0] + h.ed[3] * h.ed[3] + h.ed[5] * h.ed[5] +
+               2.0 * (h.ed[1] * h.ed[1] + h.ed[2] * h.ed[2] + h.ed[4] * h.ed[4]));
+  return h;
+}
+
+// The law's Kirchhoff stress from the Hencky tensor (FP64):
+// tau = c_vol I + 2G (1 - eL xS/e_n) e', the direction term dropped for
+// e_n <= epsilon(0.0_rp) as d_i = 0 in compute_kirchoff_stress :323-332.
+__host__ __device__ __forceinline__ void kirchhoff_hencky64(const Mat& m, const Hencky64& h,
+                                                           double xS, double (&t)[6]) {
+  const double cvol = m.K * (h.theta - 3.0 * m.alfa * m.eL * xS);
+  const double twoG = 2.0 * m.G;
+  const double coef = h.e_n > 2.220446049250313080847e-16 ? twoG * (1.0 - m.eL * xS / h.e_n) : twoG;
+  t[0] = cvol + coef * h.ed[0];
+  t[1] = coef * h.ed[1];
+  t[2] = coef * h.ed[2];
+  t[3] = cvol + coef * h.ed[3];
+  t[4] = coef * h.ed[4];
+  t[5] = cvol + coef * h.ed[5];
+}
+
 // tau = c_vol I + 2G (1 - eL xS/|e'|) e'; for |e'| below FP32 epsilon the
 // direction term is dropped, as d_i = 0 in the spectral form.
 __host__ __device__ __forceinline__ void kirchhoff_dev(const Mat& m, const Hencky32& h, double xS,

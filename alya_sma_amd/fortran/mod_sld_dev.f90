@@ -13,7 +13,7 @@ module mod_sld_dev
   integer(c_int32_t), parameter :: SLD_FIELD_U = 0, SLD_FIELD_V = 1, SLD_FIELD_A = 2, &
        SLD_FIELD_FINT = 3, SLD_FIELD_MASS = 4, SLD_FIELD_SDV = 5, SLD_FIELD_STRESS = 6, &
        SLD_FIELD_FLAGS = 7, SLD_FIELD_TIME = 8, SLD_FIELD_COUNTERS = 9, SLD_FIELD_PK1 = 10, &
-       SLD_FIELD_REACTIONS = 11
+       SLD_FIELD_REACTIONS = 11, SLD_FIELD_HEALTH = 12
   integer(c_int32_t), parameter :: SLD_SCHEME_CD = 0, SLD_SCHEME_TW = 1
 
   type, bind(C) :: sld_mesh_desc
@@ -86,6 +86,23 @@ module mod_sld_dev
        real(c_double), intent(in) :: g(3)
        integer(c_int) :: rc
      end function sld_dev_set_gravity
+
+     ! ab(2, nmat) = dampi_sld(1:2, imate), RAYALPHA and RAYBETA
+     ! (sld_reaphy.f90:1046-1049; sld_elmmat.f90:160-194)
+     function sld_dev_set_damping(dev, ab) bind(C, name="sld_dev_set_damping") result(rc)
+       import :: c_int, c_double, c_ptr
+       type(c_ptr), value :: dev
+       real(c_double), intent(in) :: ab(*)
+       integer(c_int) :: rc
+     end function sld_dev_set_damping
+
+     ! func = 0-based load function of bodyf_sld (BODYF, sld_reabcs.f90:523), -1 = none
+     function sld_dev_set_body_force(dev, func) bind(C, name="sld_dev_set_body_force") result(rc)
+       import :: c_int, c_int32_t, c_ptr
+       type(c_ptr), value :: dev
+       integer(c_int32_t), value :: func
+       integer(c_int) :: rc
+     end function sld_dev_set_body_force
 
      function sld_dev_set_energy(dev, on) bind(C, name="sld_dev_set_energy") result(rc)
        import :: c_int, c_int32_t, c_ptr

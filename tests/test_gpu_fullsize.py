@@ -85,3 +85,47 @@ def test_full_size_step_matches_oracle(case, scheme):
         print(f"  FP32 one-step stress error {l2:.2e} (relative L2)", flush=True)
         assert l2 < 1e-4
         assert np.abs(S64).max() > 0.0
+
+
+def test_surrogate_mode_on_the_cantilever_against_oracle():
+    """Surrogate mode (FP32 kinematics, trained network on single-branch
+    transforming points, exact return map elsewhere) on the 400x20x20
+    cantilever: the device runs into the transforming phase in FP64, its
+    state goes to the oracle and back to the device, and one surrogate step
+    stays within 1e-3 (relative L2 stress) of the oracle's FP64 step, with
+    accepted points present and their history within the acceptance
+    tolerance of the exact map's."""
+    from alya_sma_amd import surrogate as S
+    net = S.load_default()
+    assert net is not None, "the committed network is missing"
+    prob = problems.cantilever_problem()
+    r = prob["run"]
+    print("cantilever surrogate mode: 2000 FP64 steps, then one surrogate step", flush=True)
+    with Solidz(prob) as dev:
+        dev.set_store(True)
+        dev.step(r["dt"], nsteps=2000)
+        ptr, order = dev.coloring()
+        orc = O.StepOracle(prob, elem_order=order, color_ptr=ptr)
+        _load_state(orc, dev)
+        dev.set_surrogate(net, 1e-4)
+        dev.set_mode(2)
+        c0 = dev.counters()
+        dev.step(r["dt"], nsteps=1)
+        orc.step(r["dt"], n=1)
+        c1 = dev.counters()
+        Sd = dev.get("stress")
+        l2 = np.linalg.norm(Sd - orc.S) / np.linalg.norm(orc.S)
+        used = c1["surrogate_used"] - c0["surrogate_used"]
+        fall = c1["surrogate_fallback"] - c0["surrogate_fallback"]
+        transforming = int(np.count_nonzero(orc.flags & 3))
+        fl = dev.get("flags")
+        # accepted points on the FP64 step's branch (an FP32 loading function
+        # within rounding of a threshold may take the other branch)
+        acc = ((fl & 0x20) > 0) & ((fl & 3) == (orc.flags & 3))
+        dxs = np.abs(dev.get("sdv")[..., 0] - orc.sdv[0, ..., 0])
+        print(f"  stress rel L2 {l2:.2e}, transforming {transforming}, accepted {used}, "
+              f"fallback {fall}, max |dxS| accepted {dxs[acc].max() if acc.any() else 0:.2e}",
+              flush=True)
+        assert transforming > 0 and used > 0
+        assert l2 < 1e-3
+        assert dxs[acc].max() <= 1e-4 + 1e-6
