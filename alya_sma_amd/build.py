@@ -1,4 +1,5 @@
-"""In-tree build of the HIP library (gfx950) and the Fortran host layer.
+"""In-tree build of the HIP library (gfx950) and the Fortran host layer
+(the test-side checkers under oracle/ build from their own Makefile).
 
     python -m alya_sma_amd.build
 
@@ -66,18 +67,9 @@ def build_fortran():
     return flib
 
 
-def build_oracle():
-    """Checker builds: the C oracle and, where the reference sources are
-    present, the reference harnesses (oracle/_ref)."""
-    odir = os.path.join(ROOT, "oracle")
-    _run(["make", "-s", "-C", odir])
-    _run(["make", "-s", "-C", odir, "ref"])
-
-
 def build():
     build_hip()
     build_fortran()
-    build_oracle()
 
 
 if __name__ == "__main__":
