@@ -233,10 +233,13 @@ def cpu_baseline(prob, state, budget_s):
         orc.u[:], orc.v[:], orc.a[:] = state["u"], state["v"], state["a"]
         orc.sdv[0], orc.sdv[1] = state["sdv"], state["sdv"]
         orc.s.t = float(state["time"][0])
-        orc.step(dt, scheme=0)                    # first step outside the sample (page-in)
+        if orc.step(dt, scheme=0):                # first step outside the sample (page-in)
+            raise RuntimeError(f"CPU baseline: the oracle stopped its run {orc.halt}")
         n, t0 = 0, time.perf_counter()
         while True:
-            orc.step(dt, scheme=0)
+            if orc.step(dt, scheme=0):
+                # a stopped run returns at once: the sample would not be a step
+                raise RuntimeError(f"CPU baseline: the oracle stopped its run {orc.halt}")
             n += 1
             el = time.perf_counter() - t0
             if el >= budget_s or n >= 400:

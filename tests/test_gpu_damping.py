@@ -1,10 +1,12 @@
 """Rayleigh damping (sld_elmmat.f90:160-194) and the body force
 (sld_elmmat.f90:156-157, sld_funbou.f90 itask 3) on the GPU against the
 oracle run in the same colour order.  With alpha = 1e5 (RAYALPHA of the
-reference's cantilever deck, cantiliver_nitinol.sld.dat:28) the
-load-unload cycle of a cantilever runs free -- no restart from the oracle --
-and stays within 1e-10 of it; the beta term, the reactions (which include
-the damping force) and the FP32 mode are checked on the same mesh."""
+reference's cantilever deck, cantiliver_nitinol.sld.dat:28) a cantilever's
+whole load-unload cycle agrees with the oracle to 1e-10 when restarted from
+its state every 25 steps, and as one free run stays within the oracle's
+own sensitivity to a rounding-size perturbation; the beta term, the
+reactions (which include the damping force) and the FP32 mode are checked
+on the same mesh."""
 import numpy as np
 import pytest
 
@@ -44,22 +46,44 @@ def _errors(dev, orc):
                 S=np.max(np.abs(dev.get("stress") - orc.S)) / S_scale)
 
 
+def _sync_from_oracle(dev, orc):
+    dev.set("u", orc.u)
+    dev.set("v", orc.v)
+    dev.set("a", orc.a)
+    dev.set("sdv", orc.sdv[1])
+    dev.set("time", np.array([orc.t]))
+
+
 @pytest.mark.parametrize("scheme", [0, 1])
-def test_damped_cantilever_free_run_matches_oracle(scheme):
+def test_damped_cantilever_matches_oracle(scheme):
+    """The whole load-unload cycle with alpha = 1e5: the first 250 steps as
+    one window, then the device restarted from the oracle's state every 25
+    steps (the restart path; over longer windows the run's own sensitivity
+    to rounding, measured by the free-run test below, exceeds 1e-10 in the
+    transforming phase).  Every field, the history and the branch flags
+    agree to 1e-10 at every window's end; the reactions include the damping
+    force.  (The first window is long because in the first tens of steps
+    the strain is ~1e-9 and F formed from 30 mm coordinates keeps only ~1e-7
+    of its digits, in the reference as here: displacements that small agree
+    to that conditioning, not to 1e-10 of their maximum.)"""
     prob = _cantilever()
     r = prob["run"]
     dev, orc = _pair(prob)
     transforming = 0
     with dev:
-        for chunk in range(8):
-            dev.step(r["dt"], nsteps=250, scheme=scheme, phi=r["phi"])
-            orc.step(r["dt"], scheme=scheme, phi=r["phi"], n=250)
+        for chunk in range(71):
+            if chunk:
+                _sync_from_oracle(dev, orc)
+            n = 250 if chunk == 0 else 25
+            dev.step(r["dt"], nsteps=n, scheme=scheme, phi=r["phi"])
+            orc.step(r["dt"], scheme=scheme, phi=r["phi"], n=n)
             errs = _errors(dev, orc)
             fl = dev.get("flags")
             transforming = max(transforming, int(np.count_nonzero(fl & 3)))
-            print(f"scheme {scheme} step {250 * (chunk + 1)}: "
-                  f"{ {k: f'{x:.1e}' for k, x in errs.items()} }, transforming {transforming}",
-                  flush=True)
+            if chunk % 10 == 0:
+                print(f"scheme {scheme} step {250 + 25 * chunk}: "
+                      f"{ {k: f'{x:.1e}' for k, x in errs.items()} }, transforming {transforming}",
+                      flush=True)
             for k, x in errs.items():
                 assert x < TOL, (chunk, k, x)
             np.testing.assert_array_equal(fl & 3, orc.flags & 3)
@@ -74,24 +98,94 @@ def test_damped_cantilever_free_run_matches_oracle(scheme):
         assert _rel(R[fixed], orc.rhs[fixed]) < TOL
 
 
+@pytest.mark.parametrize("scheme", [0, 1])
+def test_damped_cantilever_free_run_within_rounding_sensitivity(scheme):
+    """The same cycle as one free run (no restart).  The reference's own run
+    is reproducible only to its sensitivity to rounding: an oracle run whose
+    displacement is scaled by 1 + 1e-15 at step 20 drifts from the
+    unperturbed oracle by up to ~1e-4 (stress) by the cycle's end even with
+    this damping, as transformation starts a step earlier or later at points
+    whose loading function lies within rounding of a threshold.  So the
+    device's free run must agree with the oracle to 1e-10 while the oracle's
+    own drift stays below 1e-11, and afterwards within 30 times the largest
+    drift the perturbed oracle has shown so far."""
+    prob = _cantilever()
+    r = prob["run"]
+    dev, orc = _pair(prob)
+    ptr, order = dev.coloring()
+    pert = O.StepOracle(prob, elem_order=order, color_ptr=ptr)
+    with dev:
+        drift_max = {}
+        for chunk in range(8):
+            n = 250
+            if chunk == 0:
+                dev.step(r["dt"], nsteps=20, scheme=scheme, phi=r["phi"])
+                orc.step(r["dt"], scheme=scheme, phi=r["phi"], n=20)
+                pert.step(r["dt"], scheme=scheme, phi=r["phi"], n=20)
+                pert.u[:] = pert.u * (1.0 + 1e-15)
+                n = 230
+            dev.step(r["dt"], nsteps=n, scheme=scheme, phi=r["phi"])
+            orc.step(r["dt"], scheme=scheme, phi=r["phi"], n=n)
+            pert.step(r["dt"], scheme=scheme, phi=r["phi"], n=n)
+            errs = _errors(dev, orc)
+            drift = _errors(_OracleView(pert), orc)
+            for k, x in drift.items():
+                drift_max[k] = max(drift_max.get(k, 0.0), x)
+            print(f"scheme {scheme} step {250 * (chunk + 1)}: device "
+                  f"{ {k: f'{x:.1e}' for k, x in errs.items()} }, perturbed oracle "
+                  f"{ {k: f'{x:.1e}' for k, x in drift.items()} }", flush=True)
+            for k, x in errs.items():
+                if drift_max[k] < 1e-11:
+                    assert x < TOL, (chunk, k, x)
+                else:
+                    assert x < max(TOL, 30.0 * drift_max[k]), (chunk, k, x, drift_max[k])
+
+
+class _OracleView:
+    """The device-side field names over an oracle's state."""
+
+    def __init__(self, o):
+        self.o = o
+
+    def get(self, f):
+        return {"u": self.o.u, "v": self.o.v, "a": self.o.a, "fint": self.o.fint,
+                "stress": self.o.S}[f]
+
+
 def test_damping_takes_energy_out():
     """The free vibration after the load cycle decays with damping and
-    does not without it (same run, same state at the cycle's end)."""
+    does not without it.  Kinetic energy of the free dofs only: the
+    integrator keeps integrating the reaction acceleration at prescribed
+    dofs (their velocity is not a motion of the body), and the largest
+    value over 400 steps (about one period of the first axial mode) so that
+    the exchange with strain energy does not decide the comparison."""
     prob = _cantilever(nsteps=600)
     r = prob["run"]
     und = dict(prob)
     und.pop("damping")
+    free = np.stack([(prob["fixmask"] >> k) & 1 for k in range(3)], axis=1) == 0
     ke = {}
     for name, p in (("damped", prob), ("undamped", und)):
         with Solidz(p) as dev:
-            dev.step(r["dt"], nsteps=600)
             m = dev.get("mass")
-            k0 = 0.5 * np.sum(m[:, None] * dev.get("v") ** 2)
-            dev.step(r["dt"], nsteps=1200)
-            ke[name] = (k0, 0.5 * np.sum(m[:, None] * dev.get("v") ** 2))
-    print("kinetic energy before / after 1200 free steps:", ke, flush=True)
-    assert ke["damped"][1] < 0.2 * ke["damped"][0]
+
+            def window_max():
+                best = 0.0
+                for _ in range(8):
+                    dev.step(r["dt"], nsteps=50)
+                    v = np.where(free, dev.get("v"), 0.0)
+                    best = max(best, 0.5 * float(np.sum(m[:, None] * v * v)))
+                return best
+
+            dev.step(r["dt"], nsteps=600)       # the load cycle: the load is zero from here on
+            k0 = window_max()
+            dev.step(r["dt"], nsteps=800)
+            ke[name] = (k0, window_max())
+    print("free-dof kinetic energy (largest over 400 steps) after the load cycle and "
+          "1200 steps later:", ke, flush=True)
+    assert ke["damped"][1] < 0.1 * ke["damped"][0]
     assert ke["undamped"][1] > 0.2 * ke["undamped"][0]
+    assert ke["damped"][1] < 0.05 * ke["undamped"][1]
 
 
 @pytest.mark.parametrize("mode", [0, 1])

@@ -31,9 +31,16 @@ def _load_state(orc, dev):
     orc.s.t = float(dev.get("time")[0])
 
 
+# Device steps before the comparison.  The state handed to the oracle must
+# lie where the reference's law is defined: ahead of a loading wave the
+# displacements fall off to ~1e-300, and for 1e-162 < |F - I| < 1e-77 the
+# reference's eigen-solver underflows and its law returns NaN (the oracle
+# restates that bit for bit; tests/test_oracle_law.py).  The longest
+# specimen is 414 elements, which the loading wave (half an element per
+# step at this time step) has crossed by step 828.
 CASES = {
     "cantilever": (lambda: problems.cantilever_problem(), 1500),
-    "specimens": (lambda: problems.batched_specimens(), 300),
+    "specimens": (lambda: problems.batched_specimens(), 1000),
     "cylinder": (lambda: problems.cylinder_problem(), 3000),
 }
 
@@ -67,6 +74,15 @@ def test_full_size_step_matches_oracle(case, scheme):
         transforming = int(np.count_nonzero(orc.flags & 3))
         print(f"  errors {({k: f'{x:.1e}' for k, x in errs.items()})}, max |u| {moving:.3e}, "
               f"transforming points {transforming}", flush=True)
+        if not all(x < TOL for x in errs.values()):
+            du = np.abs(u - orc.u).max(axis=1)
+            p = int(np.argmax(du))
+            print(f"  device health {dev.health()}, oracle halt {orc.halt}, time device "
+                  f"{dev.get('time')[0]!r} oracle {orc.t!r}; largest u difference at node {p}: "
+                  f"device {u[p]} oracle {orc.u[p]}, fixmask {prob['fixmask'][p]}, "
+                  f"non-finite oracle v {int(np.count_nonzero(~np.isfinite(orc.v)))}, "
+                  f"S {int(np.count_nonzero(~np.isfinite(orc.S)))}, device v "
+                  f"{int(np.count_nonzero(~np.isfinite(dev.get('v'))))}", flush=True)
         assert moving > 0.0
         for k, x in errs.items():
             assert x < TOL, (case, k, x)

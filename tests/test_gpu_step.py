@@ -489,3 +489,39 @@ def test_pk1_and_reactions_fields():
         x = X[:, 0]
         lo, hi = R[np.abs(x) < 1e-9, 0].sum(), R[np.abs(x - x.max()) < 1e-9, 0].sum()
         assert lo * hi < 0.0
+
+
+@pytest.mark.parametrize("eps", [1e-120, 1e-90, 1e-8])
+def test_fp64_step_stress_of_tiny_strains(eps):
+    """A uniform uniaxial strain eps on a free bar, one FP64 step with zero
+    velocity and acceleration (the predictor leaves u as set): every Gauss
+    point carries the linear-elastic stress S_xx = (lambda + 2 mu) eps,
+    S_yy = S_zz = lambda eps to 1e-10.  At 1e-120 and 1e-90 the reference's
+    law returns NaN (its eigen-solver underflows, tests/test_oracle_law.py);
+    the device's FP64 element step forms the Hencky strain by its series
+    and stays defined there."""
+    prob = problems.bar_problem(nsteps=100)
+    prob["fixmask"] = np.zeros_like(prob["fixmask"])
+    prob["func_of_node"] = np.full_like(prob["func_of_node"], -1)
+    E, nu = prob["props"][0, 0], prob["props"][0, 1]
+    K, G = E / (3.0 * (1.0 - 2.0 * nu)), E / (2.0 * (1.0 + nu))
+    lam = K - 2.0 * G / 3.0
+    with Solidz(prob) as dev:
+        dev.set_store(True)
+        u = np.zeros((dev.npoin, 3))
+        u[:, 0] = eps * prob["coord"][:, 0]
+        dev.set("u", u)
+        dev.step(prob["run"]["dt"], nsteps=1)
+        assert np.array_equal(dev.get("u"), u)
+        S = dev.get("stress").reshape(-1, 9)
+        fl = dev.get("flags")
+    sxx = (lam + 2.0 * G) * eps
+    print(f"eps {eps:.0e}: S_xx {S[:, 0].min():.6e}..{S[:, 0].max():.6e} expected {sxx:.6e}",
+          flush=True)
+    assert np.isfinite(S).all()
+    assert np.max(np.abs(S[:, 0] - sxx)) < 1e-10 * sxx
+    for k in (4, 8):
+        assert np.max(np.abs(S[:, k] - lam * eps)) < 1e-10 * sxx
+    for k in (1, 2, 3, 5, 6, 7):
+        assert np.max(np.abs(S[:, k])) < 1e-10 * sxx
+    assert not np.any(fl & 3)
