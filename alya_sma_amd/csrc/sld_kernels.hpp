@@ -746,7 +746,7 @@ __device__ __forceinline__ void count_flags(unsigned long long* ctr, uint32_t fl
 // GATHER = false: read-modify-write of f_int (the launch covers one colour);
 // true: per-corner forces to efor (the launch may cover every element).
 #ifndef SLD_WAVES_F64
-#define SLD_WAVES_F64 2
+#define SLD_WAVES_F64 3
 #endif
 #ifndef SLD_WAVES_F32
 #define SLD_WAVES_F32 4
@@ -907,6 +907,21 @@ __global__ void __launch_bounds__(EPB * 8, (ElemOccupancy<R, LAWS, DAMP>::waves)
       detj = sma::inv3<R>(jc, jci);
     }
 
+    if constexpr (sizeof(R) == 8) {
+      // FP64: j^-T and detJ wait in LDS through the constitutive update (the
+      // Q rows and this point's U slot are not written before it ends), so
+      // that the law runs with 20 fewer live VGPRs
+#pragma unroll
+      for (int q = 0; q < 9; ++q) s_q[le][q][g] = jci[q];
+      s_u4[le][g][0] = detJ;
+    }
+    // det F = detj / detJ > 0 (false for NaN); det F <= 0 stops the run
+    // (NaN is left to the residual check)
+    const bool detf_pos = detj > R(0) ? detJ > R(0) : (detj < R(0) && detJ < R(0));
+    if constexpr (SUR != 1) {
+      if (!detf_pos && ((detj <= R(0) && detJ > R(0)) || (detj >= R(0) && detJ < R(0))))
+        report_detf(A.health, e, (double)detj / (double)detJ);
+    }
     const int mat = A.mat_of_elem ? A.mat_of_elem[e] : 0;
     const double* pr = A.props + 18 * mat;
     const sma::Mat m = sma::make_mat_pre(pr, A.thresholds + 4 * mat, A.kg[2 * (int64_t)e],
@@ -938,23 +953,14 @@ __global__ void __launch_bounds__(EPB * 8, (ElemOccupancy<R, LAWS, DAMP>::waves)
       for (int q = 0; q < 9; ++q) Fm[q] = H[q] + ((q % 4 == 0) ? R(1) : R(0));
       if constexpr (SUR == 1) A.sur_status[gp] = 0;
     } else if constexpr (sizeof(R) == 8) {
-      double F[9];
-#pragma unroll
-      for (int q = 0; q < 9; ++q) F[q] = H[q] + ((q % 4 == 0) ? 1.0 : 0.0);
-      // the Hencky tensor by series (theta, e' and e_n, no eigenvectors);
-      // large stretches take the reference's spectral path
+      // the Hencky tensor by series (theta, e' and e_n, no eigenvectors;
+      // large stretches first halve log b by symmetric square roots)
       const sma::Hencky64 hk = sma::hencky_fp64(H);
-      if (hk.ok) {
-        rm = sma::return_map(m, hk.theta, hk.e_n, xS0, FAS0, FSA0);
-        sma::kirchhoff_hencky64(m, hk, rm.xS, tau);
-      } else {
-        sma::Spectral<double> sp = sma::spectral_fp64(F);
-        rm = sma::return_map(m, sp.theta, sp.e_n, xS0, FAS0, FSA0);
-        sma::kirchhoff<double>(m, sp, rm.xS, tau);
-        if (!sp.eig_ok) rm.flags |= sma::FLAG_EIGEN_FAIL;
-      }
+      rm = sma::return_map(m, hk.theta, hk.e_n, xS0, FAS0, FSA0);
+      sma::kirchhoff_hencky64(m, hk, rm.xS, tau);
+      if (!hk.ok) rm.flags |= sma::FLAG_EIGEN_FAIL;
 #pragma unroll
-      for (int q = 0; q < 9; ++q) Fm[q] = F[q];
+      for (int q = 0; q < 9; ++q) Fm[q] = H[q] + ((q % 4 == 0) ? 1.0 : 0.0);
     } else {
       float Hf[9];
 #pragma unroll
@@ -1039,23 +1045,26 @@ __global__ void __launch_bounds__(EPB * 8, (ElemOccupancy<R, LAWS, DAMP>::waves)
       A.sdv_k[2][gp] = rm.F_SA;
     }
     fl = rm.flags;
+    if (!detf_pos) fl |= sma::FLAG_DETF_NONPOS;
 
     // Q = detJ * tau * j^-T
-    // det F = detj / detJ > 0 (false for NaN)
-    const bool detf_pos = detj > R(0) ? detJ > R(0) : (detj < R(0) && detJ < R(0));
-    if (!detf_pos) {
-      fl |= sma::FLAG_DETF_NONPOS;
-      // det F <= 0 stops the run (NaN is left to the residual check)
-      if ((detj <= R(0) && detJ > R(0)) || (detj >= R(0) && detJ < R(0)))
-        report_detf(A.health, e, (double)detj / (double)detJ);
-    }
     const R T[9] = {tau[0], tau[1], tau[2], tau[1], tau[3], tau[4], tau[2], tau[4], tau[5]};
+    R jq[9], dJ;
+    if constexpr (sizeof(R) == 8) {
+#pragma unroll
+      for (int q = 0; q < 9; ++q) jq[q] = s_q[le][q][g];
+      dJ = s_u4[le][g][0];
+    } else {
+#pragma unroll
+      for (int q = 0; q < 9; ++q) jq[q] = jci[q];
+      dJ = detJ;
+    }
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
       for (int k = 0; k < 3; ++k)
         s_q[le][3 * i + k][g] =
-            detJ * (T[i] * jci[k] + T[i + 3] * jci[k + 3] + T[i + 6] * jci[k + 6]);
+            dJ * (T[i] * jq[k] + T[i + 3] * jq[k + 3] + T[i + 6] * jq[k + 6]);
 
     if constexpr (DAMP) {
       // Rayleigh damping at Gauss point g.  J and the velocity's natural
@@ -1080,7 +1089,7 @@ __global__ void __launch_bounds__(EPB * 8, (ElemOccupancy<R, LAWS, DAMP>::waves)
 #pragma unroll
         for (int h = 0; h < 8; ++h) gv += wv[h] * s_v[le][h][6 + i];
         gv += (R)s_v0[le][i];
-        s_q[le][9 + i][g] = arho * detJ * gv;
+        s_q[le][9 + i][g] = arho * dJ * gv;
       }
       R z[3] = {R(0), R(0), R(0)};
       if (beta > 0.0) {
@@ -1111,14 +1120,14 @@ __global__ void __launch_bounds__(EPB * 8, (ElemOccupancy<R, LAWS, DAMP>::waves)
           for (int c = 0; c < 3; ++c) {
             L[r + 3 * c] = Gv[r] * Jid[3 * c] + Gv[r + 3] * Jid[1 + 3 * c] + Gv[r + 6] * Jid[2 + 3 * c];
             // F^-1 = J j^-1 (j = F J)
-            Fi[r + 3 * c] = Jd[r] * jci[3 * c] + Jd[r + 3] * jci[1 + 3 * c] + Jd[r + 6] * jci[2 + 3 * c];
+            Fi[r + 3 * c] = Jd[r] * jq[3 * c] + Jd[r + 3] * jq[1 + 3 * c] + Jd[r + 6] * jq[2 + 3 * c];
           }
         sma::pk2<R>(Fi, tau, S);
         R w[3];
 #pragma unroll
         for (int j = 0; j < 3; ++j)
           w[j] = L[3 * j] * S[j] + L[1 + 3 * j] * S[j + 3] + L[2 + 3 * j] * S[j + 6];
-        const R bd = (R)beta * detJ;
+        const R bd = (R)beta * dJ;
 #pragma unroll
         for (int k = 0; k < 3; ++k)
           z[k] = bd * (Jid[k] * w[0] + Jid[k + 3] * w[1] + Jid[k + 6] * w[2]);

@@ -440,12 +440,13 @@ __host__ __device__ __forceinline__ Hencky32 hencky_fp32(const float (&H)[9]) {
 // and the stress consume (theta = log J, |e'| = e_n of compute_kirchoff_stress
 // :313-322), so no eigenvectors, square roots of eigenvalues, logarithms or
 // powers are needed; the terms run until the remainder is below 2^-54 of
-// |E| (per lane).  Stretches with |Z|_F > 0.3 take the reference's spectral
-// path instead (ok = false).
+// |Z| (per lane).  Stretches with |Z|_F > 0.6 (principal stretches beyond
+// about 1/2..2) first halve the logarithm by symmetric square roots,
+// log b = 2^k log b^(1/2^k), until |Z|_F <= 0.6.
 struct Hencky64 {
   double theta, e_n;
   double ed[6];        // e' (00, 01, 02, 11, 12, 22)
-  bool ok;
+  bool ok;             // false: no finite strain (a non-finite or non-positive b)
 };
 
 __host__ __device__ __forceinline__ void sym_mul_commuting64(const double (&a)[6],
@@ -458,6 +459,52 @@ __host__ __device__ __forceinline__ void sym_mul_commuting64(const double (&a)[6
   c[5] = a[2] * b[2] + a[4] * b[4] + a[5] * b[5];
 }
 
+// Inverse of a symmetric 3x3 matrix (00, 01, 02, 11, 12, 22) by its adjugate.
+__host__ __device__ __forceinline__ void sym_inv64(const double (&m)[6], double (&r)[6]) {
+  r[0] = m[3] * m[5] - m[4] * m[4];
+  r[1] = m[2] * m[4] - m[1] * m[5];
+  r[2] = m[1] * m[4] - m[2] * m[3];
+  r[3] = m[0] * m[5] - m[2] * m[2];
+  r[4] = m[1] * m[2] - m[0] * m[4];
+  r[5] = m[0] * m[3] - m[1] * m[1];
+  const double d = 1.0 / (m[0] * r[0] + m[1] * r[1] + m[2] * r[2]);
+#pragma unroll
+  for (int q = 0; q < 6; ++q) r[q] *= d;
+}
+
+// Z = (b - I)(b + I)^-1 from c = b - I; returns |Z|_F^2.
+__host__ __device__ __forceinline__ double cayley64(const double (&c)[6], double (&z)[6]) {
+  const double m[6] = {2.0 + c[0], c[1], c[2], 2.0 + c[3], c[4], 2.0 + c[5]};
+  double a[6];
+  sym_inv64(m, a);
+  sym_mul_commuting64(c, a, z);
+  return z[0] * z[0] + z[3] * z[3] + z[5] * z[5] + 2.0 * (z[1] * z[1] + z[2] * z[2] + z[4] * z[4]);
+}
+
+// Square root of a symmetric positive definite 3x3 matrix by the
+// Denman-Beavers iteration Y <- (Y + W^-1)/2, W <- (W + Y^-1)/2 (Y -> b^1/2,
+// W -> b^-1/2), run to convergence (quadratic; ~6-10 iterations for the
+// stretches a solid reaches).
+__host__ __device__ __forceinline__ void sym_sqrt64(double (&y)[6]) {
+  double w[6] = {1.0, 0.0, 0.0, 1.0, 0.0, 1.0};
+  for (int it = 0; it < 40; ++it) {
+    double yi[6], wi[6];
+    sym_inv64(y, yi);
+    sym_inv64(w, wi);
+    double d2 = 0.0, n2 = 0.0;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      const double yn = 0.5 * (y[q] + wi[q]);
+      const double f = (q == 1 || q == 2 || q == 4) ? 2.0 : 1.0;
+      d2 += f * (yn - y[q]) * (yn - y[q]);
+      n2 += f * yn * yn;
+      y[q] = yn;
+      w[q] = 0.5 * (w[q] + yi[q]);
+    }
+    if (!(d2 > 1e-32 * n2)) break;   // converged (or not finite)
+  }
+}
+
 __host__ __device__ __forceinline__ Hencky64 hencky_fp64(const double (&H)[9]) {
   Hencky64 h;
   double c[6];
@@ -467,22 +514,20 @@ __host__ __device__ __forceinline__ Hencky64 hencky_fp64(const double (&H)[9]) {
   c[1] = H[3] + H[1] + (H[0] * H[1] + H[3] * H[4] + H[6] * H[7]);
   c[2] = H[6] + H[2] + (H[0] * H[2] + H[3] * H[5] + H[6] * H[8]);
   c[4] = H[7] + H[5] + (H[1] * H[2] + H[4] * H[5] + H[7] * H[8]);
-  const double m00 = 2.0 + c[0], m11 = 2.0 + c[3], m22 = 2.0 + c[5];
-  double a[6];
-  a[0] = m11 * m22 - c[4] * c[4];
-  a[1] = c[2] * c[4] - c[1] * m22;
-  a[2] = c[1] * c[4] - c[2] * m11;
-  a[3] = m00 * m22 - c[2] * c[2];
-  a[4] = c[1] * c[2] - m00 * c[4];
-  a[5] = m00 * m11 - c[1] * c[1];
-  const double r = 1.0 / (m00 * a[0] + c[1] * a[1] + c[2] * a[2]);
-#pragma unroll
-  for (int q = 0; q < 6; ++q) a[q] *= r;
   double z[6];
-  sym_mul_commuting64(c, a, z);
-  const double zf2 = z[0] * z[0] + z[3] * z[3] + z[5] * z[5] +
-                     2.0 * (z[1] * z[1] + z[2] * z[2] + z[4] * z[4]);
-  h.ok = zf2 <= 0.09;
+  double zf2 = cayley64(c, z);
+  // large stretches: b <- b^(1/2) until |Z|_F <= 0.6 (each halves log b)
+  int halvings = 0;
+  if (zf2 > 0.36) {
+    double b[6] = {1.0 + c[0], c[1], c[2], 1.0 + c[3], c[4], 1.0 + c[5]};
+    while (zf2 > 0.36 && halvings < 10) {
+      sym_sqrt64(b);
+      ++halvings;
+      const double cb[6] = {b[0] - 1.0, b[1], b[2], b[3] - 1.0, b[4], b[5] - 1.0};
+      zf2 = cayley64(cb, z);
+    }
+  }
+  h.ok = zf2 <= 0.36;
   double e[6];
 #pragma unroll
   for (int q = 0; q < 6; ++q) e[q] = z[q];
@@ -495,7 +540,7 @@ __host__ __device__ __forceinline__ Hencky64 hencky_fp64(const double (&H)[9]) {
 #pragma unroll
     for (int q = 0; q < 6; ++q) t[q] = z[q];
     double pw = zf2;
-    for (int k = 1; k <= 16 && pw > lim * (double)(2 * k + 1); ++k) {
+    for (int k = 1; k <= 40 && pw > lim * (double)(2 * k + 1); ++k) {
       double tn[6];
       sym_mul_commuting64(t, z2, tn);
       const double w = 1.0 / (double)(2 * k + 1);
@@ -506,6 +551,11 @@ __host__ __device__ __forceinline__ Hencky64 hencky_fp64(const double (&H)[9]) {
       }
       pw *= zf2;
     }
+  }
+  if (halvings) {
+    const double sc = (double)(1 << halvings);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) e[q] *= sc;
   }
   const double th = e[0] + e[3] + e[5];
   const double t3 = th * (1.0 / 3.0);
