@@ -9,6 +9,7 @@
 #include <cstdint>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -1721,3 +1722,99 @@ int sma160_eval(int64_t n, int32_t n_mat, const double* props, const int32_t* ma
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// sma160_eval without device allocation.  sma160_eval stages the caller's
+// host arrays through device buffers it allocates and frees on every call;
+// sma160_eval_noalloc stages them through `scratch`, a fixed array in the
+// library's device memory.  Being a file-scope __device__ object, every GPU
+// the library runs on holds its own copy (at its own address) until the
+// process ends.  Host code never uses the array's name as an address: the
+// first call on a device looks that device's copy up with
+// hipGetSymbolAddress, with the device current, and keeps the pointer per
+// device.  Every call on a GPU shares that GPU's copy and leaves its
+// contents behind (the law kernel reads only what the call copied in, so
+// nothing needs clearing); calls that overlap on one GPU race on it.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int64_t kNoallocMaxGp = SMA160_NOALLOC_MAX_GP;
+constexpr int32_t kNoallocMaxMat = SMA160_NOALLOC_MAX_MAT;
+constexpr int kNoallocMaxDevices = 64;
+// staging layout in doubles: props [max mat][18], then per point E (1), F (9),
+// history in (3), history out (3), S (9), and the material index and flags as
+// two 32-bit words (one double's room)
+constexpr int64_t kNoallocPropsDoubles = (int64_t)kNoallocMaxMat * 18;
+constexpr int64_t kNoallocPerGpDoubles = 1 + 9 + 3 + 3 + 9 + 1;
+
+__device__ __align__(256) double scratch[kNoallocPropsDoubles + kNoallocPerGpDoubles * kNoallocMaxGp];
+
+std::mutex g_noalloc_mu;
+double* g_noalloc_ptr[kNoallocMaxDevices] = {};
+
+// Device address of the current device's copy of `scratch`.
+int noalloc_scratch(double** out) {
+  int dev = -1;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNoallocMaxDevices)
+    return fail(SMA_ERR_HIP, "sma160_eval_noalloc: device index out of range");
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  if (!g_noalloc_ptr[dev]) {
+    void* p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(SMA_ERR_HIP, "sma160_eval_noalloc: no device address for the staging array");
+    g_noalloc_ptr[dev] = (double*)p;
+  }
+  *out = g_noalloc_ptr[dev];
+  return SMA_OK;
+}
+
+}  // namespace
+
+extern "C" int sma160_eval_noalloc(int64_t n, int32_t n_mat, const double* props,
+                                   const int32_t* mat_of_gp, const double* E_of_gp,
+                                   const double* F, const double* T, const double* sdv_old,
+                                   double* sdv_new, double* S, uint32_t* flags,
+                                   int32_t precision) {
+  (void)T;
+  if (n < 0 || n_mat <= 0 || (n > 0 && (!props || !F || !sdv_old || !sdv_new || !S)))
+    return fail(SMA_ERR_ARG, "sma160_eval_noalloc: bad arguments");
+  if (precision != 0 && precision != 1) return fail(SMA_ERR_ARG, "precision must be 0 or 1");
+  if (n > kNoallocMaxGp)
+    return fail(SMA_ERR_ARG, "sma160_eval_noalloc: n_gp = " + std::to_string(n) +
+                                 " exceeds the maximum " + std::to_string(kNoallocMaxGp) +
+                                 " (SMA160_NOALLOC_MAX_GP)");
+  if (n_mat > kNoallocMaxMat)
+    return fail(SMA_ERR_ARG, "sma160_eval_noalloc: n_mat = " + std::to_string(n_mat) +
+                                 " exceeds the maximum " + std::to_string(kNoallocMaxMat) +
+                                 " (SMA160_NOALLOC_MAX_MAT)");
+  if (n == 0) return SMA_OK;
+  if (mat_of_gp)
+    for (int64_t i = 0; i < n; ++i)
+      if (mat_of_gp[i] < 0 || mat_of_gp[i] >= n_mat)
+        return fail(SMA_ERR_ARG, "sma160_eval_noalloc: material index out of range");
+  double* base = nullptr;
+  if (int rc = noalloc_scratch(&base)) return rc;
+  // this call's arrays, sized by n (the kernel's SoA stride is n)
+  double* d_props = base;
+  double* d_E = d_props + kNoallocPropsDoubles;
+  double* d_F = d_E + n;
+  double* d_sold = d_F + 9 * n;
+  double* d_snew = d_sold + 3 * n;
+  double* d_S = d_snew + 3 * n;
+  int32_t* d_mat = (int32_t*)(d_S + 9 * n);
+  uint32_t* d_flags = (uint32_t*)(d_mat + n);
+  HIPCHK(hipMemcpy(d_props, props, (size_t)n_mat * 18 * sizeof(double), hipMemcpyHostToDevice));
+  if (E_of_gp) HIPCHK(hipMemcpy(d_E, E_of_gp, (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+  if (mat_of_gp) HIPCHK(hipMemcpy(d_mat, mat_of_gp, (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(d_F, F, (size_t)9 * n * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(d_sold, sdv_old, (size_t)3 * n * sizeof(double), hipMemcpyHostToDevice));
+  // the null stream, as sma160_eval
+  if (int rc = law_launch(n, n_mat, d_props, mat_of_gp ? d_mat : nullptr, E_of_gp ? d_E : nullptr,
+                          d_F, d_sold, d_snew, d_S, flags ? d_flags : nullptr, precision, nullptr))
+    return rc;
+  HIPCHK(hipMemcpy(sdv_new, d_snew, (size_t)3 * n * sizeof(double), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(S, d_S, (size_t)9 * n * sizeof(double), hipMemcpyDeviceToHost));
+  if (flags) HIPCHK(hipMemcpy(flags, d_flags, (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  return SMA_OK;
+}

@@ -67,6 +67,22 @@ int sma160_eval_device(int64_t n_gp, int32_t n_mat, const double *props,
                        double *sdv_new, double *S, uint32_t *flags,
                        int32_t precision, void *stream);
 
+/* Same arguments and results as sma160_eval (host pointers, synchronous,
+ * null stream), with no device allocation: the arrays are staged through a
+ * fixed array in the library's device memory (3.4 MB on each GPU it runs
+ * on, held until the process ends) instead of buffers allocated and freed
+ * per call.  No counterpart in the reference: it serves callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  Rejects n_gp > SMA160_NOALLOC_MAX_GP
+ * and n_mat > SMA160_NOALLOC_MAX_MAT with SMA_ERR_ARG.  Calls that overlap
+ * on one GPU (from two host threads) race on the shared staging array. */
+#define SMA160_NOALLOC_MAX_GP  16384
+#define SMA160_NOALLOC_MAX_MAT 64
+int sma160_eval_noalloc(int64_t n_gp, int32_t n_mat, const double *props,
+                        const int32_t *mat_of_gp, const double *E_of_gp,
+                        const double *F, const double *T, const double *sdv_old,
+                        double *sdv_new, double *S, uint32_t *flags, int32_t precision);
+
 /* ------------------------------------------------------------------------
  * Explicit step on the device.
  * Replaces the per-step call chain sld_explicit / sld_explicit_tw
