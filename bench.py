@@ -106,9 +106,9 @@ def kernel_name(mode, cfg):
     r = "double" if mode == "fp64" else "float"
     sur = 2 if mode == "surrogate" else 0
     ga = "true" if cfg["assembly"] == "gather" else "false"
-    # the last template argument: 0 = each element's own law (2 = the
-    # neo-Hookean-only launch that follows it on a law-grouped mesh)
-    return f"sld_element<{r}, {cfg['elems_per_block']}, false, {sur}, {ga}, 0>"
+    # the law range: 0 = each element's own law (2 = the neo-Hookean-only
+    # launch that follows it on a law-grouped mesh); no Rayleigh damping
+    return f"sld_element<{r}, {cfg['elems_per_block']}, false, {sur}, {ga}, 0, false>"
 
 
 def _sma_mask(prob):
@@ -171,7 +171,7 @@ def measured_traffic(workload, kernel):
     kernel): on a law-grouped mesh the phase is two launches, `kernel` over
     the SMA range (last template argument 0) and its neo-Hookean-only
     variant (2), whose bytes add.  None when the pair was not measured."""
-    nh = kernel[:-2] + "2>"
+    nh = kernel.replace(", 0, false>", ", 2, false>")
     found = {}
     try:
         with open(TRAFFIC_FILE) as f:

@@ -55,7 +55,7 @@ def test_algorithmic_flops_mix():
 def test_kernel_names_and_committed_traffic():
     cfg = {"assembly": "gather", "elems_per_block": 8}
     k32 = bench.kernel_name("fp32", cfg)
-    assert k32 == "sld_element<float, 8, false, 0, true, 0>"
+    assert k32 == "sld_element<float, 8, false, 0, true, 0, false>"
     t = bench.measured_traffic("cantilever", k32)
     assert t is not None and 0.9 * 98.09e6 < t < 2 * 98.09e6
     # the tube in tissue: the SMA launch and the neo-Hookean-only launch add
