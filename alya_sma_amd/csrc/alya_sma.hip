@@ -335,13 +335,8 @@ void launch_element_mode(const sld_dev* d, const sld::ElemArgs& A, hipStream_t s
     else launch_element_damp<R, SUR, false>(d, A, s);
     return;
   }
-  if (gather) {
-    if (st) launch_element_epb<R, true, SUR, true>(d->epb, A, s, d->persistent);
-    else launch_element_epb<R, false, SUR, true>(d->epb, A, s, d->persistent);
-  } else {
-    if (st) launch_element_epb<R, true, SUR, false>(d->epb, A, s, d->persistent);
-    else launch_element_epb<R, false, SUR, false>(d->epb, A, s, d->persistent);
-  }
+  if (st) launch_element_epb<R, true, SUR, false>(d->epb, A, s, d->persistent);
+  else launch_element_epb<R, false, SUR, false>(d->epb, A, s, d->persistent);
 }
 
 bool surrogate_attached(const sld_dev* d) { return d->sur != nullptr && d->sur->mlp != nullptr; }

@@ -761,6 +761,17 @@ struct ElemOccupancy {
                                            : (sizeof(R) == 8 ? SLD_WAVES_F64 : SLD_WAVES_F32);
 };
 
+// c with its sign flipped when neg is 1 (neg is 0 or 1): an integer xor on
+// the sign bit, so that a lane-dependent signed weight costs no pair of
+// constant registers held through the kernel
+template <typename R>
+__device__ __forceinline__ R sign_if(R c, int neg) {
+  if constexpr (sizeof(R) == 8)
+    return __longlong_as_double(__double_as_longlong(c) ^ ((long long)(neg & 1) << 63));
+  else
+    return __int_as_float(__float_as_int(c) ^ (int)((unsigned)(neg & 1) << 31));
+}
+
 // The value of x, opaque to the optimiser: quantities derived from the lane
 // index (shape-function derivatives at the lane's Gauss point or node) are
 // then recomputed where they are used instead of being hoisted out of the
@@ -781,11 +792,21 @@ template <typename R, int EPB, bool STORE, int SUR = 0, bool GATHER = false, int
 __global__ void __launch_bounds__(EPB * 8, (ElemOccupancy<R, LAWS, DAMP>::waves))
     sld_element(ElemArgs A) {
   constexpr int NC = DAMP ? 9 : 6;        // gathered components: x, u (, v)
-  __shared__ R s_rel[EPB][NC][8];         // x_a - x_0 (0..2), u_a - u_0 (3..5), v_a - v_0 (6..8)
-  // V(h) of the components; rows padded to 8 for aligned reads, except in
-  // the neo-Hookean-only kernel, whose occupancy the LDS bounds
-  __shared__ R s_v[EPB][8][DAMP ? 10 : (LAWS == 2 ? 6 : 8)];
-  __shared__ R s_q[EPB][DAMP ? 15 : 9][8];   // Q(3i+k) (, q_i, z_k) at each Gauss point
+  constexpr int NQ = DAMP ? 15 : 9;
+  // The relative nodal values are dead once V(h) is formed (a barrier
+  // follows), before the Gauss-point rows are first written: one LDS block
+  // serves both (the barrier at the end of a tile keeps the next tile's
+  // gather off the rows phase 3 still reads)
+  __shared__ union {
+    R rel[EPB][NC][8];                    // x_a - x_0 (0..2), u_a - u_0 (3..5), v_a - v_0 (6..8)
+    R q[EPB][NQ][8];                      // Q(3i+k) (, q_i, z_k) at each Gauss point
+  } s_relq;
+  auto& s_rel = s_relq.rel;
+  auto& s_q = s_relq.q;
+  // V(h) of the components; FP32 rows padded to 8 for aligned reads (FP64
+  // rows of 6 are 16-byte aligned already; the neo-Hookean-only kernel's
+  // occupancy is bound by its LDS)
+  __shared__ R s_v[EPB][8][DAMP ? 10 : ((LAWS == 2 || sizeof(R) == 8) ? 6 : 8)];
   __shared__ R s_u4[EPB][DAMP ? 11 : 8][4];  // U0, U1, U2, U12 of Q(0..7) (, z_0..2)
   __shared__ R s_wq[DAMP ? EPB : 1][3][8];   // Walsh-Hadamard coefficients of q_i
   __shared__ double s_v0[DAMP ? EPB : 1][3]; // node 0's velocity
@@ -862,49 +883,62 @@ __global__ void __launch_bounds__(EPB * 8, (ElemOccupancy<R, LAWS, DAMP>::waves)
   uint32_t fl = 0;
   if (valid) {
     const int gk = opaque(g);
-    R J[9], Gu[9];  // col-major: J(i,k) = J[i + 3k]
-    {
-      // signed weights of the lane's GP: c s_gm / 8 and c^2 s_gm s_gn / 8
+    // signed weights of the lane's GP: c s_gm / 8 and c^2 s_gm s_gn / 8;
+    // row i of J (i < 3) or of G_u (i = 3..5) at the GP, column k
+    auto grad_row = [&](int i, R (&out)[3]) {
       const R c1 = (R)(0.125 * HEX_GPOS), c2 = (R)(0.125 * HEX_GPOS * HEX_GPOS);
       const R k0 = (R)0.125;
-      const R ax = (gk & 4) ? c1 : -c1, ay = (gk & 2) ? c1 : -c1, az = (gk & 1) ? c1 : -c1;
-      const R byz = ((gk >> 1) ^ gk) & 1 ? -c2 : c2;
-      const R bxz = ((gk >> 2) ^ gk) & 1 ? -c2 : c2;
-      const R bxy = ((gk >> 2) ^ (gk >> 1)) & 1 ? -c2 : c2;
+      const R ax = sign_if(c1, ((gk >> 2) & 1) ^ 1), ay = sign_if(c1, ((gk >> 1) & 1) ^ 1),
+              az = sign_if(c1, (gk & 1) ^ 1);
+      const R byz = sign_if(c2, ((gk >> 1) ^ gk) & 1);
+      const R bxz = sign_if(c2, ((gk >> 2) ^ gk) & 1);
+      const R bxy = sign_if(c2, ((gk >> 2) ^ (gk >> 1)) & 1);
       const R* V7 = s_v[le][7];
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         const int hk = 4 >> k, h1 = k == 0 ? 2 : 4, h2 = k == 2 ? 2 : 1;
         const R w1 = k == 0 ? ay : ax, w2 = k == 2 ? ay : az, w3 = k == 0 ? byz : k == 1 ? bxz : bxy;
-        const R* Vk = s_v[le][hk];
-        const R* V1 = s_v[le][hk | h1];
-        const R* V2 = s_v[le][hk | h2];
-#pragma unroll
-        for (int i = 0; i < 6; ++i) {
-          const R val = k0 * Vk[i] + w1 * V1[i] + w2 * V2[i] + w3 * V7[i];
-          if (i < 3) J[i + 3 * k] = val;
-          else Gu[i - 3 + 3 * k] = val;
-        }
+        out[k] = k0 * s_v[le][hk][i] + w1 * s_v[le][hk | h1][i] + w2 * s_v[le][hk | h2][i] +
+                 w3 * V7[i];
       }
-    }
+    };
+    // J^-1 first (J is then dead), H = F - I = G_u J^-1 (G_u row by row)
     R Ji[9];
-    const R detJ = sma::inv3<R>(J, Ji);
-    // H = F - I = Gu J^-1
+    R detJ;
+    {
+      R J[9];  // col-major: J(i,k) = J[i + 3k]
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        R r3[3];
+        grad_row(i, r3);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) J[i + 3 * k] = r3[k];
+      }
+      detJ = sma::inv3<R>(J, Ji);
+    }
     R H[9];
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
+    for (int i = 0; i < 3; ++i) {
+      R gu[3];
+      grad_row(3 + i, gu);
 #pragma unroll
       for (int j = 0; j < 3; ++j)
-        H[i + 3 * j] = Gu[i] * Ji[3 * j] + Gu[i + 3] * Ji[1 + 3 * j] + Gu[i + 6] * Ji[2 + 3 * j];
-    // current Jacobian j = J + Gu = F J and its inverse, formed before the
-    // constitutive update so that J and Gu are dead through it
+        H[i + 3 * j] = gu[0] * Ji[3 * j] + gu[1] * Ji[1 + 3 * j] + gu[2] * Ji[2 + 3 * j];
+    }
+    // current Jacobian j = J + G_u = F J: j^-1 = J^-1 F^-1 and det j =
+    // det J det F, formed before the constitutive update
     R jci[9];
-    R detj;
+    R detF;
     {
-      R jc[9];
+      R Fm1[9], Fi[9];
 #pragma unroll
-      for (int q = 0; q < 9; ++q) jc[q] = J[q] + Gu[q];
-      detj = sma::inv3<R>(jc, jci);
+      for (int q = 0; q < 9; ++q) Fm1[q] = H[q] + ((q % 4 == 0) ? R(1) : R(0));
+      detF = sma::inv3<R>(Fm1, Fi);
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+          jci[r + 3 * c] = Ji[r] * Fi[3 * c] + Ji[r + 3] * Fi[1 + 3 * c] + Ji[r + 6] * Fi[2 + 3 * c];
     }
 
     if constexpr (sizeof(R) == 8) {
@@ -915,12 +949,11 @@ __global__ void __launch_bounds__(EPB * 8, (ElemOccupancy<R, LAWS, DAMP>::waves)
       for (int q = 0; q < 9; ++q) s_q[le][q][g] = jci[q];
       s_u4[le][g][0] = detJ;
     }
-    // det F = detj / detJ > 0 (false for NaN); det F <= 0 stops the run
-    // (NaN is left to the residual check)
-    const bool detf_pos = detj > R(0) ? detJ > R(0) : (detj < R(0) && detJ < R(0));
+    // det F > 0 (false for NaN); det F <= 0 stops the run (NaN is left to
+    // the residual check)
+    const bool detf_pos = detF > R(0);
     if constexpr (SUR != 1) {
-      if (!detf_pos && ((detj <= R(0) && detJ > R(0)) || (detj >= R(0) && detJ < R(0))))
-        report_detf(A.health, e, (double)detj / (double)detJ);
+      if (detF <= R(0)) report_detf(A.health, e, (double)detF);
     }
     const int mat = A.mat_of_elem ? A.mat_of_elem[e] : 0;
     const double* pr = A.props + 18 * mat;
@@ -1074,11 +1107,12 @@ __global__ void __launch_bounds__(EPB * 8, (ElemOccupancy<R, LAWS, DAMP>::waves)
       const int gd = opaque(g);
       const R c1 = (R)(0.125 * HEX_GPOS), c2 = (R)(0.125 * HEX_GPOS * HEX_GPOS);
       const R c3 = (R)(0.125 * HEX_GPOS * HEX_GPOS * HEX_GPOS), k0 = (R)0.125;
-      const R ax = (gd & 4) ? c1 : -c1, ay = (gd & 2) ? c1 : -c1, az = (gd & 1) ? c1 : -c1;
-      const R byz = ((gd >> 1) ^ gd) & 1 ? -c2 : c2;
-      const R bxz = ((gd >> 2) ^ gd) & 1 ? -c2 : c2;
-      const R bxy = ((gd >> 2) ^ (gd >> 1)) & 1 ? -c2 : c2;
-      const R bxyz = ((gd >> 2) ^ (gd >> 1) ^ gd) & 1 ? c3 : -c3;
+      const R ax = sign_if(c1, ((gd >> 2) & 1) ^ 1), ay = sign_if(c1, ((gd >> 1) & 1) ^ 1),
+              az = sign_if(c1, (gd & 1) ^ 1);
+      const R byz = sign_if(c2, ((gd >> 1) ^ gd) & 1);
+      const R bxz = sign_if(c2, ((gd >> 2) ^ gd) & 1);
+      const R bxy = sign_if(c2, ((gd >> 2) ^ (gd >> 1)) & 1);
+      const R bxyz = sign_if(c3, (((gd >> 2) ^ (gd >> 1) ^ gd) & 1) ^ 1);
       // weights of V(h): h = 0, z, y, yz, x, xz, xy, xyz (bits x = 4, y = 2, z = 1)
       const R wv[8] = {k0, az, ay, byz, ax, bxz, bxy, bxyz};
       const double alpha = A.damping[2 * mat], beta = A.damping[2 * mat + 1];
@@ -1194,10 +1228,10 @@ __global__ void __launch_bounds__(EPB * 8, (ElemOccupancy<R, LAWS, DAMP>::waves)
       const int sk = k == 0 ? sx : k == 1 ? sy : sz;
       const int s1 = k == 0 ? sy : sx, s2 = k == 2 ? sy : sz;
       // signs s_ak, s_ak s_a1, s_ak s_a2, s_ak s_a1 s_a2 as bit parities
-      const R l0 = sk ? c0 : -c0;
-      const R l1 = (sk ^ s1 ^ 1) ? c1 : -c1;
-      const R l2 = (sk ^ s2 ^ 1) ? c1 : -c1;
-      const R l3 = (sk ^ s1 ^ s2) ? c2 : -c2;
+      const R l0 = sign_if(c0, sk ^ 1);
+      const R l1 = sign_if(c1, sk ^ s1);
+      const R l2 = sign_if(c1, sk ^ s2);
+      const R l3 = sign_if(c2, (sk ^ s1 ^ s2) ^ 1);
 #pragma unroll
       for (int i = 0; i < 3; ++i) {
         const R* u = (3 * i + k == 8) ? u8 : s_u4[le][3 * i + k];
@@ -1215,17 +1249,17 @@ __global__ void __launch_bounds__(EPB * 8, (ElemOccupancy<R, LAWS, DAMP>::waves)
       for (int k = 0; k < 3; ++k) {
         const int sk = k == 0 ? sx : k == 1 ? sy : sz;
         const int s1 = k == 0 ? sy : sx, s2 = k == 2 ? sy : sz;
-        const R l0 = sk ? c0 : -c0;
-        const R l1 = (sk ^ s1 ^ 1) ? c1 : -c1;
-        const R l2 = (sk ^ s2 ^ 1) ? c1 : -c1;
-        const R l3 = (sk ^ s1 ^ s2) ? c2 : -c2;
+        const R l0 = sign_if(c0, sk ^ 1);
+        const R l1 = sign_if(c1, sk ^ s1);
+        const R l2 = sign_if(c1, sk ^ s2);
+        const R l3 = sign_if(c2, (sk ^ s1 ^ s2) ^ 1);
         const R* u = s_u4[le][8 + k];
         cb += l0 * u[0] + l1 * u[1] + l2 * u[2] + l3 * u[3];
       }
       const R c3 = (R)(0.125 * HEX_GPOS * HEX_GPOS * HEX_GPOS);
-      const R px = sx ? c1 : -c1, py = sy ? c1 : -c1, pz = sz ? c1 : -c1;
-      const R pyz = (sy ^ sz) ? -c2 : c2, pxz = (sx ^ sz) ? -c2 : c2, pxy = (sx ^ sy) ? -c2 : c2;
-      const R pxyz = (sx ^ sy ^ sz) ? c3 : -c3;
+      const R px = sign_if(c1, sx ^ 1), py = sign_if(c1, sy ^ 1), pz = sign_if(c1, sz ^ 1);
+      const R pyz = sign_if(c2, sy ^ sz), pxz = sign_if(c2, sx ^ sz), pxy = sign_if(c2, sx ^ sy);
+      const R pxyz = sign_if(c3, (sx ^ sy ^ sz) ^ 1);
       const R wa[8] = {c0, pz, py, pyz, px, pxz, pxy, pxyz};
       R fd[3];
 #pragma unroll
@@ -1255,6 +1289,8 @@ __global__ void __launch_bounds__(EPB * 8, (ElemOccupancy<R, LAWS, DAMP>::waves)
       fz[node] += f2;
     }
   }
+  // the next tile's gather writes the LDS rows phase 3 has just read
+  if (t_end - t_begin > 1) __syncthreads();
   }  // tile loop
 }
 

@@ -227,7 +227,7 @@ int sma_mlp_forward_device(sma_mlp* m, int64_t n, const float* X, float* Y, void
   (void)hipGetDevice(&dev);
   (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
   const unsigned grid = (unsigned)std::min<int64_t>(ntile, ncu);
-  hipLaunchKernelGGL(mlp::mlp_infer, dim3(grid), dim3(256), mlp::INF_SMEM, (hipStream_t)stream, m->Wp,
+  hipLaunchKernelGGL(mlp::mlp_infer, dim3(grid), dim3(mlp::INF_THREADS), mlp::INF_SMEM, (hipStream_t)stream, m->Wp,
                      m->bp, X, n, (const unsigned*)nullptr, Y);
   SCHK(hipGetLastError());
   return SMA_OK;
@@ -284,7 +284,7 @@ int sma_mlp_infer_counted(sma_mlp* m, const float* X, const unsigned* n_dev, int
   (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
   const int64_t rows = mlp::INF_ROWS_PER_WAVE * mlp::INF_WAVES;
   const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>((cap + rows - 1) / rows, ncu));
-  hipLaunchKernelGGL(mlp::mlp_infer, dim3(grid), dim3(256), mlp::INF_SMEM, s, m->Wp, m->bp, X,
+  hipLaunchKernelGGL(mlp::mlp_infer, dim3(grid), dim3(mlp::INF_THREADS), mlp::INF_SMEM, s, m->Wp, m->bp, X,
                      (int64_t)0, n_dev, Y);
   SCHK(hipGetLastError());
   return 0;

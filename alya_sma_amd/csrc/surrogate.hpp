@@ -267,15 +267,24 @@ __global__ void pack_params(const float* __restrict__ W, const float* __restrict
 // Fused inference: all 5 layers per 128-row tile, weights staged in LDS once
 // per workgroup (persistent over row tiles), activations transposed through a
 // per-wave LDS tile between layers.  Y [n][12] fp32.
+//
+// The weights (110 KB of BF16) leave room for one workgroup per CU, so the
+// workgroup carries 8 waves of 16 rows: two waves per SIMD, so that one
+// wave's SiLU epilogue (two quarter-rate transcendentals per activation)
+// overlaps the other's MFMA tiles.  Each B fragment read from LDS feeds one
+// 16x16x32 MFMA.
 // ----------------------------------------------------------------------------
-constexpr int INF_ROWS_PER_WAVE = 32;
-constexpr int INF_WAVES = 4;
+constexpr int INF_ROWS_PER_WAVE = 16;
+constexpr int INF_WAVES = 8;
+constexpr int INF_THREADS = INF_WAVES * 64;
+constexpr int INF_RB = INF_ROWS_PER_WAVE / 16;   // 16-row MFMA blocks per wave
 constexpr int ACT_LD = NH + 8;   // padded row stride (bank spread)
 
-__global__ void __launch_bounds__(256) mlp_infer(const __bf16* __restrict__ Wp, const float* __restrict__ bp,
-                                                 const float* __restrict__ X, int64_t n_host,
-                                                 const unsigned* __restrict__ n_dev,
-                                                 float* __restrict__ Y) {
+__global__ void __launch_bounds__(INF_THREADS) mlp_infer(const __bf16* __restrict__ Wp,
+                                                         const float* __restrict__ bp,
+                                                         const float* __restrict__ X, int64_t n_host,
+                                                         const unsigned* __restrict__ n_dev,
+                                                         float* __restrict__ Y) {
   // row count from the host, or from a device counter (graph-captured steps)
   const int64_t n = n_dev ? (int64_t)n_dev[0] : n_host;
   extern __shared__ __align__(16) unsigned char smem[];
@@ -293,52 +302,55 @@ __global__ void __launch_bounds__(256) mlp_infer(const __bf16* __restrict__ Wp, 
   const int64_t ntile = (n + rows_per_block - 1) / rows_per_block;
   for (int64_t t = blockIdx.x; t < ntile; t += gridDim.x) {
     const int64_t row0 = t * rows_per_block + wave * INF_ROWS_PER_WAVE;
-    // input tile -> act[:, 0:32]
+    // input tile -> act[:, 0:32]; every activation row belongs to one wave,
+    // which alone writes and reads it, so no barrier separates the layers
     for (int i = lane; i < INF_ROWS_PER_WAVE * KP0; i += 64) {
       const int rr = i / KP0, k = i % KP0;
       const int64_t m = row0 + rr;
       act[rr * ACT_LD + k] = (__bf16)((m < n && k < NIN) ? X[m * NIN + k] : 0.f);
     }
-    __syncthreads();
 #pragma unroll 1
     for (int l = 0; l < NLAYER; ++l) {
       const int K = layer_k(l), N = layer_n(l);
       const __bf16* W = sW + woff(l);
-      f32x4 acc[2][8];
+      f32x4 acc[INF_RB][8];
 #pragma unroll
-      for (int a = 0; a < 2; ++a)
+      for (int a = 0; a < INF_RB; ++a)
 #pragma unroll
         for (int b = 0; b < 8; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
       for (int k0 = 0; k0 < K; k0 += 32) {
-        bf16x8 af[2];
+        bf16x8 af[INF_RB];
 #pragma unroll
-        for (int a = 0; a < 2; ++a)
+        for (int a = 0; a < INF_RB; ++a)
           af[a] = *reinterpret_cast<const bf16x8*>(act + (16 * a + r) * ACT_LD + k0 + kq);
 #pragma unroll
         for (int b = 0; b < 8; ++b) {
           if (16 * b >= N) break;
           const bf16x8 bfr = *reinterpret_cast<const bf16x8*>(W + (16 * b + r) * K + k0 + kq);
 #pragma unroll
-          for (int a = 0; a < 2; ++a)
+          for (int a = 0; a < INF_RB; ++a)
             acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a], bfr, acc[a][b], 0, 0, 0);
         }
       }
-      __syncthreads();   // every wave has read its A tile before it is overwritten
+      // a wave reads and writes only its own activation rows: the wave's
+      // own ordering (its LDS reads above precede its writes below) suffices
       const float* bias = sb + boff(l);
       if (l < NLAYER - 1) {
 #pragma unroll
-        for (int a = 0; a < 2; ++a)
+        for (int a = 0; a < INF_RB; ++a)
 #pragma unroll
-          for (int b = 0; b < 8; ++b)
+          for (int b = 0; b < 8; ++b) {
+            const int col = 16 * b + r;
+            const float bc = bias[col];
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-              const int row = 16 * a + rq + i, col = 16 * b + r;
-              act[row * ACT_LD + col] = (__bf16)silu(acc[a][b][i] + bias[col]);
+              const int row = 16 * a + rq + i;
+              act[row * ACT_LD + col] = (__bf16)silu(acc[a][b][i] + bc);
             }
-        __syncthreads();
+          }
       } else {
 #pragma unroll
-        for (int a = 0; a < 2; ++a)
+        for (int a = 0; a < INF_RB; ++a)
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
             const int row = 16 * a + rq + i, col = r;
