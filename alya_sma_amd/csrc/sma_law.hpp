@@ -505,6 +505,15 @@ __host__ __device__ __forceinline__ void sym_sqrt64(double (&y)[6]) {
   }
 }
 
+// 1/(2k+1), k = 0..40, the series' coefficients (rounded once, here, instead
+// of a division per term on the device)
+constexpr double kInvOdd64[41] = {
+    1.0 / 1,  1.0 / 3,  1.0 / 5,  1.0 / 7,  1.0 / 9,  1.0 / 11, 1.0 / 13, 1.0 / 15, 1.0 / 17,
+    1.0 / 19, 1.0 / 21, 1.0 / 23, 1.0 / 25, 1.0 / 27, 1.0 / 29, 1.0 / 31, 1.0 / 33, 1.0 / 35,
+    1.0 / 37, 1.0 / 39, 1.0 / 41, 1.0 / 43, 1.0 / 45, 1.0 / 47, 1.0 / 49, 1.0 / 51, 1.0 / 53,
+    1.0 / 55, 1.0 / 57, 1.0 / 59, 1.0 / 61, 1.0 / 63, 1.0 / 65, 1.0 / 67, 1.0 / 69, 1.0 / 71,
+    1.0 / 73, 1.0 / 75, 1.0 / 77, 1.0 / 79, 1.0 / 81};
+
 __host__ __device__ __forceinline__ Hencky64 hencky_fp64(const double (&H)[9]) {
   Hencky64 h;
   double c[6];
@@ -543,7 +552,7 @@ __host__ __device__ __forceinline__ Hencky64 hencky_fp64(const double (&H)[9]) {
     for (int k = 1; k <= 40 && pw > lim * (double)(2 * k + 1); ++k) {
       double tn[6];
       sym_mul_commuting64(t, z2, tn);
-      const double w = 1.0 / (double)(2 * k + 1);
+      const double w = kInvOdd64[k];
 #pragma unroll
       for (int q = 0; q < 6; ++q) {
         t[q] = tn[q];

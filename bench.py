@@ -228,9 +228,16 @@ def cpu_baseline(prob, state, budget_s):
     nproc, quota = host_cores()
     threads = nproc if quota is None else max(1, min(nproc, int(math.ceil(quota))))
     res = {}
+    # a generic state: the device's FP64 step keeps strains of 1e-162..1e-77
+    # (ahead of a loading wave, zero shears of symmetric specimens) where the
+    # reference's law -- and so the oracle -- returns NaN and stops; a
+    # perturbation of 1e-15 of the largest displacement lifts them out of
+    # that band without changing the work of a step
+    rng = np.random.default_rng(1)
+    u0 = state["u"] + 1e-15 * np.abs(state["u"]).max() * rng.standard_normal(state["u"].shape)
     for nt in (threads, 1):
         O.set_threads(nt)
-        orc.u[:], orc.v[:], orc.a[:] = state["u"], state["v"], state["a"]
+        orc.u[:], orc.v[:], orc.a[:] = u0, state["v"], state["a"]
         orc.sdv[0], orc.sdv[1] = state["sdv"], state["sdv"]
         orc.s.t = float(state["time"][0])
         if orc.step(dt, scheme=0):                # first step outside the sample (page-in)
