@@ -300,8 +300,16 @@ void launch_element_epb(int epb, const sld::ElemArgs& A, hipStream_t s, int pers
   } else {
     if (epb == 8) launch_element<R, 8, STORE, SUR, GATHER, LAWS>(A, s, persistent);
     else if (epb == 16) launch_element<R, 16, STORE, SUR, GATHER, LAWS>(A, s, persistent);
-    else launch_element<R, 32, STORE, SUR, GATHER>(A, s, persistent);
+    else launch_element<R, 32, STORE, SUR, GATHER, LAWS == 2 ? 0 : LAWS>(A, s, persistent);
   }
+}
+
+// A launch whose elements all take the SMA law (no law_of_elem, or the SMA
+// range of a law-grouped mesh) runs the kernel without the neo-Hookean path.
+template <typename R, bool STORE, int SUR, bool GATHER, bool DAMP = false>
+void launch_element_laws(const sld_dev* d, const sld::ElemArgs& A, hipStream_t s) {
+  if (d->law_of_elem) launch_element_epb<R, STORE, SUR, GATHER, 0, DAMP>(d->epb, A, s, d->persistent);
+  else launch_element_epb<R, STORE, SUR, GATHER, 1, DAMP>(d->epb, A, s, d->persistent);
 }
 
 template <typename R, int SUR, bool DAMP>
@@ -309,12 +317,12 @@ void launch_element_damp(const sld_dev* d, const sld::ElemArgs& A, hipStream_t s
   const bool st = d->store != 0;
   if constexpr (SUR == 0) {
     if (d->nh_begin >= 0 && A.e_end > d->nh_begin) {
-      // law-grouped mesh: the SMA range with the per-element kernel, the
-      // neo-Hookean range with the kernel that has only that law
+      // law-grouped mesh: the SMA range and the neo-Hookean range each with
+      // the kernel that has only its law
       sld::ElemArgs B = A;
       B.e_end = std::min(A.e_end, d->nh_begin);
-      if (st) launch_element_epb<R, true, 0, true, 0, DAMP>(d->epb, B, s, d->persistent);
-      else launch_element_epb<R, false, 0, true, 0, DAMP>(d->epb, B, s, d->persistent);
+      if (st) launch_element_epb<R, true, 0, true, 1, DAMP>(d->epb, B, s, d->persistent);
+      else launch_element_epb<R, false, 0, true, 1, DAMP>(d->epb, B, s, d->persistent);
       B.e_begin = std::max(A.e_begin, d->nh_begin);
       B.e_end = A.e_end;
       if (st) launch_element_epb<R, true, 0, true, 2, DAMP>(d->epb, B, s, d->persistent);
@@ -322,8 +330,8 @@ void launch_element_damp(const sld_dev* d, const sld::ElemArgs& A, hipStream_t s
       return;
     }
   }
-  if (st) launch_element_epb<R, true, SUR, true, 0, DAMP>(d->epb, A, s, d->persistent);
-  else launch_element_epb<R, false, SUR, true, 0, DAMP>(d->epb, A, s, d->persistent);
+  if (st) launch_element_laws<R, true, SUR, true, DAMP>(d, A, s);
+  else launch_element_laws<R, false, SUR, true, DAMP>(d, A, s);
 }
 
 template <typename R, int SUR>
@@ -335,8 +343,8 @@ void launch_element_mode(const sld_dev* d, const sld::ElemArgs& A, hipStream_t s
     else launch_element_damp<R, SUR, false>(d, A, s);
     return;
   }
-  if (st) launch_element_epb<R, true, SUR, false>(d->epb, A, s, d->persistent);
-  else launch_element_epb<R, false, SUR, false>(d->epb, A, s, d->persistent);
+  if (st) launch_element_laws<R, true, SUR, false>(d, A, s);
+  else launch_element_laws<R, false, SUR, false>(d, A, s);
 }
 
 bool surrogate_attached(const sld_dev* d) { return d->sur != nullptr && d->sur->mlp != nullptr; }
@@ -477,7 +485,7 @@ int enqueue_step(sld_dev* d, double dt, int scheme, double phi, int tn, KernelTi
       E.e_end = (int32_t)d->nelem;
       E.elem_list = nullptr;
       TBEGIN(3);
-      launch_element_epb<float, false, 1, false>(d->epb, E, s, d->persistent);
+      launch_element_laws<float, false, 1, false>(d, E, s);
       TEND();
       TBEGIN(3);
       int rc = sma_mlp_infer_counted(S->mlp, S->feat, S->count, S->cap, S->Y, s);

@@ -751,14 +751,24 @@ __device__ __forceinline__ void count_flags(unsigned long long* ctr, uint32_t fl
 #ifndef SLD_WAVES_F32
 #define SLD_WAVES_F32 4
 #endif
-// LAWS = 0: each element takes its own law (law_of_elem); LAWS = 2: every
-// element of the launch is neo-Hookean (a range of a law-grouped mesh),
-// without the SMA path and its registers (FP64: 128 VGPRs, 4 waves per SIMD,
-// a few values spilled; it runs 5% faster than at 3 waves without spills).
-template <typename R, int LAWS = 0, bool DAMP = false>
+#ifndef SLD_WAVES_F64_SMA
+#define SLD_WAVES_F64_SMA 4
+#endif
+// LAWS = 0: each element takes its own law (law_of_elem); LAWS = 1: every
+// element of the launch is SMA (a mesh without neo-Hookean elements, or the
+// SMA range of a law-grouped mesh), without the neo-Hookean path, whose
+// registers would otherwise add to the SMA path's (FP64: 124 VGPRs and 4
+// waves per SIMD against 163 and 3); LAWS = 2: every element of the launch
+// is neo-Hookean (the tissue range of a law-grouped mesh), without the SMA
+// path (FP64: 128 VGPRs, 4 waves per SIMD, a few values spilled; it runs 5%
+// faster than at 3 waves without spills).  Kernels that also store the
+// stresses (STORE) keep 3 waves in FP64, where 4 would spill.
+template <typename R, int LAWS = 0, bool DAMP = false, bool STORE = false>
 struct ElemOccupancy {
   static constexpr int waves = DAMP ? (sizeof(R) == 8 ? 1 : 2)
                                : LAWS == 2 ? (sizeof(R) == 8 ? 4 : 6)
+                               : LAWS == 1 ? (sizeof(R) == 8 ? (STORE ? 3 : SLD_WAVES_F64_SMA)
+                                                             : SLD_WAVES_F32)
                                            : (sizeof(R) == 8 ? SLD_WAVES_F64 : SLD_WAVES_F32);
 };
 
@@ -790,7 +800,7 @@ __device__ __forceinline__ int opaque(int x) {
 // alpha sum_g N_a q(g) - sum_g sum_k dN_a/dxi_k z_k to edamp.
 template <typename R, int EPB, bool STORE, int SUR = 0, bool GATHER = false, int LAWS = 0,
           bool DAMP = false>
-__global__ void __launch_bounds__(EPB * 8, (ElemOccupancy<R, LAWS, DAMP>::waves))
+__global__ void __launch_bounds__(EPB * 8, (ElemOccupancy<R, LAWS, DAMP, STORE>::waves))
     sld_element(ElemArgs A) {
   constexpr int NC = DAMP ? 9 : 6;        // gathered components: x, u (, v)
   constexpr int NQ = DAMP ? 15 : 9;
@@ -964,7 +974,7 @@ __global__ void __launch_bounds__(EPB * 8, (ElemOccupancy<R, LAWS, DAMP>::waves)
     // a neo-Hookean element has no history: its values stay as they are in
     // both slots (equal from creation and every set), so they are neither
     // read nor written
-    const bool neo = LAWS == 2 || (A.law_of_elem && A.law_of_elem[e]);
+    const bool neo = LAWS == 2 || (LAWS == 0 && A.law_of_elem && A.law_of_elem[e]);
     double xS0 = 0.0, FAS0 = 0.0, FSA0 = 0.0;
     if (!neo) {
       xS0 = A.sdv_n[0][gp];

@@ -102,13 +102,18 @@ def parse():
     return ap.parse_args()
 
 
-def kernel_name(mode, cfg):
+def kernel_name(mode, cfg, mixed=False):
+    """The element kernel that carries the SMA elements.  Its law range: 1 =
+    SMA only (every launch over a mesh without neo-Hookean elements, and the
+    SMA range of a law-grouped mesh in the gather assembly, which a
+    neo-Hookean-only launch, range 2, follows); 0 = each element's own law
+    (a mixed mesh in the scatter assembly or in surrogate mode).  No Rayleigh
+    damping."""
     r = "double" if mode == "fp64" else "float"
     sur = 2 if mode == "surrogate" else 0
     ga = "true" if cfg["assembly"] == "gather" else "false"
-    # the law range: 0 = each element's own law (2 = the neo-Hookean-only
-    # launch that follows it on a law-grouped mesh); no Rayleigh damping
-    return f"sld_element<{r}, {cfg['elems_per_block']}, false, {sur}, {ga}, 0, false>"
+    laws = 0 if mixed and (mode == "surrogate" or cfg["assembly"] != "gather") else 1
+    return f"sld_element<{r}, {cfg['elems_per_block']}, false, {sur}, {ga}, {laws}, false>"
 
 
 def _sma_mask(prob):
@@ -169,9 +174,9 @@ def measured_traffic(workload, kernel):
     """HBM bytes per element phase of `kernel` on `workload` from the committed
     rocprofv3 PMC summary (FETCH_SIZE and WRITE_SIZE scaled by a plain copy
     kernel): on a law-grouped mesh the phase is two launches, `kernel` over
-    the SMA range (last template argument 0) and its neo-Hookean-only
-    variant (2), whose bytes add.  None when the pair was not measured."""
-    nh = kernel.replace(", 0, false>", ", 2, false>")
+    the SMA range (law range 1) and its neo-Hookean-only variant (2), whose
+    bytes add.  None when the pair was not measured."""
+    nh = kernel.replace(", 1, false>", ", 2, false>")
     found = {}
     try:
         with open(TRAFFIC_FILE) as f:
@@ -427,7 +432,7 @@ def main():
         ms = 1e3 * el / K
         n_as = (c1["kf_AS"] - c0["kf_AS"]) / K
         n_sa = (c1["kf_SA"] - c0["kf_SA"]) / K
-        kname = kernel_name(mode, cfg)
+        kname = kernel_name(mode, cfg, mixed=not _sma_mask(prob).all())
         launch_s = tk["element_ms"] * 1e-3
         nbytes = algorithmic_bytes(prob, mode)
         achieved = nbytes / launch_s / 1e9
