@@ -772,6 +772,17 @@ struct ElemOccupancy {
                                            : (sizeof(R) == 8 ? SLD_WAVES_F64 : SLD_WAVES_F32);
 };
 
+// Elements of R in one element's LDS block, padded from n: a multiple of 16
+// bytes whose size in 4-byte words is an odd multiple of 4, so that the
+// blocks of 8 consecutive elements start 4 banks apart (mod 32) and values
+// read at the same offset by the 8 elements of a wave fall in 8 different
+// 4-bank groups.
+constexpr int lds_block(int n, int rsize) {
+  int m = n;
+  while ((m * rsize) % 16 != 0 || ((m * rsize) / 4) % 8 != 4) ++m;
+  return m;
+}
+
 // c with its sign flipped when neg is 1 (neg is 0 or 1): an integer xor on
 // the sign bit, so that a lane-dependent signed weight costs no pair of
 // constant registers held through the kernel
@@ -807,18 +818,34 @@ __global__ void __launch_bounds__(EPB * 8, (ElemOccupancy<R, LAWS, DAMP, STORE>:
   // The relative nodal values are dead once V(h) is formed (a barrier
   // follows), before the Gauss-point rows are first written: one LDS block
   // serves both (the barrier at the end of a tile keeps the next tile's
-  // gather off the rows phase 3 still reads)
+  // gather off the rows phase 3 still reads).  Each element's block is
+  // padded (lds_block) so that the 8 elements of a wave start in different
+  // LDS banks: the eight lanes of an element read the same value at once,
+  // and unpadded power-of-two strides put all eight elements' values in one
+  // bank.
+  // The FP32 neo-Hookean-only kernel, whose occupancy its LDS bounds (7
+  // waves per SIMD), keeps the unpadded blocks: the padding would cost it a
+  // wave and more time than the conflicts it removes.
+  constexpr bool PAD = !(LAWS == 2 && sizeof(R) == 4);
+  constexpr int VC = DAMP ? 10 : ((LAWS == 2 || sizeof(R) == 8) ? 6 : 8);  // V(h) row
+  constexpr int U4R = DAMP ? 11 : 8;
+  constexpr int RELN = PAD ? lds_block(NC * 8, sizeof(R)) : NC * 8;
+  constexpr int QN = PAD ? lds_block(NQ * 8, sizeof(R)) : NQ * 8;
+  constexpr int VN = PAD ? lds_block(8 * VC, sizeof(R)) : 8 * VC;
+  constexpr int UN = PAD ? lds_block(U4R * 4, sizeof(R)) : U4R * 4;
   __shared__ union {
-    R rel[EPB][NC][8];                    // x_a - x_0 (0..2), u_a - u_0 (3..5), v_a - v_0 (6..8)
-    R q[EPB][NQ][8];                      // Q(3i+k) (, q_i, z_k) at each Gauss point
+    R rel[EPB][RELN];                     // [c][a]: x_a - x_0 (0..2), u_a - u_0 (3..5), v_a - v_0 (6..8)
+    R q[EPB][QN];                         // [j][g]: Q(3i+k) (, q_i, z_k) at each Gauss point
   } s_relq;
-  auto& s_rel = s_relq.rel;
-  auto& s_q = s_relq.q;
-  // V(h) of the components; FP32 rows padded to 8 for aligned reads (FP64
+  // V(h) of the components ([h][c]; FP32 rows of 8 for aligned reads, FP64
   // rows of 6 are 16-byte aligned already; the neo-Hookean-only kernel's
   // occupancy is bound by its LDS)
-  __shared__ R s_v[EPB][8][DAMP ? 10 : ((LAWS == 2 || sizeof(R) == 8) ? 6 : 8)];
-  __shared__ R s_u4[EPB][DAMP ? 11 : 8][4];  // U0, U1, U2, U12 of Q(0..7) (, z_0..2)
+  __shared__ R s_vm[EPB][VN];
+  __shared__ R s_u4m[EPB][UN];            // [j][0..3]: U0, U1, U2, U12 of Q(0..7) (, z_0..2)
+  auto s_rel = [&](int l) { return reinterpret_cast<R(*)[8]>(&s_relq.rel[l][0]); };
+  auto s_q = [&](int l) { return reinterpret_cast<R(*)[8]>(&s_relq.q[l][0]); };
+  auto s_v = [&](int l) { return reinterpret_cast<R(*)[VC]>(&s_vm[l][0]); };
+  auto s_u4 = [&](int l) { return reinterpret_cast<R(*)[4]>(&s_u4m[l][0]); };
   __shared__ R s_wq[DAMP ? EPB : 1][3][8];   // Walsh-Hadamard coefficients of q_i
   __shared__ double s_v0[DAMP ? EPB : 1][3]; // node 0's velocity
 
@@ -869,7 +896,7 @@ __global__ void __launch_bounds__(EPB * 8, (ElemOccupancy<R, LAWS, DAMP, STORE>:
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
       const double x0 = __shfl(xa[c], lane0, 64);
-      if (valid) s_rel[le][c][g] = (R)(xa[c] - x0);
+      if (valid) s_rel(le)[c][g] = (R)(xa[c] - x0);
       if constexpr (DAMP)
         if (c >= 6 && valid && g == 0) s_v0[le][c - 6] = x0;
     }
@@ -880,12 +907,12 @@ __global__ void __launch_bounds__(EPB * 8, (ElemOccupancy<R, LAWS, DAMP, STORE>:
     for (int c = opaque(g); c < NC; c += 8) {
       R v[8];
 #pragma unroll
-      for (int b = 0; b < 8; ++b) v[b] = s_rel[le][c][hex_node((b >> 2) & 1, (b >> 1) & 1, b & 1)];
+      for (int b = 0; b < 8; ++b) v[b] = s_rel(le)[c][hex_node((b >> 2) & 1, (b >> 1) & 1, b & 1)];
       wht_stage(v, 4);
       wht_stage(v, 2);
       wht_stage(v, 1);
 #pragma unroll
-      for (int h = 0; h < 8; ++h) s_v[le][h][c] = v[h];
+      for (int h = 0; h < 8; ++h) s_v(le)[h][c] = v[h];
     }
   }
   __syncthreads();
@@ -904,12 +931,12 @@ __global__ void __launch_bounds__(EPB * 8, (ElemOccupancy<R, LAWS, DAMP, STORE>:
       const R byz = sign_if(c2, ((gk >> 1) ^ gk) & 1);
       const R bxz = sign_if(c2, ((gk >> 2) ^ gk) & 1);
       const R bxy = sign_if(c2, ((gk >> 2) ^ (gk >> 1)) & 1);
-      const R* V7 = s_v[le][7];
+      const R* V7 = s_v(le)[7];
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         const int hk = 4 >> k, h1 = k == 0 ? 2 : 4, h2 = k == 2 ? 2 : 1;
         const R w1 = k == 0 ? ay : ax, w2 = k == 2 ? ay : az, w3 = k == 0 ? byz : k == 1 ? bxz : bxy;
-        out[k] = k0 * s_v[le][hk][i] + w1 * s_v[le][hk | h1][i] + w2 * s_v[le][hk | h2][i] +
+        out[k] = k0 * s_v(le)[hk][i] + w1 * s_v(le)[hk | h1][i] + w2 * s_v(le)[hk | h2][i] +
                  w3 * V7[i];
       }
     };
@@ -957,8 +984,8 @@ __global__ void __launch_bounds__(EPB * 8, (ElemOccupancy<R, LAWS, DAMP, STORE>:
       // Q rows and this point's U slot are not written before it ends), so
       // that the law runs with 20 fewer live VGPRs
 #pragma unroll
-      for (int q = 0; q < 9; ++q) s_q[le][q][g] = jci[q];
-      s_u4[le][g][0] = detJ;
+      for (int q = 0; q < 9; ++q) s_q(le)[q][g] = jci[q];
+      s_u4(le)[g][0] = detJ;
     }
     // det F > 0 (false for NaN); det F <= 0 stops the run (NaN is left to
     // the residual check)
@@ -1096,8 +1123,8 @@ __global__ void __launch_bounds__(EPB * 8, (ElemOccupancy<R, LAWS, DAMP, STORE>:
     R jq[9], dJ;
     if constexpr (sizeof(R) == 8) {
 #pragma unroll
-      for (int q = 0; q < 9; ++q) jq[q] = s_q[le][q][g];
-      dJ = s_u4[le][g][0];
+      for (int q = 0; q < 9; ++q) jq[q] = s_q(le)[q][g];
+      dJ = s_u4(le)[g][0];
     } else {
 #pragma unroll
       for (int q = 0; q < 9; ++q) jq[q] = jci[q];
@@ -1107,7 +1134,7 @@ __global__ void __launch_bounds__(EPB * 8, (ElemOccupancy<R, LAWS, DAMP, STORE>:
     for (int i = 0; i < 3; ++i)
 #pragma unroll
       for (int k = 0; k < 3; ++k)
-        s_q[le][3 * i + k][g] =
+        s_q(le)[3 * i + k][g] =
             dJ * (T[i] * jq[k] + T[i + 3] * jq[k + 3] + T[i + 6] * jq[k + 6]);
 
     if constexpr (DAMP) {
@@ -1132,9 +1159,9 @@ __global__ void __launch_bounds__(EPB * 8, (ElemOccupancy<R, LAWS, DAMP, STORE>:
       for (int i = 0; i < 3; ++i) {
         R gv = R(0);
 #pragma unroll
-        for (int h = 0; h < 8; ++h) gv += wv[h] * s_v[le][h][6 + i];
+        for (int h = 0; h < 8; ++h) gv += wv[h] * s_v(le)[h][6 + i];
         gv += (R)s_v0[le][i];
-        s_q[le][9 + i][g] = arho * dJ * gv;
+        s_q(le)[9 + i][g] = arho * dJ * gv;
       }
       R z[3] = {R(0), R(0), R(0)};
       if (beta > 0.0) {
@@ -1142,14 +1169,14 @@ __global__ void __launch_bounds__(EPB * 8, (ElemOccupancy<R, LAWS, DAMP, STORE>:
         // z_k = beta detJ sum_j J^-1_kj w_j, w_j = sum_k L_kj S_jk with the
         // velocity gradient L = G_v J^-1 (gprat, sld_elmpre.f90:150-157)
         R Jd[9], Gv[9];
-        const R* V7 = s_v[le][7];
+        const R* V7 = s_v(le)[7];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
           const int hk = 4 >> k, h1 = k == 0 ? 2 : 4, h2 = k == 2 ? 2 : 1;
           const R w1 = k == 0 ? ay : ax, w2 = k == 2 ? ay : az, w3 = k == 0 ? byz : k == 1 ? bxz : bxy;
-          const R* Vk = s_v[le][hk];
-          const R* V1 = s_v[le][hk | h1];
-          const R* V2 = s_v[le][hk | h2];
+          const R* Vk = s_v(le)[hk];
+          const R* V1 = s_v(le)[hk | h1];
+          const R* V2 = s_v(le)[hk | h2];
 #pragma unroll
           for (int i = 0; i < 3; ++i) {
             Jd[i + 3 * k] = k0 * Vk[i] + w1 * V1[i] + w2 * V2[i] + w3 * V7[i];
@@ -1178,7 +1205,7 @@ __global__ void __launch_bounds__(EPB * 8, (ElemOccupancy<R, LAWS, DAMP, STORE>:
           z[k] = bd * (Jid[k] * w[0] + Jid[k + 3] * w[1] + Jid[k + 6] * w[2]);
       }
 #pragma unroll
-      for (int k = 0; k < 3; ++k) s_q[le][12 + k][g] = z[k];
+      for (int k = 0; k < 3; ++k) s_q(le)[12 + k][g] = z[k];
     }
 
     if constexpr (STORE) {
@@ -1207,14 +1234,14 @@ __global__ void __launch_bounds__(EPB * 8, (ElemOccupancy<R, LAWS, DAMP, STORE>:
   // Walsh-Hadamard coefficients of q_0..2
   if (valid) {
     const int j = opaque(g);
-    u4_of_q(s_q[le][j], j % 3, s_u4[le][j]);
+    u4_of_q(s_q(le)[j], j % 3, s_u4(le)[j]);
     if constexpr (DAMP) {
       if (j < 3) {
-        u4_of_q(s_q[le][12 + j], j, s_u4[le][8 + j]);
+        u4_of_q(s_q(le)[12 + j], j, s_u4(le)[8 + j]);
       } else if (j < 6) {
         R v[8];
 #pragma unroll
-        for (int b = 0; b < 8; ++b) v[b] = s_q[le][9 + j - 3][b];
+        for (int b = 0; b < 8; ++b) v[b] = s_q(le)[9 + j - 3][b];
         wht_stage(v, 4);
         wht_stage(v, 2);
         wht_stage(v, 1);
@@ -1232,7 +1259,7 @@ __global__ void __launch_bounds__(EPB * 8, (ElemOccupancy<R, LAWS, DAMP, STORE>:
     const int sx = (0x66 >> a) & 1, sy = (0xCC >> a) & 1, sz = (0xF0 >> a) & 1;
     const R c0 = (R)0.125, c1 = (R)(0.125 * HEX_GPOS), c2 = (R)(0.125 * HEX_GPOS * HEX_GPOS);
     R u8[4];
-    u4_of_q(s_q[le][8], 2, u8);
+    u4_of_q(s_q(le)[8], 2, u8);
     R f[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -1245,7 +1272,7 @@ __global__ void __launch_bounds__(EPB * 8, (ElemOccupancy<R, LAWS, DAMP, STORE>:
       const R l3 = sign_if(c2, (sk ^ s1 ^ s2) ^ 1);
 #pragma unroll
       for (int i = 0; i < 3; ++i) {
-        const R* u = (3 * i + k == 8) ? u8 : s_u4[le][3 * i + k];
+        const R* u = (3 * i + k == 8) ? u8 : s_u4(le)[3 * i + k];
         const R term = l0 * u[0] + l1 * u[1] + l2 * u[2] + l3 * u[3];
         f[i] = k == 0 ? term : f[i] + term;
       }
@@ -1264,7 +1291,7 @@ __global__ void __launch_bounds__(EPB * 8, (ElemOccupancy<R, LAWS, DAMP, STORE>:
         const R l1 = sign_if(c1, sk ^ s1);
         const R l2 = sign_if(c1, sk ^ s2);
         const R l3 = sign_if(c2, (sk ^ s1 ^ s2) ^ 1);
-        const R* u = s_u4[le][8 + k];
+        const R* u = s_u4(le)[8 + k];
         cb += l0 * u[0] + l1 * u[1] + l2 * u[2] + l3 * u[3];
       }
       const R c3 = (R)(0.125 * HEX_GPOS * HEX_GPOS * HEX_GPOS);
