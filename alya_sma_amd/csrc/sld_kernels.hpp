@@ -825,27 +825,38 @@ __global__ void __launch_bounds__(EPB * 8, (ElemOccupancy<R, LAWS, DAMP, STORE>:
   // bank.
   // The FP32 neo-Hookean-only kernel, whose occupancy its LDS bounds (7
   // waves per SIMD), keeps the unpadded blocks: the padding would cost it a
-  // wave and more time than the conflicts it removes.
+  // wave and more time than the conflicts it removes.  Padded kernels also
+  // give the rows of the relative values and of the Gauss-point values 9
+  // entries (18 banks apart in FP64), so that the 8 lanes of an element
+  // reading 8 different rows hit different banks, and FP64 U rows 5.
+  // Without damping, V(h) is dead once the kinematics are formed, and the U
+  // sums (and the FP64 det J kept through the law) take its block.
   constexpr bool PAD = !(LAWS == 2 && sizeof(R) == 4);
+  constexpr bool U4_IN_V = !DAMP;
+  constexpr int ROW = PAD ? 9 : 8;
+  constexpr int U4S = (PAD && !DAMP && sizeof(R) == 8) ? 5 : 4;
   constexpr int VC = DAMP ? 10 : ((LAWS == 2 || sizeof(R) == 8) ? 6 : 8);  // V(h) row
   constexpr int U4R = DAMP ? 11 : 8;
-  constexpr int RELN = PAD ? lds_block(NC * 8, sizeof(R)) : NC * 8;
-  constexpr int QN = PAD ? lds_block(NQ * 8, sizeof(R)) : NQ * 8;
-  constexpr int VN = PAD ? lds_block(8 * VC, sizeof(R)) : 8 * VC;
-  constexpr int UN = PAD ? lds_block(U4R * 4, sizeof(R)) : U4R * 4;
+  constexpr int RELN = PAD ? lds_block(NC * ROW, sizeof(R)) : NC * ROW;
+  constexpr int QN = PAD ? lds_block(NQ * ROW, sizeof(R)) : NQ * ROW;
+  constexpr int VN0 = U4_IN_V ? (8 * VC > U4R * U4S ? 8 * VC : U4R * U4S) : 8 * VC;
+  constexpr int VN = PAD ? lds_block(VN0, sizeof(R)) : VN0;
+  constexpr int UN = U4_IN_V ? 1 : (PAD ? lds_block(U4R * U4S, sizeof(R)) : U4R * U4S);
   __shared__ union {
     R rel[EPB][RELN];                     // [c][a]: x_a - x_0 (0..2), u_a - u_0 (3..5), v_a - v_0 (6..8)
     R q[EPB][QN];                         // [j][g]: Q(3i+k) (, q_i, z_k) at each Gauss point
   } s_relq;
   // V(h) of the components ([h][c]; FP32 rows of 8 for aligned reads, FP64
-  // rows of 6 are 16-byte aligned already; the neo-Hookean-only kernel's
-  // occupancy is bound by its LDS)
+  // rows of 6 are 16-byte aligned already)
   __shared__ R s_vm[EPB][VN];
-  __shared__ R s_u4m[EPB][UN];            // [j][0..3]: U0, U1, U2, U12 of Q(0..7) (, z_0..2)
-  auto s_rel = [&](int l) { return reinterpret_cast<R(*)[8]>(&s_relq.rel[l][0]); };
-  auto s_q = [&](int l) { return reinterpret_cast<R(*)[8]>(&s_relq.q[l][0]); };
+  __shared__ R s_u4m[U4_IN_V ? 1 : EPB][UN];  // [j][0..3]: U0, U1, U2, U12 of Q(0..7) (, z_0..2)
+  auto s_rel = [&](int l) { return reinterpret_cast<R(*)[ROW]>(&s_relq.rel[l][0]); };
+  auto s_q = [&](int l) { return reinterpret_cast<R(*)[ROW]>(&s_relq.q[l][0]); };
   auto s_v = [&](int l) { return reinterpret_cast<R(*)[VC]>(&s_vm[l][0]); };
-  auto s_u4 = [&](int l) { return reinterpret_cast<R(*)[4]>(&s_u4m[l][0]); };
+  auto s_u4 = [&](int l) {
+    if constexpr (U4_IN_V) return reinterpret_cast<R(*)[U4S]>(&s_vm[l][0]);
+    else return reinterpret_cast<R(*)[U4S]>(&s_u4m[l][0]);
+  };
   __shared__ R s_wq[DAMP ? EPB : 1][3][8];   // Walsh-Hadamard coefficients of q_i
   __shared__ double s_v0[DAMP ? EPB : 1][3]; // node 0's velocity
 
