@@ -761,12 +761,13 @@ __device__ __forceinline__ void count_flags(unsigned long long* ctr, uint32_t fl
 // waves per SIMD against 163 and 3); LAWS = 2: every element of the launch
 // is neo-Hookean (the tissue range of a law-grouped mesh), without the SMA
 // path (FP64: 128 VGPRs, 4 waves per SIMD, a few values spilled; it runs 5%
-// faster than at 3 waves without spills).  Kernels that also store the
-// stresses (STORE) keep 3 waves in FP64, where 4 would spill.
+// faster than at 3 waves without spills; FP32: 64 VGPRs, 8 waves, 4.3 KB of
+// LDS per workgroup).  Kernels that also store the stresses (STORE) keep 3
+// waves in FP64 and 6 in the FP32 neo-Hookean kernel, where more would spill.
 template <typename R, int LAWS = 0, bool DAMP = false, bool STORE = false>
 struct ElemOccupancy {
   static constexpr int waves = DAMP ? (sizeof(R) == 8 ? 1 : 2)
-                               : LAWS == 2 ? (sizeof(R) == 8 ? 4 : 6)
+                               : LAWS == 2 ? (sizeof(R) == 8 ? 4 : (STORE ? 6 : 8))
                                : LAWS == 1 ? (sizeof(R) == 8 ? (STORE ? 3 : SLD_WAVES_F64_SMA)
                                                              : SLD_WAVES_F32)
                                            : (sizeof(R) == 8 ? SLD_WAVES_F64 : SLD_WAVES_F32);
@@ -823,25 +824,21 @@ __global__ void __launch_bounds__(EPB * 8, (ElemOccupancy<R, LAWS, DAMP, STORE>:
   // LDS banks: the eight lanes of an element read the same value at once,
   // and unpadded power-of-two strides put all eight elements' values in one
   // bank.
-  // The FP32 neo-Hookean-only kernel, whose occupancy its LDS bounds (7
-  // waves per SIMD), keeps the unpadded blocks: the padding would cost it a
-  // wave and more time than the conflicts it removes.  Padded kernels also
-  // give the rows of the relative values and of the Gauss-point values 9
+  // The rows of the relative values and of the Gauss-point values have 9
   // entries (18 banks apart in FP64), so that the 8 lanes of an element
   // reading 8 different rows hit different banks, and FP64 U rows 5.
   // Without damping, V(h) is dead once the kinematics are formed, and the U
   // sums (and the FP64 det J kept through the law) take its block.
-  constexpr bool PAD = !(LAWS == 2 && sizeof(R) == 4);
   constexpr bool U4_IN_V = !DAMP;
-  constexpr int ROW = PAD ? 9 : 8;
-  constexpr int U4S = (PAD && !DAMP && sizeof(R) == 8) ? 5 : 4;
+  constexpr int ROW = 9;
+  constexpr int U4S = (!DAMP && sizeof(R) == 8) ? 5 : 4;
   constexpr int VC = DAMP ? 10 : ((LAWS == 2 || sizeof(R) == 8) ? 6 : 8);  // V(h) row
   constexpr int U4R = DAMP ? 11 : 8;
-  constexpr int RELN = PAD ? lds_block(NC * ROW, sizeof(R)) : NC * ROW;
-  constexpr int QN = PAD ? lds_block(NQ * ROW, sizeof(R)) : NQ * ROW;
-  constexpr int VN0 = U4_IN_V ? (8 * VC > U4R * U4S ? 8 * VC : U4R * U4S) : 8 * VC;
-  constexpr int VN = PAD ? lds_block(VN0, sizeof(R)) : VN0;
-  constexpr int UN = U4_IN_V ? 1 : (PAD ? lds_block(U4R * U4S, sizeof(R)) : U4R * U4S);
+  constexpr int RELN = lds_block(NC * ROW, sizeof(R));
+  constexpr int QN = lds_block(NQ * ROW, sizeof(R));
+  constexpr int VN = lds_block(U4_IN_V ? (8 * VC > U4R * U4S ? 8 * VC : U4R * U4S) : 8 * VC,
+                               sizeof(R));
+  constexpr int UN = U4_IN_V ? 1 : lds_block(U4R * U4S, sizeof(R));
   __shared__ union {
     R rel[EPB][RELN];                     // [c][a]: x_a - x_0 (0..2), u_a - u_0 (3..5), v_a - v_0 (6..8)
     R q[EPB][QN];                         // [j][g]: Q(3i+k) (, q_i, z_k) at each Gauss point
