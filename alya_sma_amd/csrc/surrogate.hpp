@@ -37,7 +37,7 @@ __host__ __device__ constexpr int woff(int l) {
   return l == 0 ? 0 : l == 1 ? NH * KP0 : l == 2 ? NH * KP0 + NH * NH
        : l == 3 ? NH * KP0 + 2 * NH * NH : NH * KP0 + 3 * NH * NH;
 }
-constexpr int WPAD_TOTAL = NH * KP0 + 3 * NH * NH + NP4 * NH;   // 55296
+constexpr int WPAD_TOTAL = NH * KP0 + 3 * NH * NH + NP4 * NH;   // 57344
 __host__ __device__ constexpr int boff(int l) { return l * NH; }
 constexpr int BPAD_TOTAL = 4 * NH + NP4;                         // 544
 // canonical (unpadded) master layout: W_l [nreal][kreal] then all biases
@@ -268,7 +268,7 @@ __global__ void pack_params(const float* __restrict__ W, const float* __restrict
 // per workgroup (persistent over row tiles), activations transposed through a
 // per-wave LDS tile between layers.  Y [n][12] fp32.
 //
-// The weights (110 KB of BF16) leave room for one workgroup per CU, so the
+// The weights (120 KB of BF16 with the row padding) leave room for one workgroup per CU, so the
 // workgroup carries 8 waves of 16 rows: two waves per SIMD, so that one
 // wave's SiLU epilogue (two quarter-rate transcendentals per activation)
 // overlaps the other's MFMA tiles.  Each B fragment read from LDS feeds one
@@ -279,6 +279,14 @@ constexpr int INF_WAVES = 8;
 constexpr int INF_THREADS = INF_WAVES * 64;
 constexpr int INF_RB = INF_ROWS_PER_WAVE / 16;   // 16-row MFMA blocks per wave
 constexpr int ACT_LD = NH + 8;   // padded row stride (bank spread)
+// Weight rows in LDS: K + 8 BF16 (16 more bytes) per row.  A B fragment is 16
+// rows x 16 bytes read at once; with rows of K = 128 (or 32) BF16 every row
+// starts in the same LDS bank group, and the 16 reads would queue on it.
+__host__ __device__ constexpr int wld(int l) { return layer_k(l) + 8; }
+__host__ __device__ constexpr int woff_lds(int l) {
+  return l == 0 ? 0 : woff_lds(l - 1) + layer_n(l - 1) * wld(l - 1);
+}
+constexpr int WLDS_TOTAL = woff_lds(NLAYER);
 
 __global__ void __launch_bounds__(INF_THREADS) mlp_infer(const __bf16* __restrict__ Wp,
                                                          const float* __restrict__ bp,
@@ -289,11 +297,17 @@ __global__ void __launch_bounds__(INF_THREADS) mlp_infer(const __bf16* __restric
   const int64_t n = n_dev ? (int64_t)n_dev[0] : n_host;
   extern __shared__ __align__(16) unsigned char smem[];
   __bf16* sW = reinterpret_cast<__bf16*>(smem);
-  float* sb = reinterpret_cast<float*>(smem + WPAD_TOTAL * sizeof(__bf16));
-  __bf16* sAct = reinterpret_cast<__bf16*>(smem + WPAD_TOTAL * sizeof(__bf16) + BPAD_TOTAL * sizeof(float));
+  float* sb = reinterpret_cast<float*>(smem + WLDS_TOTAL * sizeof(__bf16));
+  __bf16* sAct = reinterpret_cast<__bf16*>(smem + WLDS_TOTAL * sizeof(__bf16) + BPAD_TOTAL * sizeof(float));
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  for (int i = tid; i < WPAD_TOTAL / 8; i += blockDim.x)
-    reinterpret_cast<bf16x8*>(sW)[i] = reinterpret_cast<const bf16x8*>(Wp)[i];
+  // packed rows of K BF16 -> LDS rows of K + 8, 16 bytes per copy
+  for (int i = tid; i < WPAD_TOTAL / 8; i += blockDim.x) {
+    int l = 0;
+    while (l < NLAYER - 1 && 8 * i >= woff(l + 1)) ++l;
+    const int j = 8 * i - woff(l), n = j / layer_k(l), k = j % layer_k(l);
+    *reinterpret_cast<bf16x8*>(sW + woff_lds(l) + n * wld(l) + k) =
+        reinterpret_cast<const bf16x8*>(Wp)[i];
+  }
   for (int i = tid; i < BPAD_TOTAL; i += blockDim.x) sb[i] = bp[i];
   __syncthreads();
   __bf16* act = sAct + wave * INF_ROWS_PER_WAVE * ACT_LD;
@@ -312,7 +326,8 @@ __global__ void __launch_bounds__(INF_THREADS) mlp_infer(const __bf16* __restric
 #pragma unroll 1
     for (int l = 0; l < NLAYER; ++l) {
       const int K = layer_k(l), N = layer_n(l);
-      const __bf16* W = sW + woff(l);
+      const __bf16* W = sW + woff_lds(l);
+      const int LDW = wld(l);
       f32x4 acc[INF_RB][8];
 #pragma unroll
       for (int a = 0; a < INF_RB; ++a)
@@ -326,7 +341,7 @@ __global__ void __launch_bounds__(INF_THREADS) mlp_infer(const __bf16* __restric
 #pragma unroll
         for (int b = 0; b < 8; ++b) {
           if (16 * b >= N) break;
-          const bf16x8 bfr = *reinterpret_cast<const bf16x8*>(W + (16 * b + r) * K + k0 + kq);
+          const bf16x8 bfr = *reinterpret_cast<const bf16x8*>(W + (16 * b + r) * LDW + k0 + kq);
 #pragma unroll
           for (int a = 0; a < INF_RB; ++a)
             acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a], bfr, acc[a][b], 0, 0, 0);
@@ -362,7 +377,7 @@ __global__ void __launch_bounds__(INF_THREADS) mlp_infer(const __bf16* __restric
   }
 }
 
-constexpr size_t INF_SMEM = WPAD_TOTAL * sizeof(__bf16) + BPAD_TOTAL * sizeof(float) +
+constexpr size_t INF_SMEM = WLDS_TOTAL * sizeof(__bf16) + BPAD_TOTAL * sizeof(float) +
                             INF_WAVES * INF_ROWS_PER_WAVE * ACT_LD * sizeof(__bf16);
 
 // splitmix64 counter-based generator
