@@ -49,6 +49,17 @@ def _compare(dev, orc, label, quiet=False):
     assert _rel(sdv[..., 1:], orc.sdv[0, ..., 1:]) < TOL
 
 
+def _sync_to_oracle(orc, dev):
+    """Copy the device's state into the oracle (both history slots)."""
+    orc.u[:] = dev.get("u")
+    orc.v[:] = dev.get("v")
+    orc.a[:] = dev.get("a")
+    sdv = dev.get("sdv")
+    orc.sdv[0] = sdv
+    orc.sdv[1] = sdv
+    orc.s.t = float(dev.get("time")[0])
+
+
 def _sync_from_oracle(dev, orc):
     """Copy the oracle's state into the device (restart path)."""
     dev.set("u", orc.u)
@@ -383,6 +394,70 @@ def test_tube_in_tissue_two_materials_match_oracle():
         mat = prob["mat_of_elem"]
         assert xs[mat == 0].max() > 0          # the tube transforms under the crimp
         assert np.all(xs[mat == 1] == 0)       # the tissue has no history
+
+
+def test_tube_in_tissue_scatter_runs_each_elements_law():
+    """The coloured scatter's colours cut across the law groups, so its
+    launches run the kernel that takes each element's own law; the gather
+    runs the SMA-only and the neo-Hookean-only kernels over the two ranges.
+    Same arithmetic per point, same additions per node: bitwise equal, in
+    FP64 and FP32, and the scatter matches the oracle."""
+    prob = problems.cylinder_problem(ntheta=16, nz=6, nr_tube=2, nr_tissue=3, nsteps=400)
+    dt = prob["run"]["dt"]
+    print("tube in tissue: gather vs scatter", flush=True)
+    for mode in (0, 1):
+        with Solidz(prob) as g, Solidz(prob) as sc:
+            sc.set_assembly("scatter")
+            for d in (g, sc):
+                d.set_mode(mode)
+                d.step(dt, nsteps=350)
+            for f in ("u", "v", "a", "fint", "sdv"):
+                assert np.array_equal(g.get(f), sc.get(f)), (mode, f)
+            assert sc.get("sdv")[..., 0].max() > 0
+    dev, orc = _pair(prob)
+    with dev:
+        dev.set_assembly("scatter")
+        for chunk in range(8):
+            if chunk:
+                _sync_from_oracle(dev, orc)
+            dev.step(dt, nsteps=25)
+            orc.step(dt, n=25)
+        _compare(dev, orc, "tube scatter 200")
+
+
+def test_tube_in_tissue_surrogate_mode_one_step():
+    """Surrogate mode on a mixed mesh runs the per-element-law kernels (the
+    prepass that lists points for the network, and the step): neo-Hookean
+    elements take their law, SMA points the network's update where the check
+    accepts it and the exact map elsewhere.  One step from an FP64 state in
+    the crimp stays within 1e-3 (relative L2 stress) of the oracle's FP64
+    step, and the tissue keeps no history."""
+    from alya_sma_amd import surrogate as S
+    net = S.load_default()
+    assert net is not None, "the committed network is missing"
+    prob = problems.cylinder_problem(ntheta=16, nz=6, nr_tube=2, nr_tissue=3, nsteps=400)
+    dt = prob["run"]["dt"]
+    print("tube in tissue: one surrogate step after 350 FP64 steps", flush=True)
+    dev, orc = _pair(prob)
+    with dev:
+        dev.step(dt, nsteps=350)
+        _sync_to_oracle(orc, dev)
+        dev.set_surrogate(net, 1e-4)
+        dev.set_mode(2)
+        c0 = dev.counters()
+        dev.step(dt, nsteps=1)
+        orc.step(dt, n=1)
+        c1 = dev.counters()
+        Sd = dev.get("stress")
+        l2 = np.linalg.norm(Sd - orc.S) / np.linalg.norm(orc.S)
+        listed = (c1["surrogate_used"] - c0["surrogate_used"]) + \
+            (c1["surrogate_fallback"] - c0["surrogate_fallback"])
+        print(f"  stress rel L2 {l2:.2e}, transforming {int(np.count_nonzero(orc.flags & 3))}, "
+              f"listed {listed}", flush=True)
+        assert np.count_nonzero(orc.flags & 3) > 0 and listed > 0
+        assert l2 < 1e-3
+        mat = prob["mat_of_elem"]
+        assert np.all(dev.get("sdv")[..., 0][mat == 1] == 0)
 
 
 def test_neo_hookean_only_mesh_matches_oracle():
