@@ -25,6 +25,7 @@ namespace mlp {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 
 constexpr int KP0 = 32, NP4 = 32;      // padded input / output widths
 constexpr int NLAYER = 5;
@@ -333,18 +334,22 @@ __global__ void __launch_bounds__(INF_THREADS) mlp_infer(const __bf16* __restric
       for (int a = 0; a < INF_RB; ++a)
 #pragma unroll
         for (int b = 0; b < 8; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+      // transposed product: A = the weight rows (16 output features x 32
+      // inputs), B = the activation rows (32 inputs x 16 rows), so that the
+      // result gives each lane 4 consecutive features of one row (features
+      // 16 b + rq .. + 3 of row 16 a + r), stored as one 8-byte write
       for (int k0 = 0; k0 < K; k0 += 32) {
-        bf16x8 af[INF_RB];
+        bf16x8 xf[INF_RB];
 #pragma unroll
         for (int a = 0; a < INF_RB; ++a)
-          af[a] = *reinterpret_cast<const bf16x8*>(act + (16 * a + r) * ACT_LD + k0 + kq);
+          xf[a] = *reinterpret_cast<const bf16x8*>(act + (16 * a + r) * ACT_LD + k0 + kq);
 #pragma unroll
         for (int b = 0; b < 8; ++b) {
           if (16 * b >= N) break;
-          const bf16x8 bfr = *reinterpret_cast<const bf16x8*>(W + (16 * b + r) * LDW + k0 + kq);
+          const bf16x8 wf = *reinterpret_cast<const bf16x8*>(W + (16 * b + r) * LDW + k0 + kq);
 #pragma unroll
           for (int a = 0; a < INF_RB; ++a)
-            acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a], bfr, acc[a][b], 0, 0, 0);
+            acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf, xf[a], acc[a][b], 0, 0, 0);
         }
       }
       // a wave reads and writes only its own activation rows: the wave's
@@ -355,23 +360,23 @@ __global__ void __launch_bounds__(INF_THREADS) mlp_infer(const __bf16* __restric
         for (int a = 0; a < INF_RB; ++a)
 #pragma unroll
           for (int b = 0; b < 8; ++b) {
-            const int col = 16 * b + r;
-            const float bc = bias[col];
+            const int col = 16 * b + rq;
+            const f32x4 bc = *reinterpret_cast<const f32x4*>(bias + col);
+            bf16x4 o;
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-              const int row = 16 * a + rq + i;
-              act[row * ACT_LD + col] = (__bf16)silu(acc[a][b][i] + bc);
-            }
+            for (int i = 0; i < 4; ++i) o[i] = (__bf16)silu(acc[a][b][i] + bc[i]);
+            *reinterpret_cast<bf16x4*>(act + (16 * a + r) * ACT_LD + col) = o;
           }
       } else {
 #pragma unroll
-        for (int a = 0; a < INF_RB; ++a)
+        for (int a = 0; a < INF_RB; ++a) {
+          const int64_t m = row0 + 16 * a + r;
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
-            const int row = 16 * a + rq + i, col = r;
-            const int64_t m = row0 + row;
+            const int col = rq + i;
             if (m < n && col < NOUT) Y[m * NOUT + col] = acc[a][0][i] + bias[col];
           }
+        }
       }
     }
   }
