@@ -157,14 +157,18 @@ struct sld_dev {
     sma_mlp* mlp = nullptr;
     double tol = 1e-4;
     double T = 310.0;
-    int64_t cap = 0;
+    int64_t seg_cap = 0;               // slots per segment of the list
+    int64_t cap = 0;                   // slots of the list (LIST_SEGS x seg_cap)
     uint8_t* status = nullptr;
     double* x = nullptr;
     float* feat = nullptr;
     int32_t* gp = nullptr;
     double* aux = nullptr;
-    unsigned* count = nullptr;
+    unsigned* count = nullptr;         // segment counters, LIST_STRIDE apart
     float* Y = nullptr;
+    // candidate elements by history slot: cand[tn] is read by the prepass
+    // of the step from slot tn, cand[1 - tn] written by its element launch
+    uint8_t* cand[2] = {nullptr, nullptr};
   };
   Sur* sur = nullptr;                // attached surrogate (mode 2)
 };
@@ -349,9 +353,27 @@ void launch_element_mode(const sld_dev* d, const sld::ElemArgs& A, hipStream_t s
 
 bool surrogate_attached(const sld_dev* d) { return d->sur != nullptr && d->sur->mlp != nullptr; }
 
+// Surrogate state for a step that does not follow the previous one: after a
+// state set from outside the stepping (restart, autotune trials, clearing a
+// halt), a mode switch or attaching a network, every element is a candidate
+// of the next prepass (the previous step's transforming elements are not
+// known) and the list starts empty (a step that halted before its
+// corrector leaves its counts behind).
+int surrogate_restart(sld_dev* d) {
+  if (!d->sur) return SMA_OK;
+  for (int k = 0; k < 2; ++k)
+    if (d->sur->cand[k]) HIPCHK(hipMemsetAsync(d->sur->cand[k], 1, (size_t)d->nelem, d->stream));
+  if (d->sur->count)
+    HIPCHK(hipMemsetAsync(d->sur->count, 0, mlp::LIST_SEGS * mlp::LIST_STRIDE * sizeof(unsigned),
+                          d->stream));
+  HIPCHK(hipStreamSynchronize(d->stream));
+  return SMA_OK;
+}
+
 void surrogate_detach(sld_dev* d) {
   if (!d->sur) return;
-  void* ps[] = {d->sur->status, d->sur->x, d->sur->feat, d->sur->gp, d->sur->aux, d->sur->count, d->sur->Y};
+  void* ps[] = {d->sur->status, d->sur->x, d->sur->feat, d->sur->gp, d->sur->aux, d->sur->count, d->sur->Y,
+                d->sur->cand[0], d->sur->cand[1]};
   for (void* p : ps) if (p) (void)hipFree(p);
   delete d->sur;
   d->sur = nullptr;
@@ -365,7 +387,8 @@ void surrogate_detach(sld_dev* d) {
 // scratch_counters), so the bracket measures launches, not event overhead.
 struct KernelTimer {
   std::vector<hipEvent_t> ev;
-  std::vector<int> kind;        // 0 predict, 1 element phase, 2 correct, 3 other
+  std::vector<int> kind;        // 0 predict, 1 element phase, 2 correct, 3-5 surrogate
+                                // prepass, inference, check
   size_t used = 0;
   int element_launches = 0;
   int elem_reps = 1;
@@ -472,14 +495,18 @@ int enqueue_step(sld_dev* d, double dt, int scheme, double phi, int tn, KernelTi
     E.xcd_remap = d->xcd;
     const bool sur = d->mode == SLD_MODE_SURROGATE;
     if (sur) {
-      // prepass over all elements: flags and features of transforming points
+      // prepass over the candidate elements (those with a transforming
+      // point in the previous step): flags and features of their
+      // transforming points; the other points take the exact map
       sld_dev::Sur* S = d->sur;
+      E.sur_cand = S->cand[tn];
       E.sur_status = S->status;
       E.sur_x = S->x;
       E.sur_feat = S->feat;
       E.sur_gp = S->gp;
       E.sur_aux = S->aux;
       E.sur_count = S->count;
+      E.sur_seg_cap = S->seg_cap;
       E.T_feat = S->T;
       E.e_begin = 0;
       E.e_end = (int32_t)d->nelem;
@@ -487,12 +514,13 @@ int enqueue_step(sld_dev* d, double dt, int scheme, double phi, int tn, KernelTi
       TBEGIN(3);
       launch_element_laws<float, false, 1, false>(d, E, s);
       TEND();
-      TBEGIN(3);
-      int rc = sma_mlp_infer_counted(S->mlp, S->feat, S->count, S->cap, S->Y, s);
+      TBEGIN(4);
+      int rc = sma_mlp_infer_listed(S->mlp, S->feat, S->count, S->seg_cap, S->Y, s);
       TEND();
       if (rc) return rc;
       sld::SurCheckArgs C{};
       C.count = S->count;
+      C.seg_cap = S->seg_cap;
       C.gp_of_slot = S->gp;
       C.aux = S->aux;
       C.feat = S->feat;
@@ -504,10 +532,12 @@ int enqueue_step(sld_dev* d, double dt, int scheme, double phi, int tn, KernelTi
       C.status = S->status;
       C.x_out = S->x;
       C.tol = S->tol;
-      C.cap = S->cap;
-      TBEGIN(3);
-      hipLaunchKernelGGL(sld::sld_sur_check, dim3(nblocks(S->cap, 256)), dim3(256), 0, s, C);
+      TBEGIN(5);
+      hipLaunchKernelGGL(sld::sld_sur_check, dim3(std::min<int64_t>(nblocks(S->cap, 256), 1024)),
+                         dim3(256), 0, s, C);
       TEND();
+      E.sur_cand = nullptr;
+      E.sur_cand_next = S->cand[1 - tn];
     }
     // gather: one launch over every element; scatter: one launch per colour
     const int nlaunch = gather ? (T ? T->elem_reps : 1) : d->ncolor;
@@ -928,18 +958,23 @@ int sld_dev_set_surrogate(sld_dev* d, sma_mlp* mlp, double tol) {
     auto* S = new sld_dev::Sur();
     d->sur = S;
     const int64_t ngp = d->nelem * 8;
-    S->cap = ngp;
+    // the listing pass runs waves of 8 elements (64 points) over the mesh
+    S->seg_cap = mlp::list_seg_cap((d->nelem + 7) / 8);
+    S->cap = S->seg_cap * mlp::LIST_SEGS;
+    const int64_t cap = S->cap, nctr = mlp::LIST_SEGS * mlp::LIST_STRIDE;
     int rc = 0;
     if ((rc = dalloc(&S->status, ngp)) || (rc = dalloc(&S->x, ngp)) ||
-        (rc = dalloc(&S->feat, ngp * 16)) || (rc = dalloc(&S->gp, ngp)) ||
-        (rc = dalloc(&S->aux, ngp * 2)) || (rc = dalloc(&S->count, 1)) ||
-        (rc = dalloc(&S->Y, ngp * 12))) {
+        (rc = dalloc(&S->feat, cap * 16)) || (rc = dalloc(&S->gp, cap)) ||
+        (rc = dalloc(&S->aux, cap * 2)) || (rc = dalloc(&S->count, nctr)) ||
+        (rc = dalloc(&S->Y, cap * 12)) || (rc = dalloc(&S->cand[0], d->nelem)) ||
+        (rc = dalloc(&S->cand[1], d->nelem))) {
       surrogate_detach(d);
       return rc;
     }
-    HIPCHK(hipMemset(S->count, 0, sizeof(unsigned)));
+    HIPCHK(hipMemset(S->count, 0, nctr * sizeof(unsigned)));
     HIPCHK(hipMemset(S->status, 0, ngp));
   }
+  if (int rc = surrogate_restart(d)) return rc;
   d->sur->mlp = mlp;
   d->sur->tol = tol;
   return SMA_OK;
@@ -960,6 +995,8 @@ int sld_dev_set_mode(sld_dev* d, int32_t mode) {
       HIPCHK(hipStreamSynchronize(d->stream));
     }
     d->mode = mode;
+    if (mode == SLD_MODE_SURROGATE)
+      if (int rc = surrogate_restart(d)) return rc;
   }
   return SMA_OK;
 }
@@ -1225,8 +1262,8 @@ int sld_dev_time_kernels(sld_dev* d, double dt, int32_t scheme, double phi, int3
     void* p;
     ~FreeScratch() { (void)hipFree(p); }
   } free_scratch{T.scratch_counters};
-  double sum[4] = {0, 0, 0, 0};
-  int64_t cnt[4] = {0, 0, 0, 0};
+  double sum[6] = {0, 0, 0, 0, 0, 0};
+  int64_t cnt[6] = {0, 0, 0, 0, 0, 0};
   hipEvent_t t0, t1;
   HIPCHK(hipEventCreate(&t0));
   HIPCHK(hipEventCreate(&t1));
@@ -1264,7 +1301,9 @@ int sld_dev_time_kernels(sld_dev* d, double dt, int32_t scheme, double phi, int3
   const double per_step = d->assembly == SLD_ASSEMBLY_GATHER
                               ? 1.0 : (double)T.element_launches / nsteps;
   out[3] = per_step;
-  out[4] = out[0] + out[2] + (cnt[3] ? sum[3] / nsteps : 0.0) + out[1] * per_step;  // kernel ms per step
+  // surrogate prepass, inference and check, ms per launch (0 in the other modes)
+  for (int k = 3; k < 6; ++k) out[k + 3] = cnt[k] ? sum[k] / cnt[k] : 0.0;
+  out[4] = out[0] + out[2] + out[6] + out[7] + out[8] + out[1] * per_step;  // kernel ms per step
   out[5] = (double)tot / nsteps;  // eager wall ms per step on the stream, timing repeats included
   return SMA_OK;
 }
@@ -1368,6 +1407,7 @@ int sld_dev_autotune(sld_dev* d, double dt, int32_t scheme, double phi, int32_t 
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   int rr = restore();
+  if (!rr) rr = surrogate_restart(d);
   (void)hipStreamSynchronize(s);
   cleanup();
   d->epb = b_epb;
@@ -1561,8 +1601,10 @@ int sld_dev_set(sld_dev* d, int32_t field, const void* host, int64_t count) {
   std::vector<double> tmp;
   // a state set by the caller (restart) starts a healthy run
   if (field == SLD_FIELD_U || field == SLD_FIELD_V || field == SLD_FIELD_A ||
-      field == SLD_FIELD_SDV || field == SLD_FIELD_TIME)
+      field == SLD_FIELD_SDV || field == SLD_FIELD_TIME) {
     if (int rc = clear_health(d)) return rc;
+    if (int rc = surrogate_restart(d)) return rc;
+  }
   switch (field) {
     case SLD_FIELD_U: case SLD_FIELD_V: case SLD_FIELD_A: {
       if (count != 3 * np) return fail(SMA_ERR_ARG, "count does not match field size");

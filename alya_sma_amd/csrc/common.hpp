@@ -20,6 +20,7 @@ inline int fail(int code, const std::string& msg) {
   } while (0)
 
 struct sma_mlp;
-// Fused MFMA inference on a device-side row count (surrogate.hip).
-int sma_mlp_infer_counted(sma_mlp* m, const float* X, const unsigned* n_dev, int64_t cap, float* Y,
-                          hipStream_t s);
+// Fused MFMA inference over the step's list of points, segment counters
+// on the device (surrogate.hip).
+int sma_mlp_infer_listed(sma_mlp* m, const float* X, const unsigned* seg_count, int64_t seg_cap,
+                         float* Y, hipStream_t s);

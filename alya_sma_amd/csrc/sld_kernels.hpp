@@ -515,7 +515,9 @@ __device__ __forceinline__ void correct_node(const NodalArgs& A, int64_t i, doub
   }
   if (i == 0) {
     A.clock[1 - A.tn] = cutim;
-    if (A.sur_count) A.sur_count[0] = 0u;   // the next step's compact list
+    // the next step's list
+    if (A.sur_count)
+      for (int k = 0; k < mlp::LIST_SEGS; ++k) A.sur_count[k * mlp::LIST_STRIDE] = 0u;
   }
 }
 
@@ -713,8 +715,15 @@ struct ElemArgs {
   float* sur_feat;              // [cap][16]
   int32_t* sur_gp;              // [cap]
   double* sur_aux;              // [cap][2] theta, e_n
-  unsigned* sur_count;
+  unsigned* sur_count;          // the list's segment counters (mlp::LIST_SEGS)
+  int64_t sur_seg_cap;          // slots per segment
   double T_feat;                // temperature fed to the surrogate (inert in the law)
+  // surrogate candidates: per element, 1 when one of its points transformed
+  // in the previous step (the prepass lists points of those elements only;
+  // every other point takes the exact map), and the flags the step writes
+  // for the next one; null: every element is a candidate
+  const uint8_t* sur_cand;
+  uint8_t* sur_cand_next;
 };
 
 // Flag counters live in COUNTER_SLOTS slots of 8 (one 64-byte line each);
@@ -878,6 +887,14 @@ __global__ void __launch_bounds__(EPB * 8, (ElemOccupancy<R, LAWS, DAMP, STORE>:
   const int i_el = A.e_begin + blk * EPB + le;
   const bool valid = i_el < A.e_end;
   const int e = !valid ? 0 : A.elem_list ? A.elem_list[i_el] : i_el;
+  if constexpr (SUR == 1) {
+    // prepass: a workgroup without a candidate element only marks its
+    // points as not listed (workgroup-uniform, before any barrier)
+    if (A.sur_cand && !__syncthreads_or(valid && A.sur_cand[e])) {
+      if (valid) A.sur_status[(int64_t)e * 8 + g] = 0;
+      continue;
+    }
+  }
 
   // -- phase 1: gather the element's 8 nodes (thread g loads node g); node
   // 0's values come from the element's first lane by a wave shuffle
@@ -1060,26 +1077,31 @@ __global__ void __launch_bounds__(EPB * 8, (ElemOccupancy<R, LAWS, DAMP, STORE>:
         bool kf_AS, kf_SA;
         double FAS, FSA;
         const bool flagged = sma::flags_only(m, th, en, xS0, FAS0, FSA0, kf_AS, kf_SA, FAS, FSA);
-        // single-branch points go to the compact list: one atomic per wave
+        // single-branch points go to the list: one atomic per wave, on the
+        // counter of the wave's segment (wave index mod LIST_SEGS; the wave
+        // covers elements 8 w .. 8 w + 7 of the launch)
         const bool want = flagged && (kf_AS != kf_SA);
         const unsigned long long wm = __ballot(want);
         const int lane = threadIdx.x & 63;
         const int leader = wm ? __ffsll((long long)wm) - 1 : 0;
+        const int seg = ((A.e_begin + blk * EPB) / 8 + (tid >> 6)) % mlp::LIST_SEGS;
         unsigned base = 0;
-        if (wm && lane == leader) base = atomicAdd(A.sur_count, (unsigned)__popcll(wm));
+        if (wm && lane == leader)
+          base = atomicAdd(A.sur_count + seg * mlp::LIST_STRIDE, (unsigned)__popcll(wm));
         base = __shfl(base, leader);
         uint8_t st = 0;
         if (flagged) {
           st = 2;
           if (want) {
             if (hk.ok) sp = sma::spectral_fp32(Hf);
-            const unsigned slot = base + (unsigned)__popcll(wm & ((1ull << lane) - 1ull));
+            const int64_t slot = (int64_t)seg * A.sur_seg_cap + base +
+                                 (unsigned)__popcll(wm & ((1ull << lane) - 1ull));
             const double E = A.E_of_elem ? A.E_of_elem[e] : pr[0];
             mlp::make_features(pr, E, A.T_feat, sp.e[0], sp.e[1], sp.e[2], th, xS0, FAS0,
-                               FSA0, FAS, A.sur_feat + (int64_t)slot * mlp::NIN);
+                               FSA0, FAS, A.sur_feat + slot * mlp::NIN);
             A.sur_gp[slot] = (int32_t)gp;
-            A.sur_aux[2 * (int64_t)slot] = th;
-            A.sur_aux[2 * (int64_t)slot + 1] = en;
+            A.sur_aux[2 * slot] = th;
+            A.sur_aux[2 * slot + 1] = en;
           }
         }
         A.sur_status[gp] = st;
@@ -1236,6 +1258,13 @@ __global__ void __launch_bounds__(EPB * 8, (ElemOccupancy<R, LAWS, DAMP, STORE>:
   }
   if constexpr (SUR == 1) continue;   // uniform: the prepass has no scatter
   count_flags(A.counters, fl, valid);
+  if constexpr (SUR == 2) {
+    // the next step's candidates: elements with a transforming point now
+    if (A.sur_cand_next) {
+      const unsigned long long tm = __ballot(valid && (fl & 3u));
+      if (valid && g == 0) A.sur_cand_next[e] = ((tm >> (8 * (le & 7))) & 0xffull) != 0ull;
+    }
+  }
   __syncthreads();
   // thread g: U of Q(j = g) (Q(8) is formed in phase 3 by every thread);
   // with damping threads 0-2 also form U of z_0..2 and threads 3-5 the
@@ -1347,7 +1376,8 @@ __global__ void __launch_bounds__(EPB * 8, (ElemOccupancy<R, LAWS, DAMP, STORE>:
 // values.
 // ----------------------------------------------------------------------------
 struct SurCheckArgs {
-  const unsigned* count;
+  const unsigned* count;        // the list's segment counters
+  int64_t seg_cap;
   const int32_t* gp_of_slot;
   const double* aux;            // theta, e_n
   const float* feat;            // [slot][16]
@@ -1359,12 +1389,9 @@ struct SurCheckArgs {
   uint8_t* status;
   double* x_out;
   double tol;
-  int64_t cap;
 };
 
-__global__ void __launch_bounds__(256) sld_sur_check(SurCheckArgs A) {
-  const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (slot >= A.cap || slot >= (int64_t)A.count[0]) return;
+__device__ inline void check_slot(const SurCheckArgs& A, int64_t slot) {
   const int64_t gp = A.gp_of_slot[slot];
   const int64_t e = gp >> 3;
   const int mat = A.mat_of_elem ? A.mat_of_elem[e] : 0;
@@ -1379,6 +1406,13 @@ __global__ void __launch_bounds__(256) sld_sur_check(SurCheckArgs A) {
   const bool ok = in_range && (r <= A.tol) && (x >= -1e-3) && (x <= 1.0 + 1e-3);
   A.status[gp] = ok ? 1 : 2;
   A.x_out[gp] = x;
+}
+
+__global__ void __launch_bounds__(256) sld_sur_check(SurCheckArgs A) {
+  __shared__ unsigned off[mlp::LIST_SEGS + 1];
+  const unsigned n = mlp::list_scan(A.count, off);
+  for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    check_slot(A, mlp::list_slot(off, i, A.seg_cap));
 }
 
 // ----------------------------------------------------------------------------

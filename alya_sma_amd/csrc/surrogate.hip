@@ -214,6 +214,7 @@ int sma_mlp_set_loss_weights(sma_mlp* m, const float* w12) {
 
 int sma_mlp_forward_device(sma_mlp* m, int64_t n, const float* X, float* Y, void* stream) {
   if (!m || n < 0 || (n > 0 && (!X || !Y))) return sma_abi::fail(SMA_ERR_ARG, "sma_mlp_forward_device: bad arguments");
+  if (n > INT32_MAX) return sma_abi::fail(SMA_ERR_ARG, "sma_mlp_forward_device: n must be below 2^31");
   if (n == 0) return SMA_OK;
   static bool attr = false;
   if (!attr) {
@@ -228,7 +229,7 @@ int sma_mlp_forward_device(sma_mlp* m, int64_t n, const float* X, float* Y, void
   (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
   const unsigned grid = (unsigned)std::min<int64_t>(ntile, ncu);
   hipLaunchKernelGGL(mlp::mlp_infer, dim3(grid), dim3(mlp::INF_THREADS), mlp::INF_SMEM, (hipStream_t)stream, m->Wp,
-                     m->bp, X, n, (const unsigned*)nullptr, Y);
+                     m->bp, X, n, (const unsigned*)nullptr, (int64_t)0, Y);
   SCHK(hipGetLastError());
   return SMA_OK;
 }
@@ -269,10 +270,10 @@ int sma_mlp_gen_samples_device(int64_t n, uint64_t seed, const double* props_dev
 
 }  // extern "C"
 
-// Internal: padded parameter pointers and an inference launch on a device
-// row count (used by the step's surrogate mode).
-int sma_mlp_infer_counted(sma_mlp* m, const float* X, const unsigned* n_dev, int64_t cap, float* Y,
-                          hipStream_t s) {
+// Internal: inference over the step's list of points (its segment counters
+// are read on the device; seg_cap slots per segment).
+int sma_mlp_infer_listed(sma_mlp* m, const float* X, const unsigned* seg_count, int64_t seg_cap,
+                         float* Y, hipStream_t s) {
   static bool attr = false;
   if (!attr) {
     SCHK(hipFuncSetAttribute((const void*)mlp::mlp_infer, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -283,9 +284,10 @@ int sma_mlp_infer_counted(sma_mlp* m, const float* X, const unsigned* n_dev, int
   (void)hipGetDevice(&dev);
   (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
   const int64_t rows = mlp::INF_ROWS_PER_WAVE * mlp::INF_WAVES;
+  const int64_t cap = seg_cap * mlp::LIST_SEGS;
   const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>((cap + rows - 1) / rows, ncu));
   hipLaunchKernelGGL(mlp::mlp_infer, dim3(grid), dim3(mlp::INF_THREADS), mlp::INF_SMEM, s, m->Wp, m->bp, X,
-                     (int64_t)0, n_dev, Y);
+                     (int64_t)0, seg_count, seg_cap, Y);
   SCHK(hipGetLastError());
   return 0;
 }

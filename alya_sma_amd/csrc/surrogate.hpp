@@ -292,10 +292,17 @@ constexpr int WLDS_TOTAL = woff_lds(NLAYER);
 __global__ void __launch_bounds__(INF_THREADS) mlp_infer(const __bf16* __restrict__ Wp,
                                                          const float* __restrict__ bp,
                                                          const float* __restrict__ X, int64_t n_host,
-                                                         const unsigned* __restrict__ n_dev,
-                                                         float* __restrict__ Y) {
-  // row count from the host, or from a device counter (graph-captured steps)
-  const int64_t n = n_dev ? (int64_t)n_dev[0] : n_host;
+                                                         const unsigned* __restrict__ seg_count,
+                                                         int64_t seg_cap, float* __restrict__ Y) {
+  // rows 0 .. n_host - 1 of X and Y, or the step's list: its segment
+  // counters are read on the device (graph-captured steps), the i-th listed
+  // point's row is its slot
+  __shared__ unsigned off[LIST_SEGS + 1];
+  const int64_t n = seg_count ? (int64_t)list_scan(seg_count, off) : n_host;
+  const int64_t rows_per_block = INF_ROWS_PER_WAVE * INF_WAVES;
+  const int64_t ntile = (n + rows_per_block - 1) / rows_per_block;
+  // a workgroup without a tile leaves before copying the weights
+  if ((int64_t)blockIdx.x >= ntile) return;
   extern __shared__ __align__(16) unsigned char smem[];
   __bf16* sW = reinterpret_cast<__bf16*>(smem);
   float* sb = reinterpret_cast<float*>(smem + WLDS_TOTAL * sizeof(__bf16));
@@ -313,16 +320,20 @@ __global__ void __launch_bounds__(INF_THREADS) mlp_infer(const __bf16* __restric
   __syncthreads();
   __bf16* act = sAct + wave * INF_ROWS_PER_WAVE * ACT_LD;
   const int r = lane & 15, kq = 8 * (lane >> 4), rq = 4 * (lane >> 4);
-  const int64_t rows_per_block = INF_ROWS_PER_WAVE * INF_WAVES;
-  const int64_t ntile = (n + rows_per_block - 1) / rows_per_block;
+  static_assert(INF_ROWS_PER_WAVE <= 64, "one lane holds the slot of each row of a wave");
   for (int64_t t = blockIdx.x; t < ntile; t += gridDim.x) {
     const int64_t row0 = t * rows_per_block + wave * INF_ROWS_PER_WAVE;
+    // lane l < INF_ROWS_PER_WAVE: the slot of row row0 + l
+    int slot_l = 0;
+    if (lane < INF_ROWS_PER_WAVE && row0 + lane < n)
+      slot_l = seg_count ? (int)list_slot(off, (unsigned)(row0 + lane), seg_cap) : (int)(row0 + lane);
     // input tile -> act[:, 0:32]; every activation row belongs to one wave,
     // which alone writes and reads it, so no barrier separates the layers
     for (int i = lane; i < INF_ROWS_PER_WAVE * KP0; i += 64) {
       const int rr = i / KP0, k = i % KP0;
       const int64_t m = row0 + rr;
-      act[rr * ACT_LD + k] = (__bf16)((m < n && k < NIN) ? X[m * NIN + k] : 0.f);
+      const int64_t sl = __shfl(slot_l, rr);
+      act[rr * ACT_LD + k] = (__bf16)((m < n && k < NIN) ? X[sl * NIN + k] : 0.f);
     }
 #pragma unroll 1
     for (int l = 0; l < NLAYER; ++l) {
@@ -371,10 +382,11 @@ __global__ void __launch_bounds__(INF_THREADS) mlp_infer(const __bf16* __restric
 #pragma unroll
         for (int a = 0; a < INF_RB; ++a) {
           const int64_t m = row0 + 16 * a + r;
+          const int64_t sl = __shfl(slot_l, 16 * a + r);
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
             const int col = rq + i;
-            if (m < n && col < NOUT) Y[m * NOUT + col] = acc[a][0][i] + bias[col];
+            if (m < n && col < NOUT) Y[sl * NOUT + col] = acc[a][0][i] + bias[col];
           }
         }
       }

@@ -2,6 +2,7 @@
 // step kernels and the surrogate's own kernels).
 #pragma once
 #include <hip/hip_runtime.h>
+#include <stdint.h>
 
 namespace mlp {
 
@@ -40,6 +41,56 @@ __host__ __device__ __forceinline__ void make_features(const double* p, double E
   f[13] = (float)((p[9] - 250.0) / 25.0);
   f[14] = (float)((p[10] - 150.0) / 15.0);
   f[15] = (float)(Ftrial * Scale::F);
+}
+
+// ----------------------------------------------------------------------------
+// The step's list of points for the network.  Appends go to LIST_SEGS
+// segments, each with its own counter (counters LIST_STRIDE words, 128
+// bytes, apart): wave w of the listing pass appends to segment
+// w % LIST_SEGS, so the appends of the waves spread over LIST_SEGS
+// addresses instead of queueing on one device-wide atomic.  Segment s owns
+// slots [s * seg_cap, s * seg_cap + count_s); seg_cap = 64 x the number of
+// waves that append to one segment (a wave lists at most its 64 points).
+// The readers number the listed points densely, 0 .. total - 1, across the
+// segments in order.
+// ----------------------------------------------------------------------------
+constexpr int LIST_SEGS = 64;
+constexpr int LIST_STRIDE = 32;
+
+__host__ __device__ constexpr int64_t list_seg_cap(int64_t nwaves) {
+  return 64 * ((nwaves + LIST_SEGS - 1) / LIST_SEGS);
+}
+
+// Exclusive prefix sums of the segment counts into off[0 .. LIST_SEGS]
+// (off[LIST_SEGS] = total), formed by the block's first wave; every thread
+// of the block must call it (it ends with a barrier).
+__device__ inline unsigned list_scan(const unsigned* count, unsigned* off) {
+  if (threadIdx.x < 64) {
+    const int l = threadIdx.x;
+    const unsigned c = count[l * LIST_STRIDE];
+    unsigned v = c;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const unsigned u = __shfl_up(v, d);
+      if (l >= d) v += u;
+    }
+    off[l] = v - c;
+    if (l == 63) off[LIST_SEGS] = v;
+  }
+  __syncthreads();
+  return off[LIST_SEGS];
+}
+
+// Slot of the i-th listed point (i < off[LIST_SEGS]): the segment s with
+// off[s] <= i < off[s + 1] (binary search; empty segments are passed over).
+__device__ inline int64_t list_slot(const unsigned* off, unsigned i, int64_t seg_cap) {
+  int lo = 0, hi = LIST_SEGS;   // off[lo] <= i < off[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (off[mid] <= i) lo = mid;
+    else hi = mid;
+  }
+  return (int64_t)lo * seg_cap + (i - off[lo]);
 }
 
 }  // namespace mlp

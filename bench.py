@@ -439,7 +439,8 @@ def main():
         flops = algorithmic_flops(prob, n_as, n_sa)
         tflops = flops / launch_s / 1e12
         res = {"value": ngp / (ms * 1e-3), "ms_per_step": ms, "launch": cfg,
-               "kernel_ms": {k: round(v, 5) for k, v in tk.items()},
+               "kernel_ms": {k: round(v, 5) for k, v in tk.items()
+                             if mode == "surrogate" or not k.startswith("sur_")},
                "transforming_gp_updates": int(c1["kf_AS"] + c1["kf_SA"] - c0["kf_AS"] - c0["kf_SA"]),
                "transforming_fraction": (n_as + n_sa) / ngp,
                "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS,

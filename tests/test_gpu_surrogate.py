@@ -178,9 +178,14 @@ def test_surrogate_in_step_gain():
             c1 = d.counters()
             fl = d.get("flags")
             used = (fl & 0x20) != 0
+            listed = int(c1["surrogate_fallback"] - c0["surrogate_fallback"] +
+                         c1["surrogate_used"] - c0["surrogate_used"])
+            # the state was set, so every element is a candidate: each
+            # single-branch transforming point is listed and checked
+            single = int(np.count_nonzero(((fl & 3) == 1) | ((fl & 3) == 2)))
+            assert listed == single, (name, target, listed, single)
             acc[name][0] += int(c1["surrogate_used"] - c0["surrogate_used"])
-            acc[name][1] += int(c1["surrogate_fallback"] - c0["surrogate_fallback"] +
-                                c1["surrogate_used"] - c0["surrogate_used"])
+            acc[name][1] += listed
             xb = d.get("sdv")[..., 0]
             # the surrogate modes form the FP32 strain spectrally (their
             # features are principal strains), the FP32 mode by series: the
