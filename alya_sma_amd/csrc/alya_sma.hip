@@ -159,11 +159,9 @@ struct sld_dev {
     double T = 310.0;
     int64_t seg_cap = 0;               // slots per segment of the list
     int64_t cap = 0;                   // slots of the list (LIST_SEGS x seg_cap)
-    uint8_t* status = nullptr;
-    double* x = nullptr;
-    float* feat = nullptr;
-    int32_t* gp = nullptr;
-    double* aux = nullptr;
+    uint8_t* status = nullptr;         // [gp] 0 not flagged, 1 listed, 2 flagged, not listed
+    int32_t* slot = nullptr;           // [gp] slot of a listed point
+    float* feat = nullptr;             // [cap][16]
     unsigned* count = nullptr;         // segment counters, LIST_STRIDE apart
     float* Y = nullptr;
     // candidate elements by history slot: cand[tn] is read by the prepass
@@ -372,7 +370,7 @@ int surrogate_restart(sld_dev* d) {
 
 void surrogate_detach(sld_dev* d) {
   if (!d->sur) return;
-  void* ps[] = {d->sur->status, d->sur->x, d->sur->feat, d->sur->gp, d->sur->aux, d->sur->count, d->sur->Y,
+  void* ps[] = {d->sur->status, d->sur->slot, d->sur->feat, d->sur->count, d->sur->Y,
                 d->sur->cand[0], d->sur->cand[1]};
   for (void* p : ps) if (p) (void)hipFree(p);
   delete d->sur;
@@ -387,8 +385,8 @@ void surrogate_detach(sld_dev* d) {
 // scratch_counters), so the bracket measures launches, not event overhead.
 struct KernelTimer {
   std::vector<hipEvent_t> ev;
-  std::vector<int> kind;        // 0 predict, 1 element phase, 2 correct, 3-5 surrogate
-                                // prepass, inference, check
+  std::vector<int> kind;        // 0 predict, 1 element phase, 2 correct, 3-4 surrogate
+                                // prepass and inference
   size_t used = 0;
   int element_launches = 0;
   int elem_reps = 1;
@@ -497,14 +495,15 @@ int enqueue_step(sld_dev* d, double dt, int scheme, double phi, int tn, KernelTi
     if (sur) {
       // prepass over the candidate elements (those with a transforming
       // point in the previous step): flags and features of their
-      // transforming points; the other points take the exact map
+      // transforming points, listed for the network; the element launch
+      // checks the network's answers (the other points take the exact map)
       sld_dev::Sur* S = d->sur;
       E.sur_cand = S->cand[tn];
       E.sur_status = S->status;
-      E.sur_x = S->x;
+      E.sur_slot = S->slot;
       E.sur_feat = S->feat;
-      E.sur_gp = S->gp;
-      E.sur_aux = S->aux;
+      E.sur_Y = S->Y;
+      E.sur_tol = S->tol;
       E.sur_count = S->count;
       E.sur_seg_cap = S->seg_cap;
       E.T_feat = S->T;
@@ -518,24 +517,6 @@ int enqueue_step(sld_dev* d, double dt, int scheme, double phi, int tn, KernelTi
       int rc = sma_mlp_infer_listed(S->mlp, S->feat, S->count, S->seg_cap, S->Y, s);
       TEND();
       if (rc) return rc;
-      sld::SurCheckArgs C{};
-      C.count = S->count;
-      C.seg_cap = S->seg_cap;
-      C.gp_of_slot = S->gp;
-      C.aux = S->aux;
-      C.feat = S->feat;
-      C.Y = S->Y;
-      for (int k = 0; k < 3; ++k) C.sdv_n[k] = d->sdv[tn][k];
-      C.props = d->props;
-      C.mat_of_elem = d->mat_of_elem;
-      C.E_of_elem = d->E_of_elem;
-      C.status = S->status;
-      C.x_out = S->x;
-      C.tol = S->tol;
-      TBEGIN(5);
-      hipLaunchKernelGGL(sld::sld_sur_check, dim3(std::min<int64_t>(nblocks(S->cap, 256), 1024)),
-                         dim3(256), 0, s, C);
-      TEND();
       E.sur_cand = nullptr;
       E.sur_cand_next = S->cand[1 - tn];
     }
@@ -963,9 +944,8 @@ int sld_dev_set_surrogate(sld_dev* d, sma_mlp* mlp, double tol) {
     S->cap = S->seg_cap * mlp::LIST_SEGS;
     const int64_t cap = S->cap, nctr = mlp::LIST_SEGS * mlp::LIST_STRIDE;
     int rc = 0;
-    if ((rc = dalloc(&S->status, ngp)) || (rc = dalloc(&S->x, ngp)) ||
-        (rc = dalloc(&S->feat, cap * 16)) || (rc = dalloc(&S->gp, cap)) ||
-        (rc = dalloc(&S->aux, cap * 2)) || (rc = dalloc(&S->count, nctr)) ||
+    if ((rc = dalloc(&S->status, ngp)) || (rc = dalloc(&S->slot, ngp)) ||
+        (rc = dalloc(&S->feat, cap * 16)) || (rc = dalloc(&S->count, nctr)) ||
         (rc = dalloc(&S->Y, cap * 12)) || (rc = dalloc(&S->cand[0], d->nelem)) ||
         (rc = dalloc(&S->cand[1], d->nelem))) {
       surrogate_detach(d);
@@ -1262,8 +1242,8 @@ int sld_dev_time_kernels(sld_dev* d, double dt, int32_t scheme, double phi, int3
     void* p;
     ~FreeScratch() { (void)hipFree(p); }
   } free_scratch{T.scratch_counters};
-  double sum[6] = {0, 0, 0, 0, 0, 0};
-  int64_t cnt[6] = {0, 0, 0, 0, 0, 0};
+  double sum[5] = {0, 0, 0, 0, 0};
+  int64_t cnt[5] = {0, 0, 0, 0, 0};
   hipEvent_t t0, t1;
   HIPCHK(hipEventCreate(&t0));
   HIPCHK(hipEventCreate(&t1));
@@ -1301,9 +1281,9 @@ int sld_dev_time_kernels(sld_dev* d, double dt, int32_t scheme, double phi, int3
   const double per_step = d->assembly == SLD_ASSEMBLY_GATHER
                               ? 1.0 : (double)T.element_launches / nsteps;
   out[3] = per_step;
-  // surrogate prepass, inference and check, ms per launch (0 in the other modes)
-  for (int k = 3; k < 6; ++k) out[k + 3] = cnt[k] ? sum[k] / cnt[k] : 0.0;
-  out[4] = out[0] + out[2] + out[6] + out[7] + out[8] + out[1] * per_step;  // kernel ms per step
+  // surrogate prepass and inference, ms per launch (0 in the other modes)
+  for (int k = 3; k < 5; ++k) out[k + 3] = cnt[k] ? sum[k] / cnt[k] : 0.0;
+  out[4] = out[0] + out[2] + out[6] + out[7] + out[1] * per_step;  // kernel ms per step
   out[5] = (double)tot / nsteps;  // eager wall ms per step on the stream, timing repeats included
   return SMA_OK;
 }

@@ -708,13 +708,14 @@ struct ElemArgs {
   void* edamp;                  // [slot][3] (double or float)
   int32_t xcd_remap;            // 1 = spread consecutive tiles per XCD
   int32_t tiles_per_block;      // > 0: persistent launch, tiles per workgroup
-  // surrogate (mode 2): per-GP status (0 exact, 1 accepted, 2 fallback) and
-  // accepted xS; prepass compact list of flagged points
+  // surrogate (mode 2): per-GP status from the prepass (0 not flagged,
+  // 1 listed, 2 flagged and not listed), the listed points' slots, their
+  // features and the network's outputs by slot
   uint8_t* sur_status;
-  const double* sur_x;
+  int32_t* sur_slot;            // [gp], valid where status is 1
   float* sur_feat;              // [cap][16]
-  int32_t* sur_gp;              // [cap]
-  double* sur_aux;              // [cap][2] theta, e_n
+  const float* sur_Y;           // [cap][12]
+  double sur_tol;               // acceptance tolerance on xS
   unsigned* sur_count;          // the list's segment counters (mlp::LIST_SEGS)
   int64_t sur_seg_cap;          // slots per segment
   double T_feat;                // temperature fed to the surrogate (inert in the law)
@@ -733,6 +734,35 @@ struct ElemArgs {
 constexpr int COUNTER_SLOTS = 256;
 
 // Wave-level flag counters: one 64-bit atomic per wave and per set bit type.
+// Surrogate mode, one Gauss point: a listed point takes the network's xS
+// when it lies within tol of the return map's solution (|R(x)/R'(x)| of the
+// single-branch consistency residual, exact because that residual is linear
+// in the increment on one branch) and in [-1e-3, 1 + 1e-3]; every other
+// point takes the exact map.  Flags: FLAG_SURR_USED on accepted transforming
+// points, FLAG_SURR_FALLBACK on flagged points that took the exact map.
+__device__ inline sma::ReturnMap surrogate_or_exact(const ElemArgs& A, const sma::Mat& m, int64_t gp,
+                                                   double theta, double e_n, double xS0,
+                                                   double FAS0, double FSA0) {
+  const uint8_t st = A.sur_status[gp];
+  bool ok = false;
+  double x = 0.0;
+  if (st == 1) {
+    const int64_t slot = A.sur_slot[gp];
+    x = xS0 + (double)A.sur_Y[slot * mlp::NOUT + 11] / mlp::Scale::dx;
+    ok = (x >= -1e-3) && (x <= 1.0 + 1e-3) &&
+         sma::consistency_residual(m, theta, e_n, xS0, FAS0, FSA0, x) <= A.sur_tol;
+  }
+  sma::ReturnMap rm;
+  if (ok) {
+    rm = sma::return_map_given(m, theta, e_n, xS0, FAS0, FSA0, x);
+    if (rm.flags & 3u) rm.flags |= sma::FLAG_SURR_USED;
+  } else {
+    rm = sma::return_map(m, theta, e_n, xS0, FAS0, FSA0);
+    if (st != 0 && (rm.flags & 3u)) rm.flags |= sma::FLAG_SURR_FALLBACK;
+  }
+  return rm;
+}
+
 __device__ __forceinline__ void count_flags(unsigned long long* ctr, uint32_t fl, bool valid) {
   if (__ballot(valid && fl != 0u) == 0ull) return;   // wave-uniform: nothing to count
   ctr += 8 * (blockIdx.x % COUNTER_SLOTS);
@@ -1077,10 +1107,21 @@ __global__ void __launch_bounds__(EPB * 8, (ElemOccupancy<R, LAWS, DAMP, STORE>:
         bool kf_AS, kf_SA;
         double FAS, FSA;
         const bool flagged = sma::flags_only(m, th, en, xS0, FAS0, FSA0, kf_AS, kf_SA, FAS, FSA);
-        // single-branch points go to the list: one atomic per wave, on the
-        // counter of the wave's segment (wave index mod LIST_SEGS; the wave
-        // covers elements 8 w .. 8 w + 7 of the launch)
-        const bool want = flagged && (kf_AS != kf_SA);
+        // single-branch points whose features lie in the trained range
+        // (|f| <= 8) go to the list
+        float f[mlp::NIN];
+        bool want = false;
+        if (flagged && kf_AS != kf_SA) {
+          if (hk.ok) sp = sma::spectral_fp32(Hf);
+          const double E = A.E_of_elem ? A.E_of_elem[e] : pr[0];
+          mlp::make_features(pr, E, A.T_feat, sp.e[0], sp.e[1], sp.e[2], th, xS0, FAS0, FSA0, FAS, f);
+          want = true;
+#pragma unroll
+          for (int k = 0; k < mlp::NIN; ++k) want = want && fabsf(f[k]) <= 8.0f;
+        }
+        // one atomic per wave, on the counter of the wave's segment (wave
+        // index mod LIST_SEGS; the wave covers elements 8 w .. 8 w + 7 of
+        // the launch)
         const unsigned long long wm = __ballot(want);
         const int lane = threadIdx.x & 63;
         const int leader = wm ? __ffsll((long long)wm) - 1 : 0;
@@ -1089,32 +1130,19 @@ __global__ void __launch_bounds__(EPB * 8, (ElemOccupancy<R, LAWS, DAMP, STORE>:
         if (wm && lane == leader)
           base = atomicAdd(A.sur_count + seg * mlp::LIST_STRIDE, (unsigned)__popcll(wm));
         base = __shfl(base, leader);
-        uint8_t st = 0;
-        if (flagged) {
-          st = 2;
-          if (want) {
-            if (hk.ok) sp = sma::spectral_fp32(Hf);
-            const int64_t slot = (int64_t)seg * A.sur_seg_cap + base +
-                                 (unsigned)__popcll(wm & ((1ull << lane) - 1ull));
-            const double E = A.E_of_elem ? A.E_of_elem[e] : pr[0];
-            mlp::make_features(pr, E, A.T_feat, sp.e[0], sp.e[1], sp.e[2], th, xS0, FAS0,
-                               FSA0, FAS, A.sur_feat + slot * mlp::NIN);
-            A.sur_gp[slot] = (int32_t)gp;
-            A.sur_aux[2 * slot] = th;
-            A.sur_aux[2 * slot + 1] = en;
-          }
+        if (want) {
+          const int64_t slot = (int64_t)seg * A.sur_seg_cap + base +
+                               (unsigned)__popcll(wm & ((1ull << lane) - 1ull));
+          float4* dst = reinterpret_cast<float4*>(A.sur_feat + slot * mlp::NIN);
+#pragma unroll
+          for (int k = 0; k < mlp::NIN / 4; ++k)
+            dst[k] = make_float4(f[4 * k], f[4 * k + 1], f[4 * k + 2], f[4 * k + 3]);
+          A.sur_slot[gp] = (int32_t)slot;
         }
-        A.sur_status[gp] = st;
+        A.sur_status[gp] = !flagged ? 0 : want ? 1 : 2;
       } else if (hk.ok) {
         if constexpr (SUR == 2) {
-          const uint8_t st = A.sur_status[gp];
-          if (st == 1) {
-            rm = sma::return_map_given(m, hk.theta, hk.e_n, xS0, FAS0, FSA0, A.sur_x[gp]);
-            if (rm.flags & 3u) rm.flags |= sma::FLAG_SURR_USED;
-          } else {
-            rm = sma::return_map(m, hk.theta, hk.e_n, xS0, FAS0, FSA0);
-            if (st == 2 && (rm.flags & 3u)) rm.flags |= sma::FLAG_SURR_FALLBACK;
-          }
+          rm = surrogate_or_exact(A, m, gp, hk.theta, hk.e_n, xS0, FAS0, FSA0);
         } else {
           rm = sma::return_map(m, hk.theta, hk.e_n, xS0, FAS0, FSA0);
         }
@@ -1122,14 +1150,7 @@ __global__ void __launch_bounds__(EPB * 8, (ElemOccupancy<R, LAWS, DAMP, STORE>:
       } else {
         sma::Spectral<float> sp = sma::spectral_fp32(Hf);
         if constexpr (SUR == 2) {
-          const uint8_t st = A.sur_status[gp];
-          if (st == 1) {
-            rm = sma::return_map_given(m, sp.theta, sp.e_n, xS0, FAS0, FSA0, A.sur_x[gp]);
-            if (rm.flags & 3u) rm.flags |= sma::FLAG_SURR_USED;
-          } else {
-            rm = sma::return_map(m, sp.theta, sp.e_n, xS0, FAS0, FSA0);
-            if (st == 2 && (rm.flags & 3u)) rm.flags |= sma::FLAG_SURR_FALLBACK;
-          }
+          rm = surrogate_or_exact(A, m, gp, sp.theta, sp.e_n, xS0, FAS0, FSA0);
         } else {
           rm = sma::return_map(m, sp.theta, sp.e_n, xS0, FAS0, FSA0);
         }
@@ -1367,52 +1388,6 @@ __global__ void __launch_bounds__(EPB * 8, (ElemOccupancy<R, LAWS, DAMP, STORE>:
   // the next tile's gather writes the LDS rows phase 3 has just read
   if (t_end - t_begin > 1) __syncthreads();
   }  // tile loop
-}
-
-// ----------------------------------------------------------------------------
-// Surrogate check on the compact list: accept the network's transformation
-// update when it lies within tol (in xS) of the return map's solution and
-// the inputs lie in the trained range; the element kernel uses accepted
-// values.
-// ----------------------------------------------------------------------------
-struct SurCheckArgs {
-  const unsigned* count;        // the list's segment counters
-  int64_t seg_cap;
-  const int32_t* gp_of_slot;
-  const double* aux;            // theta, e_n
-  const float* feat;            // [slot][16]
-  const float* Y;               // [slot][12] network outputs
-  const double* sdv_n[3];
-  const double* props;
-  const int32_t* mat_of_elem;
-  const double* E_of_elem;
-  uint8_t* status;
-  double* x_out;
-  double tol;
-};
-
-__device__ inline void check_slot(const SurCheckArgs& A, int64_t slot) {
-  const int64_t gp = A.gp_of_slot[slot];
-  const int64_t e = gp >> 3;
-  const int mat = A.mat_of_elem ? A.mat_of_elem[e] : 0;
-  const double* pr = A.props + 18 * mat;
-  const sma::Mat m = sma::make_mat(pr, A.E_of_elem ? A.E_of_elem[e] : pr[0]);
-  const double xS0 = A.sdv_n[0][gp], FAS0 = A.sdv_n[1][gp], FSA0 = A.sdv_n[2][gp];
-  const float* y = A.Y + slot * mlp::NOUT;
-  const double x = xS0 + (double)y[11] / mlp::Scale::dx;
-  bool in_range = true;
-  for (int k = 0; k < mlp::NIN; ++k) in_range &= fabsf(A.feat[slot * mlp::NIN + k]) <= 8.0f;
-  const double r = sma::consistency_residual(m, A.aux[2 * slot], A.aux[2 * slot + 1], xS0, FAS0, FSA0, x);
-  const bool ok = in_range && (r <= A.tol) && (x >= -1e-3) && (x <= 1.0 + 1e-3);
-  A.status[gp] = ok ? 1 : 2;
-  A.x_out[gp] = x;
-}
-
-__global__ void __launch_bounds__(256) sld_sur_check(SurCheckArgs A) {
-  __shared__ unsigned off[mlp::LIST_SEGS + 1];
-  const unsigned n = mlp::list_scan(A.count, off);
-  for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-    check_slot(A, mlp::list_slot(off, i, A.seg_cap));
 }
 
 // ----------------------------------------------------------------------------

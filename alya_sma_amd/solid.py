@@ -226,14 +226,13 @@ class Solidz:
 
     def time_kernels(self, dt, nsteps=4, scheme=0, phi=1.033):
         """HIP-event timing of the predictor, element phase and corrector
-        over nsteps eager steps (in surrogate mode also its prepass,
-        inference and check)."""
-        out = np.zeros(9)
+        over nsteps eager steps (in surrogate mode also its prepass and
+        inference)."""
+        out = np.zeros(8)
         check(lib().sld_dev_time_kernels(self.h, float(dt), int(scheme), float(phi), int(nsteps),
                                          _p(out)), "sld_dev_time_kernels")
         keys = ["predict_ms", "element_ms", "correct_ms", "element_launches_per_step",
-                "kernel_ms_per_step", "stream_ms_per_step", "sur_prepass_ms", "sur_infer_ms",
-                "sur_check_ms"]
+                "kernel_ms_per_step", "stream_ms_per_step", "sur_prepass_ms", "sur_infer_ms"]
         return dict(zip(keys, out.tolist()))
 
     def critical_dt(self):

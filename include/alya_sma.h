@@ -163,12 +163,12 @@ int sld_dev_step(sld_dev *dev, double dt, int32_t scheme, double phi, int64_t ns
  * brackets the same phases with its times(3..7) timers,
  * sld_element_operations.f90:125-256).  With the gather assembly the
  * element launch is repeated 8 times back to back inside the bracket (same
- * inputs, same outputs; flag counts of the repeats are discarded).  out[9]:
+ * inputs, same outputs; flag counts of the repeats are discarded).  out[8]:
  * ms per launch of the predictor, the element kernel (bracket / launches
  * in it) and the corrector, element launches per step, kernel ms per step,
  * eager stream ms per step (repeats included), and in surrogate mode ms per
- * launch of the prepass, the inference and the check (0 in the other
- * modes).  The steps are enqueued back
+ * launch of the prepass and of the inference (0 in the other modes).  The
+ * steps are enqueued back
  * to back; with nsteps > 1 the first bracket (opened on an idle stream) is
  * left out of the averages. */
 int sld_dev_time_kernels(sld_dev *dev, double dt, int32_t scheme, double phi,
