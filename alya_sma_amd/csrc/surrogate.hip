@@ -26,6 +26,7 @@ struct sma_mlp {
   float *gW = nullptr, *gb = nullptr;
   // padded copies read by the kernels
   __bf16* Wp = nullptr;
+  __bf16* Wi = nullptr;        // inference layout (mlp::inf_index)
   float* bp = nullptr;
   int64_t step = 0;
   // training workspace (grown on demand)
@@ -150,7 +151,8 @@ int sma_mlp_create(uint64_t seed, sma_mlp** out) {
       (rc = A(&m->mW, mlp::W_TOTAL * 4)) || (rc = A(&m->vW, mlp::W_TOTAL * 4)) ||
       (rc = A(&m->mb, mlp::B_TOTAL * 4)) || (rc = A(&m->vb, mlp::B_TOTAL * 4)) ||
       (rc = A(&m->gW, mlp::W_TOTAL * 4)) || (rc = A(&m->gb, mlp::B_TOTAL * 4)) ||
-      (rc = A(&m->Wp, mlp::WPAD_TOTAL * 2)) || (rc = A(&m->bp, mlp::BPAD_TOTAL * 4)) ||
+      (rc = A(&m->Wp, mlp::WPAD_TOTAL * 2)) || (rc = A(&m->Wi, mlp::WLDS_TOTAL * 2)) ||
+      (rc = A(&m->bp, mlp::BPAD_TOTAL * 4)) ||
       (rc = A(&m->gWp, mlp::WPAD_TOTAL * 4)) || (rc = A(&m->gbp, mlp::BPAD_TOTAL * 4)) ||
       (rc = A(&m->wloss, 16 * 4))) {
     sma_mlp_destroy(m);
@@ -179,7 +181,8 @@ int sma_mlp_create(uint64_t seed, sma_mlp** out) {
 int sma_mlp_destroy(sma_mlp* m) {
   if (!m) return SMA_OK;
   for (void* p : m->ws) (void)hipFree(p);
-  void* ps[] = {m->W, m->b, m->mW, m->vW, m->mb, m->vb, m->gW, m->gb, m->Wp, m->bp, m->gWp, m->gbp, m->wloss};
+  void* ps[] = {m->W, m->b, m->mW, m->vW, m->mb, m->vb, m->gW, m->gb, m->Wp, m->Wi, m->bp, m->gWp, m->gbp,
+                m->wloss};
   for (void* p : ps) if (p) (void)hipFree(p);
   delete m;
   return SMA_OK;
@@ -190,7 +193,7 @@ int sma_mlp_set_params(sma_mlp* m, const float* W, const float* b) {
   SCHK(hipMemcpy(m->W, W, mlp::W_TOTAL * 4, hipMemcpyHostToDevice));
   SCHK(hipMemcpy(m->b, b, mlp::B_TOTAL * 4, hipMemcpyHostToDevice));
   hipLaunchKernelGGL(mlp::pack_params, dim3(nb(mlp::WPAD_TOTAL, 256)), dim3(256), 0, nullptr, m->W,
-                     m->b, m->Wp, m->bp);
+                     m->b, m->Wp, m->Wi, m->bp);
   SCHK(hipGetLastError());
   SCHK(hipDeviceSynchronize());
   return SMA_OK;
@@ -222,13 +225,12 @@ int sma_mlp_forward_device(sma_mlp* m, int64_t n, const float* X, float* Y, void
                              (int)mlp::INF_SMEM));
     attr = true;
   }
-  const int64_t rows = mlp::INF_ROWS_PER_WAVE * mlp::INF_WAVES;
-  const int64_t ntile = (n + rows - 1) / rows;
+  const int64_t nblk = (n + mlp::INF_ROWS - 1) / mlp::INF_ROWS;
   int dev = 0, ncu = 256;
   (void)hipGetDevice(&dev);
   (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-  const unsigned grid = (unsigned)std::min<int64_t>(ntile, ncu);
-  hipLaunchKernelGGL(mlp::mlp_infer, dim3(grid), dim3(mlp::INF_THREADS), mlp::INF_SMEM, (hipStream_t)stream, m->Wp,
+  const unsigned grid = (unsigned)std::min<int64_t>(nblk, ncu);
+  hipLaunchKernelGGL(mlp::mlp_infer, dim3(grid), dim3(mlp::INF_THREADS), mlp::INF_SMEM, (hipStream_t)stream, m->Wi,
                      m->bp, X, n, (const unsigned*)nullptr, (int64_t)0, Y);
   SCHK(hipGetLastError());
   return SMA_OK;
@@ -247,7 +249,7 @@ int sma_mlp_train_step_device(sma_mlp* m, int64_t n, const float* X, const float
   int rc = forward_backward(m, n, X, T, m->gW, m->gb, loss_out, s);
   if (rc) return rc;
   m->step += 1;
-  mlp::AdamArgs A{m->W, m->b, m->mW, m->vW, m->mb, m->vb, m->gW, m->gb, m->Wp, m->bp,
+  mlp::AdamArgs A{m->W, m->b, m->mW, m->vW, m->mb, m->vb, m->gW, m->gb, m->Wp, m->bp, m->Wi,
                   (float)lr, 0.9f, 0.999f, 1e-8f,
                   (float)(1.0 - std::pow(0.9, (double)m->step)),
                   (float)(1.0 - std::pow(0.999, (double)m->step))};
@@ -283,10 +285,10 @@ int sma_mlp_infer_listed(sma_mlp* m, const float* X, const unsigned* seg_count, 
   int dev = 0, ncu = 256;
   (void)hipGetDevice(&dev);
   (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-  const int64_t rows = mlp::INF_ROWS_PER_WAVE * mlp::INF_WAVES;
   const int64_t cap = seg_cap * mlp::LIST_SEGS;
-  const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>((cap + rows - 1) / rows, ncu));
-  hipLaunchKernelGGL(mlp::mlp_infer, dim3(grid), dim3(mlp::INF_THREADS), mlp::INF_SMEM, s, m->Wp, m->bp, X,
+  const unsigned grid =
+      (unsigned)std::max<int64_t>(1, std::min<int64_t>((cap + mlp::INF_ROWS - 1) / mlp::INF_ROWS, ncu));
+  hipLaunchKernelGGL(mlp::mlp_infer, dim3(grid), dim3(mlp::INF_THREADS), mlp::INF_SMEM, s, m->Wi, m->bp, X,
                      (int64_t)0, seg_count, seg_cap, Y);
   SCHK(hipGetLastError());
   return 0;

@@ -49,6 +49,43 @@ __host__ __device__ constexpr int woff_c(int l) {
 constexpr int W_TOTAL = NH * NIN + 3 * NH * NH + NOUT * NH;      // 52736
 constexpr int B_TOTAL = 4 * NH + NOUT;                           // 524
 
+// output rows of a layer held in LDS (the output layer needs its first 16)
+__host__ __device__ constexpr int inf_n(int l) { return l == NLAYER - 1 ? 16 : NH; }
+static_assert(NOUT <= 16, "the output layer is one 16-feature block");
+// Inference layout of the weights (a device copy, kept by pack_params and
+// adam_step, that the inference kernel copies into LDS as it is).  Rows of
+// K + 8 BF16 (16 more bytes) per row.  An A fragment is
+// 16 rows x 16 bytes read at once; with rows of K = 128 (or 32) BF16 every
+// row starts in the same LDS bank group, and the 16 reads would queue on it.
+__host__ __device__ constexpr int wld(int l) { return layer_k(l) + 8; }
+__host__ __device__ constexpr int woff_lds(int l) {
+  return l == 0 ? 0 : woff_lds(l - 1) + inf_n(l - 1) * wld(l - 1);
+}
+constexpr int WLDS_TOTAL = woff_lds(NLAYER);
+constexpr int BLDS_TOTAL = (NLAYER - 1) * NH + 16;
+// input position, in a hidden layer's stored weight row, of the previous
+// layer's feature f = 16 b + 4 q + i: 32 (b >> 1) + 8 q + 4 (b & 1) + i
+__host__ __device__ constexpr int kperm(int f) {
+  return 32 * (f >> 5) + 8 * ((f >> 2) & 3) + 4 * ((f >> 4) & 1) + (f & 3);
+}
+
+constexpr bool kperm_is_bijection() {
+  bool seen[NH] = {};
+  for (int f = 0; f < NH; ++f) {
+    const int p = kperm(f);
+    if (p < 0 || p >= NH || seen[p]) return false;
+    seen[p] = true;
+  }
+  return true;
+}
+static_assert(kperm_is_bijection(), "kperm permutes the hidden inputs");
+
+// index of padded weight (l, n, k) in the inference layout; -1: not kept
+__host__ __device__ constexpr int inf_index(int l, int n, int k) {
+  return n < inf_n(l) ? woff_lds(l) + n * wld(l) + (l == 0 ? k : kperm(k)) : -1;
+}
+
+
 // z sigmoid(z) with the hardware reciprocal (1 ulp): the result is rounded to
 // BF16 (8 significant bits) before it is used
 __device__ __forceinline__ float silu(float z) {
@@ -223,6 +260,7 @@ struct AdamArgs {
   float* mW; float* vW; float* mb; float* vb;
   const float* gW; const float* gb;
   __bf16* Wp; float* bp;         // padded device copies
+  __bf16* Wi;                    // inference layout
   float lr, beta1, beta2, eps, bc1, bc2;
 };
 
@@ -242,6 +280,8 @@ __global__ void adam_step(AdamArgs A) {
     const int kr = layer_kreal(l), K = layer_k(l);
     const int j = i - woff_c(l), n = j / kr, k = j % kr;
     A.Wp[woff(l) + n * K + k] = (__bf16)A.W[i];
+    const int ii = inf_index(l, n, k);
+    if (ii >= 0) A.Wi[ii] = (__bf16)A.W[i];
   }
   if (i < B_TOTAL) {
     adam1(A.b[i], A.mb[i], A.vb[i], A.gb[i], A);
@@ -251,7 +291,7 @@ __global__ void adam_step(AdamArgs A) {
 
 // Canonical FP32 params -> padded device copies (pads zero).
 __global__ void pack_params(const float* __restrict__ W, const float* __restrict__ b,
-                            __bf16* __restrict__ Wp, float* __restrict__ bp) {
+                            __bf16* __restrict__ Wp, __bf16* __restrict__ Wi, float* __restrict__ bp) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < WPAD_TOTAL) {
     int l = 0;
@@ -260,36 +300,54 @@ __global__ void pack_params(const float* __restrict__ W, const float* __restrict
     float v = 0.f;
     if (n < layer_nreal(l) && k < layer_kreal(l)) v = W[woff_c(l) + n * layer_kreal(l) + k];
     Wp[i] = (__bf16)v;
+    const int ii = inf_index(l, n, k);
+    if (ii >= 0) Wi[ii] = (__bf16)v;
   }
   if (i < BPAD_TOTAL) bp[i] = (i < B_TOTAL) ? b[i] : 0.f;
 }
 
 // ----------------------------------------------------------------------------
-// Fused inference: all 5 layers per 128-row tile, weights staged in LDS once
-// per workgroup (persistent over row tiles), activations transposed through a
-// per-wave LDS tile between layers.  Y [n][12] fp32.
+// Fused inference: all 5 layers of a block of 32 rows in one wave, weights
+// staged in LDS once per workgroup.  Y [n][12] fp32.
 //
-// The weights (120 KB of BF16 with the row padding) leave room for one workgroup per CU, so the
-// workgroup carries 8 waves of 16 rows: two waves per SIMD, so that one
-// wave's SiLU epilogue (two quarter-rate transcendentals per activation)
-// overlaps the other's MFMA tiles.  Each B fragment read from LDS feeds one
-// 16x16x32 MFMA.
+// The product is formed transposed (weights as the A operand, activations
+// as the B operand of v_mfma_f32_16x16x32_bf16), so a layer leaves lane
+// (r, q) = (lane & 15, lane >> 4) holding features 16 b + 4 q + i (b = 0..7,
+// i = 0..3) of row r of each 16-row block.  The next layer's B fragment for
+// inputs 32 c .. 32 c + 31 wants, in lane (r, q), 8 inputs of row r: the
+// hidden layers' weight columns are stored permuted (kperm) so that these 8
+// are the lane's own features of blocks b = 2 c and 2 c + 1.  Activations
+// therefore stay in registers from layer to layer (no LDS round trip, no
+// barrier), and LDS holds only the weights: 121 KB, one workgroup per CU.
+// A wave carries two 16-row blocks (each A fragment read from LDS feeds two
+// MFMA) in 228 VGPRs, so the workgroup has 8 waves, two per SIMD: one
+// wave's SiLU epilogue (two quarter-rate transcendentals per activation,
+// the kernel's largest cost) overlaps the other's MFMA.  Waves take 32-row
+// blocks round robin over the grid (block k on workgroup k mod grid), so a
+// short list spreads over every CU.
 // ----------------------------------------------------------------------------
-constexpr int INF_ROWS_PER_WAVE = 16;
 constexpr int INF_WAVES = 8;
 constexpr int INF_THREADS = INF_WAVES * 64;
-constexpr int INF_RB = INF_ROWS_PER_WAVE / 16;   // 16-row MFMA blocks per wave
-constexpr int ACT_LD = NH + 8;   // padded row stride (bank spread)
-// Weight rows in LDS: K + 8 BF16 (16 more bytes) per row.  A B fragment is 16
-// rows x 16 bytes read at once; with rows of K = 128 (or 32) BF16 every row
-// starts in the same LDS bank group, and the 16 reads would queue on it.
-__host__ __device__ constexpr int wld(int l) { return layer_k(l) + 8; }
-__host__ __device__ constexpr int woff_lds(int l) {
-  return l == 0 ? 0 : woff_lds(l - 1) + layer_n(l - 1) * wld(l - 1);
+constexpr int INF_RB = 2;                        // 16-row MFMA blocks per wave block
+constexpr int INF_ROWS = 16 * INF_RB;            // rows per wave block
+__device__ __forceinline__ bf16x8 cat4(bf16x4 a, bf16x4 b) {
+  return bf16x8{a[0], a[1], a[2], a[3], b[0], b[1], b[2], b[3]};
 }
-constexpr int WLDS_TOTAL = woff_lds(NLAYER);
 
-__global__ void __launch_bounds__(INF_THREADS) mlp_infer(const __bf16* __restrict__ Wp,
+// bias + SiLU of one layer's accumulators into the next layer's inputs
+__device__ __forceinline__ void hidden_epilogue(const f32x4 (&acc)[INF_RB][8], const float* bias,
+                                                int q, bf16x4 (&h)[INF_RB][8]) {
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    const f32x4 bc = *reinterpret_cast<const f32x4*>(bias + 16 * b + 4 * q);
+#pragma unroll
+    for (int a = 0; a < INF_RB; ++a)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) h[a][b][i] = (__bf16)silu(acc[a][b][i] + bc[i]);
+  }
+}
+
+__global__ void __launch_bounds__(INF_THREADS) mlp_infer(const __bf16* __restrict__ Wi,
                                                          const float* __restrict__ bp,
                                                          const float* __restrict__ X, int64_t n_host,
                                                          const unsigned* __restrict__ seg_count,
@@ -299,103 +357,114 @@ __global__ void __launch_bounds__(INF_THREADS) mlp_infer(const __bf16* __restric
   // point's row is its slot
   __shared__ unsigned off[LIST_SEGS + 1];
   const int64_t n = seg_count ? (int64_t)list_scan(seg_count, off) : n_host;
-  const int64_t rows_per_block = INF_ROWS_PER_WAVE * INF_WAVES;
-  const int64_t ntile = (n + rows_per_block - 1) / rows_per_block;
-  // a workgroup without a tile leaves before copying the weights
-  if ((int64_t)blockIdx.x >= ntile) return;
+  const int64_t nblk = (n + INF_ROWS - 1) / INF_ROWS;
+  // a workgroup without a row block leaves before copying the weights
+  if ((int64_t)blockIdx.x >= nblk) return;
   extern __shared__ __align__(16) unsigned char smem[];
   __bf16* sW = reinterpret_cast<__bf16*>(smem);
   float* sb = reinterpret_cast<float*>(smem + WLDS_TOTAL * sizeof(__bf16));
-  __bf16* sAct = reinterpret_cast<__bf16*>(smem + WLDS_TOTAL * sizeof(__bf16) + BPAD_TOTAL * sizeof(float));
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  // packed rows of K BF16 -> LDS rows of K + 8, 16 bytes per copy
-  for (int i = tid; i < WPAD_TOTAL / 8; i += blockDim.x) {
-    int l = 0;
-    while (l < NLAYER - 1 && 8 * i >= woff(l + 1)) ++l;
-    const int j = 8 * i - woff(l), n = j / layer_k(l), k = j % layer_k(l);
-    *reinterpret_cast<bf16x8*>(sW + woff_lds(l) + n * wld(l) + k) =
-        reinterpret_cast<const bf16x8*>(Wp)[i];
-  }
-  for (int i = tid; i < BPAD_TOTAL; i += blockDim.x) sb[i] = bp[i];
+  const int tid = threadIdx.x;
+  // the inference layout as it is, 16 bytes per copy
+  static_assert(WLDS_TOTAL % 8 == 0, "whole 16-byte chunks");
+  for (int t = tid; t < WLDS_TOTAL / 8; t += blockDim.x)
+    reinterpret_cast<bf16x8*>(sW)[t] = reinterpret_cast<const bf16x8*>(Wi)[t];
+  for (int i = tid; i < BLDS_TOTAL; i += blockDim.x) sb[i] = bp[i];
   __syncthreads();
-  __bf16* act = sAct + wave * INF_ROWS_PER_WAVE * ACT_LD;
-  const int r = lane & 15, kq = 8 * (lane >> 4), rq = 4 * (lane >> 4);
-  static_assert(INF_ROWS_PER_WAVE <= 64, "one lane holds the slot of each row of a wave");
-  for (int64_t t = blockIdx.x; t < ntile; t += gridDim.x) {
-    const int64_t row0 = t * rows_per_block + wave * INF_ROWS_PER_WAVE;
-    // lane l < INF_ROWS_PER_WAVE: the slot of row row0 + l
+  const int lane = tid & 63, wave = tid >> 6;
+  const int r = lane & 15, q = lane >> 4;
+  const int64_t nwaves = (int64_t)gridDim.x * INF_WAVES;
+  static_assert(INF_ROWS <= 64, "one lane holds the slot of each row of a block");
+  for (int64_t blk = (int64_t)wave * gridDim.x + blockIdx.x; blk < nblk; blk += nwaves) {
+    const int64_t row0 = blk * INF_ROWS;
+    // lane l < INF_ROWS: the slot of row row0 + l
     int slot_l = 0;
-    if (lane < INF_ROWS_PER_WAVE && row0 + lane < n)
+    if (lane < INF_ROWS && row0 + lane < n)
       slot_l = seg_count ? (int)list_slot(off, (unsigned)(row0 + lane), seg_cap) : (int)(row0 + lane);
-    // input tile -> act[:, 0:32]; every activation row belongs to one wave,
-    // which alone writes and reads it, so no barrier separates the layers
-    for (int i = lane; i < INF_ROWS_PER_WAVE * KP0; i += 64) {
-      const int rr = i / KP0, k = i % KP0;
-      const int64_t m = row0 + rr;
-      const int64_t sl = __shfl(slot_l, rr);
-      act[rr * ACT_LD + k] = (__bf16)((m < n && k < NIN) ? X[sl * NIN + k] : 0.f);
+    // layer 0's B fragments: inputs 8 q .. 8 q + 7 of row 16 a + r (NIN = 16
+    // real inputs; lanes q >= 2 hold the zero padding)
+    bf16x8 x0[INF_RB];
+#pragma unroll
+    for (int a = 0; a < INF_RB; ++a) {
+      const int64_t sl = __shfl(slot_l, 16 * a + r);
+      float4 lo = make_float4(0.f, 0.f, 0.f, 0.f), hi = lo;
+      if (q < 2 && row0 + 16 * a + r < n) {
+        const float4* src = reinterpret_cast<const float4*>(X + sl * NIN + 8 * q);
+        lo = src[0];
+        hi = src[1];
+      }
+      x0[a] = bf16x8{(__bf16)lo.x, (__bf16)lo.y, (__bf16)lo.z, (__bf16)lo.w,
+                     (__bf16)hi.x, (__bf16)hi.y, (__bf16)hi.z, (__bf16)hi.w};
     }
+    bf16x4 h[INF_RB][8];
+    f32x4 acc[INF_RB][8];
+    // layer 0 (K = 32)
+#pragma unroll
+    for (int a = 0; a < INF_RB; ++a)
+#pragma unroll
+      for (int b = 0; b < 8; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const bf16x8 wf = *reinterpret_cast<const bf16x8*>(sW + (16 * b + r) * wld(0) + 8 * q);
+#pragma unroll
+      for (int a = 0; a < INF_RB; ++a)
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf, x0[a], acc[a][b], 0, 0, 0);
+    }
+    hidden_epilogue(acc, sb + boff(0), q, h);
+    // hidden layers 1 .. NLAYER - 2 (K = 128)
 #pragma unroll 1
-    for (int l = 0; l < NLAYER; ++l) {
-      const int K = layer_k(l), N = layer_n(l);
-      const __bf16* W = sW + woff_lds(l);
-      const int LDW = wld(l);
-      f32x4 acc[INF_RB][8];
+    for (int l = 1; l < NLAYER - 1; ++l) {
+      const __bf16* W = sW + woff_lds(1) + (l - 1) * NH * wld(1);
 #pragma unroll
       for (int a = 0; a < INF_RB; ++a)
 #pragma unroll
         for (int b = 0; b < 8; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
-      // transposed product: A = the weight rows (16 output features x 32
-      // inputs), B = the activation rows (32 inputs x 16 rows), so that the
-      // result gives each lane 4 consecutive features of one row (features
-      // 16 b + rq .. + 3 of row 16 a + r), stored as one 8-byte write
-      for (int k0 = 0; k0 < K; k0 += 32) {
+#pragma unroll
+      for (int c = 0; c < NH / 32; ++c) {
         bf16x8 xf[INF_RB];
 #pragma unroll
-        for (int a = 0; a < INF_RB; ++a)
-          xf[a] = *reinterpret_cast<const bf16x8*>(act + (16 * a + r) * ACT_LD + k0 + kq);
+        for (int a = 0; a < INF_RB; ++a) xf[a] = cat4(h[a][2 * c], h[a][2 * c + 1]);
 #pragma unroll
         for (int b = 0; b < 8; ++b) {
-          if (16 * b >= N) break;
-          const bf16x8 wf = *reinterpret_cast<const bf16x8*>(W + (16 * b + r) * LDW + k0 + kq);
+          const bf16x8 wf =
+              *reinterpret_cast<const bf16x8*>(W + (16 * b + r) * wld(1) + 32 * c + 8 * q);
 #pragma unroll
           for (int a = 0; a < INF_RB; ++a)
             acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf, xf[a], acc[a][b], 0, 0, 0);
         }
       }
-      // a wave reads and writes only its own activation rows: the wave's
-      // own ordering (its LDS reads above precede its writes below) suffices
-      const float* bias = sb + boff(l);
-      if (l < NLAYER - 1) {
+      hidden_epilogue(acc, sb + boff(l), q, h);
+    }
+    // output layer: one 16-feature block, Y = Z (fp32)
+    {
+      constexpr int l = NLAYER - 1;
+      const __bf16* W = sW + woff_lds(l);
+      f32x4 o[INF_RB];
+#pragma unroll
+      for (int a = 0; a < INF_RB; ++a) o[a] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int c = 0; c < NH / 32; ++c) {
+        const bf16x8 wf = *reinterpret_cast<const bf16x8*>(W + r * wld(l) + 32 * c + 8 * q);
 #pragma unroll
         for (int a = 0; a < INF_RB; ++a)
+          o[a] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf, cat4(h[a][2 * c], h[a][2 * c + 1]),
+                                                         o[a], 0, 0, 0);
+      }
+      const float* bias = sb + boff(l);
 #pragma unroll
-          for (int b = 0; b < 8; ++b) {
-            const int col = 16 * b + rq;
-            const f32x4 bc = *reinterpret_cast<const f32x4*>(bias + col);
-            bf16x4 o;
+      for (int a = 0; a < INF_RB; ++a) {
+        const int64_t m = row0 + 16 * a + r;
+        const int64_t sl = __shfl(slot_l, 16 * a + r);
 #pragma unroll
-            for (int i = 0; i < 4; ++i) o[i] = (__bf16)silu(acc[a][b][i] + bc[i]);
-            *reinterpret_cast<bf16x4*>(act + (16 * a + r) * ACT_LD + col) = o;
-          }
-      } else {
-#pragma unroll
-        for (int a = 0; a < INF_RB; ++a) {
-          const int64_t m = row0 + 16 * a + r;
-          const int64_t sl = __shfl(slot_l, 16 * a + r);
-#pragma unroll
-          for (int i = 0; i < 4; ++i) {
-            const int col = rq + i;
-            if (m < n && col < NOUT) Y[sl * NOUT + col] = acc[a][0][i] + bias[col];
-          }
+        for (int i = 0; i < 4; ++i) {
+          const int col = 4 * q + i;
+          if (m < n && col < NOUT) Y[sl * NOUT + col] = o[a][i] + bias[col];
         }
       }
     }
   }
 }
 
-constexpr size_t INF_SMEM = WLDS_TOTAL * sizeof(__bf16) + BPAD_TOTAL * sizeof(float) +
-                            INF_WAVES * INF_ROWS_PER_WAVE * ACT_LD * sizeof(__bf16);
+constexpr size_t INF_SMEM = WLDS_TOTAL * sizeof(__bf16) + BLDS_TOTAL * sizeof(float);
 
 // splitmix64 counter-based generator
 __device__ __forceinline__ uint64_t smix(uint64_t x) {

@@ -36,6 +36,21 @@ def test_forward_matches_oracle(sur):
     assert _rel(Yg, Yf) < 5e-2
 
 
+def test_forward_large_batch_matches_oracle(sur):
+    """More rows than one pass of the grid's waves (256 workgroups x 8 waves
+    x 32 rows), and a ragged last block."""
+    print("mlp forward vs oracle, 140001 rows", flush=True)
+    rng = np.random.default_rng(1)
+    X = rng.normal(size=(140001, 16)).astype(np.float32)
+    W, b = sur.get_params()
+    Yg = sur.forward(DeviceArray.from_host(X)).to_host()
+    Yo, _ = M.forward(W, b, X, emulate=True)
+    assert _rel(Yg, Yo) < 5e-3, _rel(Yg, Yo)
+    # every row is its own: rows agree wherever they sit in the batch
+    Ys = sur.forward(DeviceArray.from_host(X[-77:])).to_host()
+    np.testing.assert_array_equal(Ys, Yg[-77:])
+
+
 def test_gradient_matches_oracle(sur):
     print("mlp gradient vs oracle", flush=True)
     rng = np.random.default_rng(1)
