@@ -201,7 +201,7 @@ struct NodalArgs {
   int32_t slot_width;          // gather with W slots per node (node i: W i ..), or 0
   const void* efor;            // gather: corner forces [slot][3] (double or float),
                                // slots of a node in ascending element order
-  unsigned* sur_count;         // surrogate compact-list counter, cleared per step
+  unsigned* sur_count;         // surrogate list segment counters, cleared per step
   // energy balance (mod_sld_energy.f90), when du is non-null: the predictor
   // stores du = u^{n+1} - u^n, the corrector writes per-block partial sums of
   // (f_int^{n+1} + f_int^n).du, (f_ext^{n+1} + f_ext^n).du and m v.v
@@ -515,9 +515,6 @@ __device__ __forceinline__ void correct_node(const NodalArgs& A, int64_t i, doub
   }
   if (i == 0) {
     A.clock[1 - A.tn] = cutim;
-    // the next step's list
-    if (A.sur_count)
-      for (int k = 0; k < mlp::LIST_SEGS; ++k) A.sur_count[k * mlp::LIST_STRIDE] = 0u;
   }
 }
 
@@ -548,6 +545,10 @@ __global__ void __launch_bounds__(256) sld_correct(NodalArgs A, double dt, doubl
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double e[3] = {0.0, 0.0, 0.0};
   if (i < A.npoin) correct_node<SCHEME, FUSE>(A, i, dt, phi, e);
+  // the next step's surrogate list starts empty
+  if (A.sur_count)
+    for (int64_t k = i; k < mlp::LIST_SEGS; k += (int64_t)gridDim.x * blockDim.x)
+      A.sur_count[k * mlp::LIST_STRIDE] = 0u;
   if (A.energy_part) block_sum3(e, A.energy_part);
 }
 

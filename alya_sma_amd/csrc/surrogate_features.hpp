@@ -65,16 +65,28 @@ __host__ __device__ constexpr int64_t list_seg_cap(int64_t nwaves) {
 // (off[LIST_SEGS] = total), formed by the block's first wave; every thread
 // of the block must call it (it ends with a barrier).
 __device__ inline unsigned list_scan(const unsigned* count, unsigned* off) {
+  static_assert(LIST_SEGS % 64 == 0, "whole lanes of segments");
+  constexpr int PER = LIST_SEGS / 64;   // segments per lane
   if (threadIdx.x < 64) {
     const int l = threadIdx.x;
-    const unsigned c = count[l * LIST_STRIDE];
-    unsigned v = c;
+    unsigned c[PER], sum = 0;
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      c[j] = count[(l * PER + j) * LIST_STRIDE];
+      sum += c[j];
+    }
+    unsigned v = sum;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
       const unsigned u = __shfl_up(v, d);
       if (l >= d) v += u;
     }
-    off[l] = v - c;
+    unsigned o = v - sum;
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      off[l * PER + j] = o;
+      o += c[j];
+    }
     if (l == 63) off[LIST_SEGS] = v;
   }
   __syncthreads();
