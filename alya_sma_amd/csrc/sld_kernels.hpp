@@ -1094,8 +1094,7 @@ __global__ void __launch_bounds__(EPB * 8, (ElemOccupancy<R, LAWS, DAMP, STORE>:
       for (int q = 0; q < 9; ++q) Hf[q] = H[q];
       // FP32: Hencky strain tensor by series (theta and |e'| are all the
       // return map and the stress need); the spectral form serves stretches
-      // beyond the series' range and, in the surrogate prepass, the listed
-      // points' principal strains (the network's features)
+      // beyond the series' range
       const sma::Hencky32 hk = sma::hencky_fp32(Hf);
       if constexpr (SUR == 1) {
         double th = hk.theta, en = hk.e_n;
@@ -1113,7 +1112,9 @@ __global__ void __launch_bounds__(EPB * 8, (ElemOccupancy<R, LAWS, DAMP, STORE>:
         float f[mlp::NIN];
         bool want = false;
         if (flagged && kf_AS != kf_SA) {
-          if (hk.ok) sp = sma::spectral_fp32(Hf);
+          // the features' principal deviatoric strains: in closed form from
+          // the series' e' (the spectral path has them already)
+          if (hk.ok) sma::principal_dev_fp32(hk.ed, sp.e[0], sp.e[1], sp.e[2]);
           const double E = A.E_of_elem ? A.E_of_elem[e] : pr[0];
           mlp::make_features(pr, E, A.T_feat, sp.e[0], sp.e[1], sp.e[2], th, xS0, FAS0, FSA0, FAS, f);
           want = true;

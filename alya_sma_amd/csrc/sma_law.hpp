@@ -372,6 +372,34 @@ struct Hencky32 {
   bool ok;             // false: |Z|_F > 0.6, use the spectral path
 };
 
+// Principal values, descending, of a symmetric deviatoric 3x3 tensor
+// s = (00, 01, 02, 11, 12, 22), in closed form: the roots of the
+// characteristic polynomial l^3 - J2 l - J3 = 0 by the trigonometric
+// solution, l_k = 2 r cos(a - 2 pi k / 3) with r = sqrt(J2 / 3),
+// cos 3a = J3 / (2 r^3), a in [0, pi / 3] (k = 0, 1, 2 give the descending
+// order).  Any trace left by rounding is removed first.  Below J2 = 1e-24
+// (|s| ~ 1e-12) the values are returned as zero.
+__host__ __device__ __forceinline__ void principal_dev_fp32(const float (&s)[6], float& l0, float& l1,
+                                                           float& l2) {
+  const float m = (s[0] + s[3] + s[5]) * (1.0f / 3.0f);
+  const float d0 = s[0] - m, d1 = s[3] - m, d2 = s[5] - m;
+  const float J2 = 0.5f * (d0 * d0 + d1 * d1 + d2 * d2) + s[1] * s[1] + s[2] * s[2] + s[4] * s[4];
+  if (!(J2 > 1e-24f)) {
+    l0 = l1 = l2 = 0.f;
+    return;
+  }
+  const float J3 = d0 * (d1 * d2 - s[4] * s[4]) - s[1] * (s[1] * d2 - s[4] * s[2]) +
+                   s[2] * (s[1] * s[4] - d1 * s[2]);
+  const float r = sqrtf(J2 * (1.0f / 3.0f));
+  const float c3 = fminf(1.0f, fmaxf(-1.0f, J3 / (2.0f * r * r * r)));
+  const float a = acosf(c3) * (1.0f / 3.0f);
+  const float c = cosf(a), sn = sinf(a);
+  const float h = 1.7320508075688772f;   // 2 sin(2 pi / 3)
+  l0 = 2.0f * r * c;
+  l1 = r * (h * sn - c);
+  l2 = -r * (c + h * sn);
+}
+
 // c = a b for symmetric a, b that commute (the product is symmetric).
 __host__ __device__ __forceinline__ void sym_mul_commuting(const float (&a)[6], const float (&b)[6],
                                                           float (&c)[6]) {
