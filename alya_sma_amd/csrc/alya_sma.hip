@@ -1679,6 +1679,35 @@ int sld_dev_destroy(sld_dev* d) {
   return SMA_OK;
 }
 
+// Staging of sma160_eval, one per host thread: a device block and a pinned
+// host block, grown on demand and kept for the thread's life, so a call
+// (the reference evaluates the law once per element, 8 points) costs one
+// upload, one launch and one download instead of allocations per array.
+struct EvalStage {
+  void* dev = nullptr;
+  void* host = nullptr;
+  size_t bytes = 0;
+  int reserve(size_t want) {
+    if (want <= bytes) return SMA_OK;
+    size_t nb = bytes ? bytes : 4096;
+    while (nb < want) nb *= 2;
+    if (dev) (void)hipFree(dev);
+    if (host) (void)hipHostFree(host);
+    dev = host = nullptr;
+    bytes = 0;
+    HIPCHK(hipMalloc(&dev, nb));
+    HIPCHK(hipHostMalloc(&host, nb, hipHostMallocDefault));
+    bytes = nb;
+    return SMA_OK;
+  }
+};
+// the blocks are left to the process's end (no destructor: freeing after
+// the runtime has shut down is not safe)
+EvalStage& eval_stage() {
+  static thread_local EvalStage* st = new EvalStage();
+  return *st;
+}
+
 static int law_launch(int64_t n, int32_t nmat, const double* props, const int32_t* mat, const double* E,
                       const double* F, const double* sdv_old, double* sdv_new, double* S,
                       uint32_t* flags, int32_t precision, hipStream_t s) {
@@ -1716,34 +1745,37 @@ int sma160_eval(int64_t n, int32_t n_mat, const double* props, const int32_t* ma
     for (int64_t i = 0; i < n; ++i)
       if (mat_of_gp[i] < 0 || mat_of_gp[i] >= n_mat)
         return fail(SMA_ERR_ARG, "sma160_eval: material index out of range");
-  double *dp = nullptr, *dE = nullptr, *dF = nullptr, *dso = nullptr, *dsn = nullptr, *dS = nullptr;
-  int32_t* dm = nullptr;
-  uint32_t* dfl = nullptr;
-  int rc = 0;
-  auto cleanup = [&]() {
-    hipFree(dp); hipFree(dE); hipFree(dF); hipFree(dso); hipFree(dsn); hipFree(dS);
-    hipFree(dm); hipFree(dfl);
+  // inputs, then outputs, each section 8-byte aligned, in one block
+  const size_t b_props = (size_t)n_mat * 18 * 8, b_mat = mat_of_gp ? (size_t)((n * 4 + 7) / 8 * 8) : 0,
+               b_E = E_of_gp ? (size_t)n * 8 : 0, b_F = (size_t)n * 72, b_so = (size_t)n * 24;
+  const size_t b_sn = (size_t)n * 24, b_S = (size_t)n * 72, b_fl = flags ? (size_t)((n * 4 + 7) / 8 * 8) : 0;
+  const size_t in_bytes = b_props + b_mat + b_E + b_F + b_so, out_bytes = b_sn + b_S + b_fl;
+  EvalStage& st = eval_stage();
+  if (int rc = st.reserve(in_bytes + out_bytes)) return rc;
+  char* h = static_cast<char*>(st.host);
+  char* dv = static_cast<char*>(st.dev);
+  size_t o = 0;
+  auto put = [&](const void* src, size_t bytes, size_t span) -> size_t {
+    if (src) std::memcpy(h + o, src, bytes);
+    const size_t at = o;
+    o += span;
+    return at;
   };
-  do {
-    if ((rc = upload(&dp, props, (size_t)n_mat * 18, nullptr))) break;
-    if (mat_of_gp && (rc = upload(&dm, mat_of_gp, (size_t)n, nullptr))) break;
-    if (E_of_gp && (rc = upload(&dE, E_of_gp, (size_t)n, nullptr))) break;
-    if ((rc = upload(&dF, F, (size_t)9 * n, nullptr))) break;
-    if ((rc = upload(&dso, sdv_old, (size_t)3 * n, nullptr))) break;
-    if ((rc = dalloc(&dsn, (size_t)3 * n))) break;
-    if ((rc = dalloc(&dS, (size_t)9 * n))) break;
-    if (flags && (rc = dalloc(&dfl, (size_t)n))) break;
-    if ((rc = law_launch(n, n_mat, dp, dm, dE, dF, dso, dsn, dS, dfl, precision, nullptr))) break;
-    hipError_t e;
-    if ((e = hipMemcpy(sdv_new, dsn, 3 * n * sizeof(double), hipMemcpyDeviceToHost)) != hipSuccess ||
-        (e = hipMemcpy(S, dS, 9 * n * sizeof(double), hipMemcpyDeviceToHost)) != hipSuccess ||
-        (flags && (e = hipMemcpy(flags, dfl, n * sizeof(uint32_t), hipMemcpyDeviceToHost)) != hipSuccess)) {
-      rc = fail(SMA_ERR_HIP, std::string("sma160_eval copy-back: ") + hipGetErrorString(e));
-      break;
-    }
-  } while (0);
-  cleanup();
-  return rc;
+  const size_t o_p = put(props, b_props, b_props), o_m = put(mat_of_gp, (size_t)n * 4, b_mat);
+  const size_t o_E = put(E_of_gp, b_E, b_E), o_F = put(F, b_F, b_F), o_so = put(sdv_old, b_so, b_so);
+  const size_t o_sn = in_bytes, o_S = o_sn + b_sn, o_fl = o_S + b_S;
+  HIPCHK(hipMemcpy(dv, h, in_bytes, hipMemcpyHostToDevice));
+  auto D = [&](size_t off, size_t bytes) -> void* { return bytes ? dv + off : nullptr; };
+  if (int rc = law_launch(n, n_mat, (const double*)D(o_p, b_props), (const int32_t*)D(o_m, b_mat),
+                          (const double*)D(o_E, b_E), (const double*)D(o_F, b_F),
+                          (const double*)D(o_so, b_so), (double*)D(o_sn, b_sn), (double*)D(o_S, b_S),
+                          (uint32_t*)D(o_fl, b_fl), precision, nullptr))
+    return rc;
+  HIPCHK(hipMemcpy(h + in_bytes, dv + in_bytes, out_bytes, hipMemcpyDeviceToHost));
+  std::memcpy(sdv_new, h + o_sn, b_sn);
+  std::memcpy(S, h + o_S, b_S);
+  if (flags) std::memcpy(flags, h + o_fl, (size_t)n * 4);
+  return SMA_OK;
 }
 
 }  // extern "C"

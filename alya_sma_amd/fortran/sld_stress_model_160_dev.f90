@@ -29,8 +29,11 @@ contains
     integer,  intent(in)            :: flagt
     ! gptmp is accepted and unused (the reference hard-codes gp_tmp = 0, :105);
     ! gpdds is only written by the implicit tangent path, which is not served
-    real(c_double), allocatable, target :: F(:,:), sdv_old(:,:), sdv_new(:,:), S(:,:), pr(:)
-    integer(c_int32_t), allocatable, target :: flags(:)
+    ! per-call staging on the stack (pgaus is an element's Gauss points);
+    ! sma160_eval keeps its device and pinned host blocks between calls
+    real(c_double), target :: F(max(pgaus, 1), 9), sdv_old(max(pgaus, 1), 3), &
+                              sdv_new(max(pgaus, 1), 3), S(max(pgaus, 1), 9), pr(18)
+    integer(c_int32_t), target :: flags(max(pgaus, 1))
     integer :: g, i, j
     integer(c_int) :: rc
 
@@ -44,7 +47,6 @@ contains
     end if
     if (pgaus <= 0) return
     ! SoA: component k of point g at (g, k), k = i + 3*(j-1) (column-major 3x3)
-    allocate(F(pgaus, 9), sdv_old(pgaus, 3), sdv_new(pgaus, 3), S(pgaus, 9), pr(18), flags(pgaus))
     pr = props(1:18)
     do g = 1, pgaus
        do j = 1, 3

@@ -52,7 +52,9 @@ const char *sma_last_error(void);
  * F [9][n_gp] in; sdv_old [3][n_gp] = TIME_N in; sdv_new [3][n_gp] = ITER_K
  *   out; S [9][n_gp] second Piola-Kirchhoff out; flags [n_gp] out (may be NULL).
  * precision: 0 = FP64, 1 = FP32 spectral/stress with FP64 return map.
- * Host pointers; synchronous.
+ * Host pointers; synchronous.  Each host thread keeps one device block and
+ * one pinned host block, grown to the largest call and held to the process's
+ * end, so that repeated small calls (an element's points) allocate nothing.
  * ------------------------------------------------------------------------ */
 int sma160_eval(int64_t n_gp, int32_t n_mat, const double *props,
                 const int32_t *mat_of_gp, const double *E_of_gp,
