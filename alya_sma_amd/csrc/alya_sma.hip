@@ -317,14 +317,18 @@ void launch_element_laws(const sld_dev* d, const sld::ElemArgs& A, hipStream_t s
 template <typename R, int SUR, bool DAMP>
 void launch_element_damp(const sld_dev* d, const sld::ElemArgs& A, hipStream_t s) {
   const bool st = d->store != 0;
-  if constexpr (SUR == 0) {
+  if constexpr (SUR != 1) {
     if (d->nh_begin >= 0 && A.e_end > d->nh_begin) {
       // law-grouped mesh: the SMA range and the neo-Hookean range each with
-      // the kernel that has only its law
+      // the kernel that has only its law (surrogate mode: the SMA range in
+      // its surrogate kernel, the neo-Hookean range, which no network
+      // serves, in the plain one)
       sld::ElemArgs B = A;
       B.e_end = std::min(A.e_end, d->nh_begin);
-      if (st) launch_element_epb<R, true, 0, true, 1, DAMP>(d->epb, B, s, d->persistent);
-      else launch_element_epb<R, false, 0, true, 1, DAMP>(d->epb, B, s, d->persistent);
+      if (B.e_end > B.e_begin) {
+        if (st) launch_element_epb<R, true, SUR, true, 1, DAMP>(d->epb, B, s, d->persistent);
+        else launch_element_epb<R, false, SUR, true, 1, DAMP>(d->epb, B, s, d->persistent);
+      }
       B.e_begin = std::max(A.e_begin, d->nh_begin);
       B.e_end = A.e_end;
       if (st) launch_element_epb<R, true, 0, true, 2, DAMP>(d->epb, B, s, d->persistent);
@@ -507,11 +511,17 @@ int enqueue_step(sld_dev* d, double dt, int scheme, double phi, int tn, KernelTi
       E.sur_count = S->count;
       E.sur_seg_cap = S->seg_cap;
       E.T_feat = S->T;
+      // a law-grouped mesh: the SMA range only (its kernel without the
+      // neo-Hookean path)
       E.e_begin = 0;
-      E.e_end = (int32_t)d->nelem;
+      E.e_end = d->nh_begin >= 0 ? d->nh_begin : (int32_t)d->nelem;
       E.elem_list = nullptr;
       TBEGIN(3);
-      launch_element_laws<float, false, 1, false>(d, E, s);
+      if (d->nh_begin >= 0) {
+        if (E.e_end > 0) launch_element_epb<float, false, 1, false, 1>(d->epb, E, s, d->persistent);
+      } else {
+        launch_element_laws<float, false, 1, false>(d, E, s);
+      }
       TEND();
       TBEGIN(4);
       int rc = sma_mlp_infer_listed(S->mlp, S->feat, S->count, S->seg_cap, S->Y, s);

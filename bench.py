@@ -106,13 +106,13 @@ def kernel_name(mode, cfg, mixed=False):
     """The element kernel that carries the SMA elements.  Its law range: 1 =
     SMA only (every launch over a mesh without neo-Hookean elements, and the
     SMA range of a law-grouped mesh in the gather assembly, which a
-    neo-Hookean-only launch, range 2, follows); 0 = each element's own law
-    (a mixed mesh in the scatter assembly or in surrogate mode).  No Rayleigh
+    neo-Hookean-only launch, range 2, follows, in every mode); 0 = each
+    element's own law (a mixed mesh in the scatter assembly).  No Rayleigh
     damping."""
     r = "double" if mode == "fp64" else "float"
     sur = 2 if mode == "surrogate" else 0
     ga = "true" if cfg["assembly"] == "gather" else "false"
-    laws = 0 if mixed and (mode == "surrogate" or cfg["assembly"] != "gather") else 1
+    laws = 0 if mixed and cfg["assembly"] != "gather" else 1
     return f"sld_element<{r}, {cfg['elems_per_block']}, false, {sur}, {ga}, {laws}, false>"
 
 

@@ -56,9 +56,10 @@ def test_kernel_names_and_committed_traffic():
     cfg = {"assembly": "gather", "elems_per_block": 8}
     k32 = bench.kernel_name("fp32", cfg)
     assert k32 == "sld_element<float, 8, false, 0, true, 1, false>"
-    # each element's own law: a mixed mesh in surrogate mode or scattered
+    # each element's own law: a mixed mesh scattered; gathered (surrogate
+    # mode too) the SMA range runs the SMA-only kernel
     assert bench.kernel_name("surrogate", cfg, mixed=True) == \
-        "sld_element<float, 8, false, 2, true, 0, false>"
+        "sld_element<float, 8, false, 2, true, 1, false>"
     assert bench.kernel_name("fp64", {"assembly": "scatter", "elems_per_block": 16}, mixed=True) == \
         "sld_element<double, 16, false, 0, false, 0, false>"
     assert bench.kernel_name("fp64", cfg, mixed=True).endswith(", 1, false>")

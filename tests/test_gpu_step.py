@@ -426,12 +426,13 @@ def test_tube_in_tissue_scatter_runs_each_elements_law():
 
 
 def test_tube_in_tissue_surrogate_mode_one_step():
-    """Surrogate mode on a mixed mesh runs the per-element-law kernels (the
-    prepass that lists points for the network, and the step): neo-Hookean
-    elements take their law, SMA points the network's update where the check
-    accepts it and the exact map elsewhere.  One step from an FP64 state in
-    the crimp stays within 1e-3 (relative L2 stress) of the oracle's FP64
-    step, and the tissue keeps no history."""
+    """Surrogate mode on a mixed (law-grouped) mesh: the prepass covers the
+    SMA range, the step runs the SMA range in the surrogate kernel (the
+    network's update where the check accepts it, the exact map elsewhere)
+    and the tissue in the neo-Hookean-only kernel of the FP32 mode, so the
+    tissue's stresses equal an FP32-mode step's bitwise.  One step from an
+    FP64 state in the crimp stays within 1e-3 (relative L2 stress) of the
+    oracle's FP64 step, and the tissue keeps no history."""
     from alya_sma_amd import surrogate as S
     net = S.load_default()
     assert net is not None, "the committed network is missing"
@@ -439,9 +440,14 @@ def test_tube_in_tissue_surrogate_mode_one_step():
     dt = prob["run"]["dt"]
     print("tube in tissue: one surrogate step after 350 FP64 steps", flush=True)
     dev, orc = _pair(prob)
-    with dev:
+    with dev, Solidz(prob) as twin:
         dev.step(dt, nsteps=350)
         _sync_to_oracle(orc, dev)
+        twin.set_store(True)
+        for f in ("u", "v", "a", "sdv", "time"):
+            twin.set(f, dev.get(f))
+        twin.set_mode(1)
+        twin.step(dt, nsteps=1)
         dev.set_surrogate(net, 1e-4)
         dev.set_mode(2)
         c0 = dev.counters()
@@ -458,6 +464,7 @@ def test_tube_in_tissue_surrogate_mode_one_step():
         assert l2 < 1e-3
         mat = prob["mat_of_elem"]
         assert np.all(dev.get("sdv")[..., 0][mat == 1] == 0)
+        np.testing.assert_array_equal(Sd[mat == 1], twin.get("stress")[mat == 1])
 
 
 def test_neo_hookean_only_mesh_matches_oracle():
