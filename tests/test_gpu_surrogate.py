@@ -203,11 +203,11 @@ def test_surrogate_in_step_gain():
             acc[name][0] += int(c1["surrogate_used"] - c0["surrogate_used"])
             acc[name][1] += seen
             xb = d.get("sdv")[..., 0]
-            # the surrogate modes form the FP32 strain spectrally (their
-            # features are principal strains), the FP32 mode by series: the
-            # two differ by FP32 rounding (1e-5 in xS bounds it here)
-            assert np.all(np.abs(xb[used] - xa[used]) <= tol + 1e-5)
-            assert np.all(np.abs(xb[~used] - xa[~used]) <= 1e-5)      # exact map elsewhere
+            # the surrogate and FP32 modes form the same FP32 strain: accepted
+            # points lie within tol of the exact map's xS, every other point
+            # is the FP32 mode's bitwise
+            assert np.all(np.abs(xb[used] - xa[used]) <= tol + 1e-9)
+            np.testing.assert_array_equal(xb[~used], xa[~used])
             assert rel(d.get("stress"), SR) < 1e-3
     ta = A.time_kernels(dt, nsteps=2)["kernel_ms_per_step"]
     tb = B["trained"].time_kernels(dt, nsteps=2)["kernel_ms_per_step"]
@@ -220,3 +220,59 @@ def test_surrogate_in_step_gain():
         d.close()
     untrained.close()
     trained.close()
+
+
+def test_surrogate_run_against_fp32_run():
+    """Runs of 600 captured surrogate steps (each step's prepass visits only
+    the elements that transformed in the step before) from one FP64 state
+    in the bar's loading phase, against the FP32 mode's run (the exact map
+    everywhere).  With a tolerance that accepts nothing the run is bitwise
+    the FP32 run, so the list, the candidate elements and the in-kernel
+    check leave the exact path untouched; with tol = 1e-5 the network is
+    used and the run stays within 1e-4 of it.  (With 1e-4 the accepted
+    answers, each within 1e-4 in xS, move the run onto another branch of
+    the bar's transformation plateau: printed, not asserted.)"""
+    from alya_sma_amd import problems, surrogate as S
+    from alya_sma_amd.solid import Solidz
+    net = S.load_default()
+    assert net is not None, "trained weights missing: python -m alya_sma_amd.surrogate"
+    prob = problems.bar_problem()
+    dt = prob["run"]["dt"]
+    with Solidz(prob) as ref:
+        ref.step(dt, nsteps=300)
+        st = {f: ref.get(f) for f in ("u", "v", "a", "sdv", "time")}
+
+    def run(mode, tol=None):
+        with Solidz(prob) as d:
+            for f, v in st.items():
+                d.set(f, v)
+            d.set_store(True)
+            if mode == 2:
+                d.set_surrogate(net, tol)
+            d.set_mode(mode)
+            c0 = d.counters()
+            d.step(dt, nsteps=600)
+            c1 = d.counters()
+            return d.get("stress"), d.get("sdv"), {k: c1[k] - c0[k] for k in c1}
+
+    Sa, sa, ca = run(1)
+    transforming = ca["kf_AS"] + ca["kf_SA"]
+    for tol in (1e-8, 1e-5, 1e-4):
+        Sb, sb, cb = run(2, tol)
+        seen = cb["surrogate_used"] + cb["surrogate_fallback"]
+        rel = float(np.linalg.norm(Sb - Sa) / np.linalg.norm(Sa))
+        dx = float(np.max(np.abs(sb[..., 0] - sa[..., 0])))
+        print(f"tol {tol:g}: used {cb['surrogate_used']}, seen {seen}, transforming "
+              f"{cb['kf_AS'] + cb['kf_SA']} (FP32 run {transforming}); stress rel L2 vs the "
+              f"FP32 run {rel:.2e}, max |dxS| {dx:.2e}", flush=True)
+        assert 0 < seen <= cb["kf_AS"] + cb["kf_SA"]
+        if tol == 1e-8:
+            assert cb["surrogate_used"] == 0
+            np.testing.assert_array_equal(Sb, Sa)
+            np.testing.assert_array_equal(sb, sa)
+            # the prepass misses only points whose element had none in the step before
+            assert seen >= 0.99 * transforming
+        elif tol == 1e-5:
+            assert cb["surrogate_used"] > 0
+            assert rel < 1e-4 and dx < 1e-4
+    net.close()
