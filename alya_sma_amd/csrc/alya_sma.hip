@@ -1697,7 +1697,17 @@ struct EvalStage {
   void* dev = nullptr;
   void* host = nullptr;
   size_t bytes = 0;
+  int device = -1;               // the device that holds `dev`
   int reserve(size_t want) {
+    int cur = 0;
+    HIPCHK(hipGetDevice(&cur));
+    if (cur != device) {
+      // another device is current: a block of its own (the other device's
+      // block stays with it until the process ends)
+      dev = nullptr;
+      bytes = 0;
+      device = cur;
+    }
     if (want <= bytes) return SMA_OK;
     size_t nb = bytes ? bytes : 4096;
     while (nb < want) nb *= 2;
