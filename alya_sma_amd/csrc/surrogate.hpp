@@ -334,17 +334,26 @@ __device__ __forceinline__ bf16x8 cat4(bf16x4 a, bf16x4 b) {
   return bf16x8{a[0], a[1], a[2], a[3], b[0], b[1], b[2], b[3]};
 }
 
-// bias + SiLU of one layer's accumulators into the next layer's inputs
-__device__ __forceinline__ void hidden_epilogue(const f32x4 (&acc)[INF_RB][8], const float* bias,
-                                                int q, bf16x4 (&h)[INF_RB][8]) {
+// A layer's accumulators start at its bias (lane (r, q) holds features
+// 16 b + 4 q + i), so the MFMA chain leaves Z = W x + b and the epilogue
+// adds nothing
+__device__ __forceinline__ void bias_init(f32x4 (&acc)[INF_RB][8], const float* bias, int q) {
 #pragma unroll
   for (int b = 0; b < 8; ++b) {
     const f32x4 bc = *reinterpret_cast<const f32x4*>(bias + 16 * b + 4 * q);
 #pragma unroll
+    for (int a = 0; a < INF_RB; ++a) acc[a][b] = bc;
+  }
+}
+
+// SiLU of one layer's Z into the next layer's inputs
+__device__ __forceinline__ void hidden_epilogue(const f32x4 (&acc)[INF_RB][8], bf16x4 (&h)[INF_RB][8]) {
+#pragma unroll
+  for (int b = 0; b < 8; ++b)
+#pragma unroll
     for (int a = 0; a < INF_RB; ++a)
 #pragma unroll
-      for (int i = 0; i < 4; ++i) h[a][b][i] = (__bf16)silu(acc[a][b][i] + bc[i]);
-  }
+      for (int i = 0; i < 4; ++i) h[a][b][i] = (__bf16)silu(acc[a][b][i]);
 }
 
 __global__ void __launch_bounds__(INF_THREADS) mlp_infer(const __bf16* __restrict__ Wi,
@@ -398,10 +407,7 @@ __global__ void __launch_bounds__(INF_THREADS) mlp_infer(const __bf16* __restric
     bf16x4 h[INF_RB][8];
     f32x4 acc[INF_RB][8];
     // layer 0 (K = 32)
-#pragma unroll
-    for (int a = 0; a < INF_RB; ++a)
-#pragma unroll
-      for (int b = 0; b < 8; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+    bias_init(acc, sb + boff(0), q);
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
       const bf16x8 wf = *reinterpret_cast<const bf16x8*>(sW + (16 * b + r) * wld(0) + 8 * q);
@@ -409,15 +415,12 @@ __global__ void __launch_bounds__(INF_THREADS) mlp_infer(const __bf16* __restric
       for (int a = 0; a < INF_RB; ++a)
         acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf, x0[a], acc[a][b], 0, 0, 0);
     }
-    hidden_epilogue(acc, sb + boff(0), q, h);
+    hidden_epilogue(acc, h);
     // hidden layers 1 .. NLAYER - 2 (K = 128)
 #pragma unroll 1
     for (int l = 1; l < NLAYER - 1; ++l) {
       const __bf16* W = sW + woff_lds(1) + (l - 1) * NH * wld(1);
-#pragma unroll
-      for (int a = 0; a < INF_RB; ++a)
-#pragma unroll
-        for (int b = 0; b < 8; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+      bias_init(acc, sb + boff(l), q);
 #pragma unroll
       for (int c = 0; c < NH / 32; ++c) {
         bf16x8 xf[INF_RB];
@@ -432,15 +435,18 @@ __global__ void __launch_bounds__(INF_THREADS) mlp_infer(const __bf16* __restric
             acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf, xf[a], acc[a][b], 0, 0, 0);
         }
       }
-      hidden_epilogue(acc, sb + boff(l), q, h);
+      hidden_epilogue(acc, h);
     }
     // output layer: one 16-feature block, Y = Z (fp32)
     {
       constexpr int l = NLAYER - 1;
       const __bf16* W = sW + woff_lds(l);
+      // features 4 q .. 4 q + 3 of the output block, from its bias (zero
+      // beyond NOUT)
+      const f32x4 bo = *reinterpret_cast<const f32x4*>(sb + boff(l) + 4 * q);
       f32x4 o[INF_RB];
 #pragma unroll
-      for (int a = 0; a < INF_RB; ++a) o[a] = f32x4{0.f, 0.f, 0.f, 0.f};
+      for (int a = 0; a < INF_RB; ++a) o[a] = bo;
 #pragma unroll
       for (int c = 0; c < NH / 32; ++c) {
         const bf16x8 wf = *reinterpret_cast<const bf16x8*>(W + r * wld(l) + 32 * c + 8 * q);
@@ -449,7 +455,6 @@ __global__ void __launch_bounds__(INF_THREADS) mlp_infer(const __bf16* __restric
           o[a] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf, cat4(h[a][2 * c], h[a][2 * c + 1]),
                                                          o[a], 0, 0, 0);
       }
-      const float* bias = sb + boff(l);
 #pragma unroll
       for (int a = 0; a < INF_RB; ++a) {
         const int64_t m = row0 + 16 * a + r;
@@ -457,7 +462,7 @@ __global__ void __launch_bounds__(INF_THREADS) mlp_infer(const __bf16* __restric
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           const int col = 4 * q + i;
-          if (m < n && col < NOUT) Y[sl * NOUT + col] = o[a][i] + bias[col];
+          if (m < n && col < NOUT) Y[sl * NOUT + col] = o[a][i];
         }
       }
     }
