@@ -357,6 +357,44 @@ def test_batched_uneven_specimens_match_oracle():
         assert all(np.any(xs[mat == s] > 0) for s in range(6))   # every specimen transformed
 
 
+def test_batched_specimens_surrogate_mode_one_step():
+    """Surrogate mode on batched specimens, each with its own material row
+    (the network's features carry each element's E and material): one
+    surrogate step from an FP64 state in the loading phase stays within
+    1e-3 (relative L2 stress) of the oracle's FP64 step, with points listed
+    for the network and every accepted point's history within the
+    acceptance tolerance of the oracle's."""
+    from alya_sma_amd import surrogate as S
+    net = S.load_default()
+    assert net is not None, "the committed network is missing"
+    prob = problems.batched_specimens(n=6, median=80, lo=30, hi=300, nsteps=400)
+    dt = prob["run"]["dt"]
+    print("batched specimens: one surrogate step after 150 FP64 steps", flush=True)
+    dev, orc = _pair(prob)
+    with dev:
+        dev.step(dt, nsteps=150)
+        _sync_to_oracle(orc, dev)
+        dev.set_surrogate(net, 1e-4)
+        dev.set_mode(2)
+        c0 = dev.counters()
+        dev.step(dt, nsteps=1)
+        orc.step(dt, n=1)
+        c1 = dev.counters()
+        Sd = dev.get("stress")
+        l2 = np.linalg.norm(Sd - orc.S) / np.linalg.norm(orc.S)
+        used = c1["surrogate_used"] - c0["surrogate_used"]
+        seen = used + c1["surrogate_fallback"] - c0["surrogate_fallback"]
+        fl = dev.get("flags")
+        acc = ((fl & 0x20) > 0) & ((fl & 3) == (orc.flags & 3))
+        dxs = np.abs(dev.get("sdv")[..., 0] - orc.sdv[0, ..., 0])
+        print(f"  stress rel L2 {l2:.2e}, transforming {int(np.count_nonzero(orc.flags & 3))}, "
+              f"seen {seen}, accepted {used}", flush=True)
+        assert np.count_nonzero(orc.flags & 3) > 0 and seen > 0
+        assert l2 < 1e-3
+        if acc.any():
+            assert dxs[acc].max() <= 1e-4 + 1e-6
+
+
 def test_batched_specimens_persistent_launch_bitwise():
     prob = problems.batched_specimens(n=12, median=200, lo=50, hi=900, nsteps=200)
     dt = prob["run"]["dt"]
