@@ -269,3 +269,43 @@ def test_body_force_matches_oracle(scheme):
         R = dev.get("reactions")
         fixed = np.stack([(prob["fixmask"] >> k) & 1 for k in range(3)], axis=1).astype(bool)
         assert _rel(R[fixed], orc.rhs[fixed]) < TOL
+
+
+def test_damped_cantilever_surrogate_mode_one_step():
+    """Surrogate mode with Rayleigh damping (the damping kernels carry the
+    surrogate's listed points and in-kernel check): one step from an FP64
+    state in the transforming phase against the oracle's FP64 step, stress
+    within 1e-3 and acceleration (damping force included) within 1e-3
+    (relative L2), with transforming points listed for the network."""
+    from alya_sma_amd import surrogate as S
+    net = S.load_default()
+    assert net is not None, "the committed network is missing"
+    prob = _cantilever(alpha=1.0e5, nsteps=2000)
+    r = prob["run"]
+    dev, orc = _pair(prob)
+    with dev:
+        dev.step(r["dt"], nsteps=1000)
+        orc.u[:] = dev.get("u")
+        orc.v[:] = dev.get("v")
+        orc.a[:] = dev.get("a")
+        sdv = dev.get("sdv")
+        orc.sdv[0] = sdv
+        orc.sdv[1] = sdv
+        orc.s.t = float(dev.get("time")[0])
+        dev.set_surrogate(net, 1e-4)
+        dev.set_mode(2)
+        c0 = dev.counters()
+        dev.step(r["dt"], nsteps=1)
+        orc.step(r["dt"], n=1)
+        c1 = dev.counters()
+        eS = np.linalg.norm(dev.get("stress") - orc.S) / np.linalg.norm(orc.S)
+        ea = np.linalg.norm(dev.get("a") - orc.a) / np.linalg.norm(orc.a)
+        seen = (c1["surrogate_used"] - c0["surrogate_used"]) + \
+            (c1["surrogate_fallback"] - c0["surrogate_fallback"])
+        tr = int(np.count_nonzero(orc.flags & 3))
+        print(f"damped surrogate step: stress {eS:.1e}, a {ea:.1e}, transforming {tr}, "
+              f"seen {seen}, accepted {c1['surrogate_used'] - c0['surrogate_used']}", flush=True)
+        assert tr > 0 and seen > 0
+        assert eS < 1e-3 and ea < 1e-3
+    net.close()
+
