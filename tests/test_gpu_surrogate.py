@@ -279,3 +279,42 @@ def test_surrogate_run_against_fp32_run():
         elif tol == 1e-5:
             assert rel < 1e-3 and dx < 1e-2
     net.close()
+
+
+def test_surrogate_mode_scatter_equals_gather():
+    """Surrogate mode in the coloured-scatter assembly (one launch per
+    colour, each writing its elements' candidate flags) against the gather
+    assembly: 1 and then 64 captured steps from one state in the bar's
+    loading phase give bitwise the same state."""
+    from alya_sma_amd import problems, surrogate as S
+    from alya_sma_amd.solid import Solidz
+    net = S.load_default()
+    assert net is not None, "trained weights missing: python -m alya_sma_amd.surrogate"
+    prob = problems.bar_problem()
+    dt = prob["run"]["dt"]
+    with Solidz(prob) as ref:
+        ref.step(dt, nsteps=350)
+        st = {f: ref.get(f) for f in ("u", "v", "a", "sdv", "time")}
+    out = {}
+    for asm in ("gather", "scatter"):
+        with Solidz(prob) as d:
+            d.set_assembly(asm)
+            for f, v in st.items():
+                d.set(f, v)
+            d.set_store(True)
+            d.set_surrogate(net, 1e-4)
+            d.set_mode(2)
+            c0 = d.counters()
+            d.step(dt, nsteps=1)
+            s1 = d.get("stress")
+            d.step(dt, nsteps=64)
+            c1 = d.counters()
+            out[asm] = (s1, d.get("stress"), d.get("u"), d.get("sdv"),
+                        c1["surrogate_used"] - c0["surrogate_used"])
+    g, sc = out["gather"], out["scatter"]
+    print(f"accepted over 65 steps: gather {g[4]}, scatter {sc[4]}", flush=True)
+    assert g[4] > 0 and g[4] == sc[4]
+    for a, b in zip(g[:4], sc[:4]):
+        np.testing.assert_array_equal(a, b)
+    net.close()
+
