@@ -734,7 +734,6 @@ struct ElemArgs {
 // sums the slots.
 constexpr int COUNTER_SLOTS = 256;
 
-// Wave-level flag counters: one 64-bit atomic per wave and per set bit type.
 // Surrogate mode, one Gauss point: a listed point takes the network's xS
 // when it lies within tol of the return map's solution (|R(x)/R'(x)| of the
 // single-branch consistency residual, exact because that residual is linear
@@ -764,6 +763,7 @@ __device__ inline sma::ReturnMap surrogate_or_exact(const ElemArgs& A, const sma
   return rm;
 }
 
+// Wave-level flag counters: one 64-bit atomic per wave and per set bit type.
 __device__ __forceinline__ void count_flags(unsigned long long* ctr, uint32_t fl, bool valid) {
   if (__ballot(valid && fl != 0u) == 0ull) return;   // wave-uniform: nothing to count
   ctr += 8 * (blockIdx.x % COUNTER_SLOTS);
