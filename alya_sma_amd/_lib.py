@@ -73,6 +73,7 @@ def lib():
     L.sld_dev_set_mode.argtypes = [p, i32]
     L.sld_dev_set_store.argtypes = [p, i32]
     L.sld_dev_step.argtypes = [p, d, i32, d, i64]
+    L.sld_dev_prepare.argtypes = [p, d, i32, d, i64]
     L.sld_dev_time_kernels.argtypes = [p, d, i32, d, i32, p]
     L.sld_dev_critical_dt.argtypes = [p]
     L.sld_dev_critical_dt.restype = d

@@ -1195,7 +1195,31 @@ int sld_dev_energy(sld_dev* d, double* out) {
   return SMA_OK;
 }
 
-int sld_dev_step(sld_dev* d, double dt, int32_t scheme, double phi, int64_t nsteps) {
+// The captured run of `len` steps from the current history slot with the
+// handle's current settings, built on first use and kept.
+static int run_graph(sld_dev* d, double dt, int32_t scheme, double phi, int len, hipGraphExec_t* out) {
+  GraphKey key{dt, phi, scheme, d->mode, d->store, d->epb, d->xcd, d->persistent, d->assembly, d->cur,
+               d->energy, len, d->damp, d->body_func};
+  auto it = d->graphs.find(key);
+  if (it == d->graphs.end()) {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    HIPCHK(hipStreamBeginCapture(d->stream, hipStreamCaptureModeThreadLocal));
+    int rc = 0;
+    for (int k = 0; k < len && !rc; ++k)
+      rc = enqueue_step(d, dt, scheme, phi, (k & 1) ? 1 - d->cur : d->cur, nullptr, k == 0,
+                        k + 1 < len);
+    hipError_t ec = hipStreamEndCapture(d->stream, &g);
+    if (rc) return rc;
+    if (ec != hipSuccess) return fail(SMA_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+    HIPCHK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+    it = d->graphs.emplace(key, std::make_pair(g, ge)).first;
+  }
+  *out = it->second.second;
+  return SMA_OK;
+}
+
+static int step_checks(sld_dev* d, int32_t scheme, double dt, int64_t nsteps) {
   if (!d) return fail(SMA_ERR_ARG, "null handle");
   if (scheme != 0 && scheme != 1) return fail(SMA_ERR_ARG, "scheme must be 0 (CD) or 1 (TW)");
   if (!(dt > 0.0)) return fail(SMA_ERR_ARG, "dt must be positive");
@@ -1203,30 +1227,32 @@ int sld_dev_step(sld_dev* d, double dt, int32_t scheme, double phi, int64_t nste
   if (d->mode == SLD_MODE_SURROGATE && !surrogate_attached(d))
     return fail(SMA_ERR_STATE, "surrogate mode without a surrogate");
   if (d->halted) return fail(SMA_ERR_STATE, d->halt_msg);
+  return SMA_OK;
+}
+
+int sld_dev_prepare(sld_dev* d, double dt, int32_t scheme, double phi, int64_t nsteps) {
+  if (int rc = step_checks(d, scheme, dt, nsteps)) return rc;
+  int64_t left = nsteps;
+  for (int len : GRAPH_RUNS) {
+    if (!d->use_graph || left < len) continue;
+    hipGraphExec_t ge = nullptr;
+    if (int rc = run_graph(d, dt, scheme, phi, len, &ge)) return rc;
+    left %= len;
+  }
+  return SMA_OK;
+}
+
+int sld_dev_step(sld_dev* d, double dt, int32_t scheme, double phi, int64_t nsteps) {
+  if (int rc = step_checks(d, scheme, dt, nsteps)) return rc;
   int64_t left = nsteps;
   // runs of 64, 16 and 2 steps replay captured graphs (an even length
   // returns the history and clock slots to where they started); inside a run
   // each corrector also predicts the next step
   for (int len : GRAPH_RUNS) {
     if (!d->use_graph || left < len) continue;
-    GraphKey key{dt, phi, scheme, d->mode, d->store, d->epb, d->xcd, d->persistent, d->assembly, d->cur,
-                 d->energy, len, d->damp, d->body_func};
-    auto it = d->graphs.find(key);
-    if (it == d->graphs.end()) {
-      hipGraph_t g = nullptr;
-      hipGraphExec_t ge = nullptr;
-      HIPCHK(hipStreamBeginCapture(d->stream, hipStreamCaptureModeThreadLocal));
-      int rc = 0;
-      for (int k = 0; k < len && !rc; ++k)
-        rc = enqueue_step(d, dt, scheme, phi, (k & 1) ? 1 - d->cur : d->cur, nullptr, k == 0,
-                          k + 1 < len);
-      hipError_t ec = hipStreamEndCapture(d->stream, &g);
-      if (rc) return rc;
-      if (ec != hipSuccess) return fail(SMA_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
-      HIPCHK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
-      it = d->graphs.emplace(key, std::make_pair(g, ge)).first;
-    }
-    for (; left >= len; left -= len) HIPCHK(hipGraphLaunch(it->second.second, d->stream));
+    hipGraphExec_t ge = nullptr;
+    if (int rc = run_graph(d, dt, scheme, phi, len, &ge)) return rc;
+    for (; left >= len; left -= len) HIPCHK(hipGraphLaunch(ge, d->stream));
   }
   for (; left > 0; --left) {
     int rc = enqueue_step(d, dt, scheme, phi, d->cur);

@@ -218,6 +218,12 @@ class Solidz:
         check(lib().sld_dev_step(self.h, float(dt), int(scheme), float(phi), int(nsteps)),
               "sld_dev_step")
 
+    def prepare(self, dt, nsteps=1, scheme=0, phi=1.033):
+        """Build the captured graphs step(dt, nsteps, ...) would replay from
+        the current state; runs no step."""
+        check(lib().sld_dev_prepare(self.h, float(dt), int(scheme), float(phi), int(nsteps)),
+              "sld_dev_prepare")
+
     def sync(self):
         check(lib().sld_dev_sync(self.h), "sld_dev_sync")
 

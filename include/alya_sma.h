@@ -160,6 +160,14 @@ int sld_dev_set_store(sld_dev *dev, int32_t store);
  * SLD_FIELD_HEALTH); sld_dev_set of the state clears the halt. */
 int sld_dev_step(sld_dev *dev, double dt, int32_t scheme, double phi, int64_t nsteps);
 
+/* Builds (captures and instantiates) the graphs that sld_dev_step with the
+ * same arguments would replay from the current state and settings, and runs
+ * no step: the state is unchanged.  No counterpart in the reference (whose
+ * steps are plain calls); for callers that time or latency-bound a run of
+ * steps and want its one-time graph set-up out of it.  sld_dev_step builds
+ * whatever is missing itself, so the call is optional. */
+int sld_dev_prepare(sld_dev *dev, double dt, int32_t scheme, double phi, int64_t nsteps);
+
 /* Runs nsteps steps eagerly with HIP events around the predictor, the
  * element phase and the corrector on the handle's stream (the reference
  * brackets the same phases with its times(3..7) timers,

@@ -357,6 +357,27 @@ def test_batched_uneven_specimens_match_oracle():
         assert all(np.any(xs[mat == s] > 0) for s in range(6))   # every specimen transformed
 
 
+def test_prepare_builds_graphs_and_runs_no_step():
+    """sld_dev_prepare builds the captured runs a step call would replay and
+    leaves the state as it was; the prepared call then gives bitwise the
+    state of an unprepared one (83 steps: runs of 64, 16 and 2, one eager)."""
+    prob = problems.bar_problem()
+    dt = prob["run"]["dt"]
+    print("prepare vs plain step, bar, 83 steps", flush=True)
+    with Solidz(prob) as a, Solidz(prob) as b:
+        for d in (a, b):
+            d.step(dt, nsteps=301)
+        before = {f: a.get(f) for f in ("u", "v", "a", "sdv", "time")}
+        a.prepare(dt, nsteps=83)
+        a.prepare(dt, nsteps=0)
+        for f, v in before.items():
+            np.testing.assert_array_equal(a.get(f), v)
+        a.step(dt, nsteps=83)
+        b.step(dt, nsteps=83)
+        for f in ("u", "v", "a", "sdv", "time"):
+            np.testing.assert_array_equal(a.get(f), b.get(f))
+
+
 def test_batched_specimens_surrogate_mode_one_step():
     """Surrogate mode on batched specimens, each with its own material row
     (the network's features carry each element's E and material): one
