@@ -498,6 +498,9 @@ def main():
                              f"{r64['transforming_fraction']:.3f} of the point updates transforming",
                    "dt": dt, "gauss_points": ngp, "nodes": int(prob["coord"].shape[0])},
         "modes": results,
+        # throughput of the FP32 + surrogate step over the FP64 step's
+        "surrogate_over_fp64": (results["surrogate"]["speedup_vs_fp64"]
+                                if "surrogate" in results else None),
         "fp64_self_sensitivity_trajectory_end": sens,
         "roofline": r64["roofline"], "roofline_fp64_pipe": r64["roofline_fp64_pipe"],
         "cpu_baseline": cpu,
